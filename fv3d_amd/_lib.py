@@ -1,0 +1,124 @@
+"""Loader of libfv3d_hip.so with the C-ABI signatures of include/*.h."""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+c_int, c_double, c_char_p, c_void_p = ctypes.c_int, ctypes.c_double, ctypes.c_char_p, ctypes.c_void_p
+P_DOUBLE = ctypes.POINTER(ctypes.c_double)
+P_INT = ctypes.POINTER(ctypes.c_int)
+
+EXACT_FT = ctypes.CFUNCTYPE(None, c_int, c_double, P_DOUBLE, P_DOUBLE)
+VOID_T_FT = ctypes.CFUNCTYPE(None, c_double)
+DOUBLE_T_FT = ctypes.CFUNCTYPE(c_double, c_double)
+
+# (name, restype, argtypes) of every function declared in include/*.h
+SIGNATURES = [
+    # fv3d_solver.h — Solver API (reference names)
+    ("add_sol_variable", c_int, [c_char_p]),
+    ("add_dep_variable", c_int, [c_char_p]),
+    ("calc_global_residual", None, [c_double]),
+    ("create_file_header", None, [c_char_p]),
+    ("finite_volume_time_derivative", None, [c_double]),
+    ("free_analyze", None, []),
+    ("free_equation", None, []),
+    ("free_explicit", None, []),
+    ("free_finite_volume", None, []),
+    ("free_implicit", None, []),
+    ("free_limiter", None, []),
+    ("free_mesh", None, []),
+    ("free_reconstruction", None, []),
+    ("free_output", None, []),
+    ("free_restart", None, []),
+    ("free_solver", None, []),
+    ("free_timedisc", None, []),
+    ("get_simulation_title", c_char_p, []),
+    ("init_analyze", None, []),
+    ("init_equation", None, []),
+    ("init_explicit", None, [c_int]),
+    ("init_finite_volume", None, []),
+    ("init_implicit", None, [c_int, c_int, c_int, c_double, c_int, c_int, c_int]),
+    ("init_limiter", None, [c_int]),
+    ("init_mesh", None, [c_char_p]),
+    ("init_reconstruction", None, [c_int]),
+    ("init_output", None, [c_int]),
+    ("init_restart", None, [c_int, c_char_p]),
+    ("init_solver", None, [c_char_p]),
+    ("init_timedisc", None, [c_int, c_int, c_int, c_double, c_double, c_double]),
+    ("is_explicit", c_int, []),
+    ("print_mesh_info", None, []),
+    ("print_variables", None, []),
+    ("read_restart_data", None, [c_char_p]),
+    ("set_calc_flux", None, [c_void_p]),
+    ("set_calc_timestep", None, [c_void_p]),
+    ("set_exact_function", None, [c_void_p]),
+    ("set_update", None, [c_void_p]),
+    ("set_update_gradients", None, [c_void_p]),
+    ("set_viscous_dt", None, [c_int]),
+    ("timedisc", None, []),
+    ("write_output", None, [c_int, c_double]),
+    ("navier_stokes_initialize", None, []),
+    ("boundary_set_region", None, [c_int, c_int, P_DOUBLE]),
+    ("flux_initialize_scheme", None, [c_char_p]),
+    ("calc_ns_flux", None, [c_double]),
+    ("calc_ns_timestep", c_double, [c_double]),
+    ("calc_ns_exact_function", None, [c_int, c_double, P_DOUBLE, P_DOUBLE]),
+    ("ns_update", None, [c_double]),
+    ("ns_update_gradients", None, [c_double]),
+    # fv3d_hip.h — extension API
+    ("fvh_last_error", c_char_p, []),
+    ("fvh_mesh_box", c_int, [c_int, c_int, c_int, c_double, c_double, c_double,
+                             c_double, c_double, c_double, c_int, c_int, c_int]),
+    ("fvh_mesh_channel", c_int, [c_int, c_int, c_int, c_double, ctypes.c_ulonglong]),
+    ("fvh_mesh_write", c_int, [c_char_p]),
+    ("fvh_get_face_shift", c_int, [P_DOUBLE]),
+    ("fvh_set_visc_dt_mode", c_int, [c_int]),
+    ("fvh_set_ic", c_int, [c_int, P_DOUBLE, c_int]),
+    ("fvh_exact_function_builtin", None, [c_int, c_double, P_DOUBLE, P_DOUBLE]),
+    ("fvh_get_state", c_int, [P_DOUBLE]),
+    ("fvh_set_state", c_int, [P_DOUBLE]),
+    ("fvh_get_phi_dt", c_int, [P_DOUBLE]),
+    ("fvh_get_field", c_int, [c_int, P_DOUBLE]),
+    ("fvh_sync_host", c_int, []),
+    ("fvh_set_verbose", c_int, [c_int]),
+    ("fvh_set_sync_interval", c_int, [c_int]),
+    ("fvh_timedisc", c_int, []),
+    ("fvh_get_log", c_int, [P_DOUBLE, c_int]),
+    ("fvh_last_status", c_int, [P_INT]),
+    ("fvh_bench_explicit", c_int, [c_int, c_int, P_DOUBLE, P_DOUBLE]),
+    ("fvh_device_info", c_int, [c_char_p, c_int, P_INT, P_DOUBLE]),
+]
+
+# exported data symbols (globals) of fv3d_solver.h
+DATA_SYMBOLS = [
+    "solver_mesh", "solver_variables", "solver_data",
+    "cfl_scale", "dfl_scale", "mu_mix", "R_mix", "Pr", "kappa",
+    "kappa_m1", "kappa_p1", "s_kappa", "s_kappa_m1", "s_kappa_p1", "cv", "cp",
+    "kappa_pr", "lambda",
+    "ic_rho", "ic_rho_u", "ic_rho_v", "ic_rho_w", "ic_rho_e",
+    "ip_u", "ip_v", "ip_w", "ip_p", "ip_T",
+]
+
+
+def lib_path():
+    return os.path.join(_HERE, "libfv3d_hip.so")
+
+
+def lib():
+    """Load libfv3d_hip.so (once).  Raises if the library was not built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = lib_path()
+    if not os.path.exists(path):
+        raise RuntimeError(
+            "libfv3d_hip.so is not built (%s); run python -c "
+            "'import __graft_entry__; __graft_entry__.build()'" % path)
+    L = ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+    for name, res, args in SIGNATURES:
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _LIB = L
+    return L

@@ -1,0 +1,1774 @@
+// Host runtime of libfv3d_hip.so: the reference's Solver API
+// (src/Solver/include/solver/solver_module.h) over device-resident SoA
+// buffers, plus the Navier-Stokes plug-in built-ins
+// (src/FV3D/src/navier_stokes/) and the fvh_* extension API.
+//
+// Ownership follows the reference: init_* allocate, free_* release; the
+// library owns every device buffer (allocated in init_finite_volume, freed
+// in free_finite_volume).  Single host thread, one HIP stream; not
+// thread-safe, like the reference.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "fv_device.h"
+#include "fvh_internal.h"
+#include "raw_container.h"
+
+// ---------------------------------------------------------------------------
+// reference globals (solver_module.h:16-18, navier_stokes.c:16-55)
+// ---------------------------------------------------------------------------
+extern "C" {
+Mesh_t *solver_mesh = nullptr;
+Variables_t *solver_variables = nullptr;
+data_t *solver_data = nullptr;
+
+double cfl_scale = 1.00;
+double dfl_scale = 1.00;
+double mu_mix = 18.13e-6;
+double R_mix = 8.31446261815324 / 28.96e-3;
+double Pr = 0.718;
+double kappa = 1.4;
+double kappa_m1, kappa_p1, s_kappa, s_kappa_m1, s_kappa_p1, cv, cp, kappa_pr,
+    lambda;
+int ic_rho = -1, ic_rho_u = -1, ic_rho_v = -1, ic_rho_w = -1, ic_rho_e = -1;
+int ip_u = -1, ip_v = -1, ip_w = -1, ip_p = -1, ip_T = -1;
+}
+
+namespace fvh {
+
+namespace {
+
+std::string g_error;
+
+#define HIP_CHECK(expr)                                                        \
+    do                                                                         \
+    {                                                                          \
+        hipError_t e_ = (expr);                                                \
+        if (e_ != hipSuccess)                                                  \
+            throw std::runtime_error(std::string("HIP error: ") +             \
+                                     hipGetErrorString(e_) + " at " +          \
+                                     __FILE__ + ":" + std::to_string(__LINE__)); \
+    } while (0)
+
+// ---- solver-wide settings (the reference's file-scope statics) ------------
+std::string g_title = "fv3d";
+int g_i_output_data = -1;
+bool g_do_output = false;
+bool g_use_restart = false;
+std::string g_restart_file;
+int g_iter_restart = 0;
+double g_t_restart = 0.0;
+
+bool g_is_explicit = true;
+int g_max_iter = 1000;
+bool g_transient = true;
+double g_abort_residual = 1e-10;
+double g_t_start = 0.0;
+double g_t_end = 0.2;
+bool g_viscous_dt = false;
+
+int g_rk_stages = 0;
+const double *g_rk_a = nullptr, *g_rk_b = nullptr, *g_rk_g = nullptr;
+
+int g_reconstruction = -1; // FirstOrder / Linear
+int g_limiter = -1;        // BarthJespersenn / NoLimter
+int g_flux = dev::FluxAUSM;
+int g_visc_dt_mode = 0;
+bool g_analyze_ready = false;
+std::vector<double> g_residual(5, 0.0);
+
+// implicit settings (implicit.c:73-77 defaults)
+int g_bdf_scheme = 0;
+int g_implicit_solver = 1;
+int g_max_iter_inner = 100;
+double g_tolerance_lsoe = 1e-12;
+int g_max_iter_lsoe = 100;
+int g_max_krylov_dims = 15;
+int g_max_krylov_restarts = 2;
+
+// hooks (finite_volume.c:13-16, timedisc.c:15-16)
+void_calc_flux_ft g_calc_flux = nullptr;
+void_calc_exact_ft g_calc_exact = nullptr;
+void_update_ft g_update = nullptr;
+void_update_gradients_ft g_update_gradients = nullptr;
+double_calc_timestep_ft g_calc_timestep = nullptr;
+
+// built-in initial conditions
+int g_ic_kind = 2;
+double g_ic_params[8] = {0};
+
+int g_verbose = 1;
+int g_sync_interval = 16;
+int g_last_status = 0;
+int g_last_iterations = 0;
+std::vector<double> g_log;
+
+std::vector<double> g_face_shift;
+
+// LSERK tables, explicit.c:19-39
+const double rk_a_euler[1] = {0.0};
+const double rk_b_euler[1] = {0.0};
+const double rk_g_euler[1] = {1.0};
+const double rk_a_rk33[3] = {0., -5. / 9., -153. / 128.};
+const double rk_b_rk33[3] = {0., 1. / 3., 3. / 4.};
+const double rk_g_rk33[3] = {1. / 3., 15. / 16., 8. / 15.};
+const double rk_a_rk45[5] = {0., -567301805773.0 / 1357537059087.0,
+                             -2404267990393.0 / 2016746695238.0,
+                             -3550918686646.0 / 2091501179385.0,
+                             -1275806237668.0 / 842570457699.0};
+const double rk_b_rk45[5] = {0., 1432997174477.0 / 9575080441755.0,
+                             2526269341429.0 / 6820363962896.0,
+                             2006345519317.0 / 3224310063776.0,
+                             2802321613138.0 / 2924317926251.0};
+const double rk_g_rk45[5] = {1432997174477.0 / 9575080441755.0,
+                             5161836677717.0 / 13612068292357.0,
+                             1720146321549.0 / 2090206949498.0,
+                             3134564353537.0 / 4481467310338.0,
+                             2277821191437.0 / 14882151754819.0};
+
+// ---- device state ----------------------------------------------------------
+struct Device
+{
+    bool ready = false;
+    hipStream_t stream = nullptr;
+    std::vector<void *> allocs;
+    dev::DevMesh m{};
+    double *U[2] = {nullptr, nullptr};
+    int cur = 0;
+    double *k = nullptr;
+    double *R = nullptr;
+    double *grad = nullptr;
+    double *mnmx = nullptr;
+    double *partials = nullptr;
+    dev::DevScalars *sc = nullptr;
+    dev::DevScalars *sc_host = nullptr; // pinned
+    double *log = nullptr;
+    int log_cap = 0;
+    double *mirror_phi = nullptr; // AoS staging for the host mirror
+    double *mirror_dt = nullptr;
+    double vglob = 0.0;
+    bool r_valid = false;
+};
+Device g_dev;
+
+template <class T> T *dmalloc(size_t n)
+{
+    void *p = nullptr;
+    if (n == 0)
+        n = 1;
+    HIP_CHECK(hipMalloc(&p, n * sizeof(T)));
+    g_dev.allocs.push_back(p);
+    return (T *)p;
+}
+
+template <class T> T *dupload(const T *h, size_t n)
+{
+    T *d = dmalloc<T>(n);
+    if (n)
+        HIP_CHECK(hipMemcpy(d, h, n * sizeof(T), hipMemcpyHostToDevice));
+    return d;
+}
+
+void free_device()
+{
+    if (g_dev.stream)
+        (void)hipStreamSynchronize(g_dev.stream);
+    for (void *p : g_dev.allocs)
+        (void)hipFree(p);
+    if (g_dev.sc_host)
+        (void)hipHostFree(g_dev.sc_host);
+    if (g_dev.stream)
+        (void)hipStreamDestroy(g_dev.stream);
+    g_dev = Device();
+}
+
+char *xstrdup(const char *s)
+{
+    size_t n = std::strlen(s) + 1;
+    char *p = (char *)std::malloc(n);
+    std::memcpy(p, s, n);
+    return p;
+}
+
+template <class T> T *xalloc(size_t n)
+{
+    T *p = (T *)std::calloc(n ? n : 1, sizeof(T));
+    if (!p)
+        fatal("host allocation failed");
+    return p;
+}
+
+template <class T> T *xcopy(const std::vector<T> &v)
+{
+    T *p = xalloc<T>(v.size());
+    if (!v.empty())
+        std::memcpy(p, v.data(), v.size() * sizeof(T));
+    return p;
+}
+
+bool builtin_hooks()
+{
+    return g_calc_flux == calc_ns_flux && g_update == ns_update &&
+           g_update_gradients == ns_update_gradients;
+}
+
+dev::PhysDev phys()
+{
+    dev::PhysDev P{};
+    P.kappa = kappa;
+    P.kappa_m1 = kappa_m1;
+    P.s_kappa_m1 = s_kappa_m1;
+    P.s_kappa_p1 = s_kappa_p1;
+    P.R = R_mix;
+    P.mu = mu_mix;
+    P.lam = lambda;
+    P.cfl = cfl_scale;
+    P.dfl = dfl_scale;
+    P.kappa_pr = kappa_pr;
+    P.visc_dt_mode = g_visc_dt_mode;
+    P.bj = (g_reconstruction == Linear && g_limiter == BarthJespersenn) ? 1 : 0;
+    return P;
+}
+
+bool is_linear() { return g_reconstruction == Linear; }
+bool is_viscous() { return mu_mix != 0.0; }
+bool needs_gradients() { return is_linear() || is_viscous(); }
+
+void require_ready()
+{
+    if (!g_dev.ready)
+        throw std::runtime_error("finite volume data not initialised "
+                                 "(init_finite_volume)");
+    if (g_reconstruction < 0 || g_limiter < 0)
+        throw std::runtime_error("reconstruction/limiter not initialised");
+    if (!builtin_hooks())
+        throw std::runtime_error(
+            "only the built-in Navier-Stokes device hooks are supported "
+            "(calc_ns_flux, ns_update, ns_update_gradients)");
+}
+
+void ensure_gradient_buffers()
+{
+    size_t N = (size_t)g_dev.m.N;
+    if (needs_gradients() && !g_dev.grad)
+        g_dev.grad = dmalloc<double>(30 * N);
+    if (is_linear() && g_limiter == BarthJespersenn && !g_dev.mnmx)
+        g_dev.mnmx = dmalloc<double>(20 * N);
+}
+
+// Residual of U_in into R (and, for a stage, into U_out/k).
+void enqueue_residual(const double *U_in, double *U_out, double *R_out,
+                      int stage, bool update, bool reduce, dev::DevScalars *sc)
+{
+    const dev::PhysDev P = phys();
+    ensure_gradient_buffers();
+    if (needs_gradients())
+        dev::launch_gradient(g_dev.stream, g_dev.m, P, U_in, g_dev.grad,
+                             g_dev.mnmx, P.bj, sc);
+    dev::StageParams sp{};
+    sp.U_in = U_in;
+    sp.U_out = U_out;
+    sp.k = g_dev.k;
+    sp.R_out = R_out;
+    sp.grad = g_dev.grad;
+    sp.mnmx = g_dev.mnmx;
+    sp.rk_a = update ? g_rk_a[stage] : 0.0;
+    sp.rk_g = update ? g_rk_g[stage] : 0.0;
+    sp.stage0 = stage == 0;
+    sp.do_update = update;
+    sp.do_reduce = reduce;
+    sp.sc = sc;
+    sp.partials = g_dev.partials;
+    dev::launch_residual(g_dev.stream, g_dev.m, P, sp, g_flux, is_linear(),
+                         is_viscous());
+}
+
+void sync_mirror(bool with_dt)
+{
+    const int N = g_dev.m.N, B = g_dev.m.B;
+    dev::launch_mirror(g_dev.stream, g_dev.m, phys(), g_dev.U[g_dev.cur],
+                       (with_dt && g_dev.r_valid) ? g_dev.R : nullptr,
+                       g_dev.mirror_phi, g_dev.mirror_dt);
+    HIP_CHECK(hipMemcpyAsync(solver_data->phi_total, g_dev.mirror_phi,
+                             (size_t)(N + B) * 10 * sizeof(double),
+                             hipMemcpyDeviceToHost, g_dev.stream));
+    if (with_dt && g_dev.r_valid)
+        HIP_CHECK(hipMemcpyAsync(solver_data->phi_dt, g_dev.mirror_dt,
+                                 (size_t)N * 5 * sizeof(double),
+                                 hipMemcpyDeviceToHost, g_dev.stream));
+    HIP_CHECK(hipStreamSynchronize(g_dev.stream));
+}
+
+void upload_state_from_mirror()
+{
+    const int N = g_dev.m.N;
+    std::vector<double> soa((size_t)5 * N);
+    for (int c = 0; c < N; ++c)
+        for (int j = 0; j < 5; ++j)
+            soa[(size_t)j * N + c] = solver_data->phi_total[(size_t)c * 10 + j];
+    HIP_CHECK(hipMemcpy(g_dev.U[g_dev.cur], soa.data(), soa.size() * 8,
+                        hipMemcpyHostToDevice));
+    g_dev.r_valid = false;
+}
+
+// ---- mesh ------------------------------------------------------------------
+void free_mesh_struct(Mesh_t *m)
+{
+    if (!m)
+        return;
+    if (m->vertices)
+        std::free(m->vertices->x), std::free(m->vertices);
+    if (Cells_t *c = m->cells)
+    {
+        std::free(c->id), std::free(c->type), std::free(c->n_vertices);
+        std::free(c->vertices), std::free(c->n_faces), std::free(c->faces);
+        std::free(c->x), std::free(c->volume), std::free(c->dx);
+        std::free(c->stride), std::free(c);
+    }
+    if (Faces_t *f = m->faces)
+    {
+        std::free(f->type), std::free(f->n_vertices), std::free(f->vertices);
+        std::free(f->cells), std::free(f->boundary), std::free(f->area);
+        std::free(f->lambda), std::free(f->x), std::free(f->n);
+        std::free(f->t1), std::free(f->t2), std::free(f->stride);
+        std::free(f->dist_cell_1), std::free(f->dist_cell_2);
+        std::free(f->internal_faces), std::free(f->boundary_faces);
+        std::free(f);
+    }
+    if (Boundaries_t *b = m->boundaries)
+    {
+        std::free(b->id), std::free(b->type), std::free(b->n_vertices);
+        std::free(b->vertices), std::free(b->face), std::free(b->distance);
+        std::free(b->n), std::free(b->t1), std::free(b->t2);
+        std::free(b->stride), std::free(b);
+    }
+    if (Regions_t *r = m->regions)
+    {
+        for (int i = 0; i < r->n_regions; ++i)
+            std::free(r->name[i]);
+        std::free(r->name), std::free(r->is_boundary);
+        std::free(r->type), std::free(r->function_id), std::free(r->phi_total);
+        std::free(r);
+    }
+    std::free(m->partition);
+    std::free(m);
+}
+
+// mesh.c:19-83 (with the periodic image of cells[1] for dist_cell_2)
+void calc_mesh_metrics(Mesh_t *mesh)
+{
+    Cells_t *cells = mesh->cells;
+    Faces_t *faces = mesh->faces;
+    Regions_t *regions = mesh->regions;
+
+    double v = 0.0;
+    for (int i = 0; i < cells->n_domain_cells; ++i)
+        v += cells->volume[i];
+    mesh->local_volume = v;
+    mesh->global_volume = v;
+
+    faces->n_internal_faces = 0;
+    for (int i = 0; i < faces->n_faces; ++i)
+        if (faces->boundary[i] < 0)
+            faces->n_internal_faces += 1;
+    faces->n_boundary_faces = faces->n_faces - faces->n_internal_faces;
+
+    faces->dist_cell_1 = xalloc<double>((size_t)faces->n_faces * DIM);
+    faces->dist_cell_2 = xalloc<double>((size_t)faces->n_faces * DIM);
+    faces->internal_faces = xalloc<int>(faces->n_internal_faces);
+    faces->boundary_faces = xalloc<int>(faces->n_boundary_faces);
+
+    int k = 0, l = 0;
+    for (int i = 0; i < faces->n_faces; ++i)
+    {
+        int *fc = &faces->cells[i * FACE_CELLS];
+        for (int j = 0; j < DIM; ++j)
+            faces->dist_cell_1[i * DIM + j] =
+                faces->x[i * DIM + j] - cells->x[fc[0] * DIM + j];
+        if (fc[1] >= 0)
+        {
+            for (int j = 0; j < DIM; ++j)
+            {
+                double xc1 = cells->x[fc[1] * DIM + j] +
+                             g_face_shift[(size_t)i * DIM + j];
+                faces->dist_cell_2[i * DIM + j] = faces->x[i * DIM + j] - xc1;
+            }
+            faces->internal_faces[k++] = i;
+        }
+        else
+        {
+            for (int j = 0; j < DIM; ++j)
+                faces->dist_cell_2[i * DIM + j] = 0.0;
+            fc[1] = cells->n_local_cells + faces->boundary[i];
+            faces->boundary_faces[l++] = i;
+        }
+    }
+    regions->flow_region = -1;
+    for (int i = 0; i < regions->n_regions; ++i)
+        if (regions->is_boundary[i] == 0)
+        {
+            regions->flow_region = i;
+            break;
+        }
+}
+
+} // namespace
+
+void set_error(const std::string &msg) { g_error = msg; }
+
+void fatal(const std::string &msg)
+{
+    std::fprintf(stderr, "fv3d_hip: fatal: %s\n", msg.c_str());
+    std::fflush(stderr);
+    std::abort();
+}
+
+const std::vector<double> &face_shift() { return g_face_shift; }
+
+void install_mesh(RawMesh &&m)
+{
+    if (solver_mesh)
+        free_mesh_struct(solver_mesh);
+    Mesh_t *mesh = xalloc<Mesh_t>(1);
+    mesh->dimension = m.dimension;
+    mesh->is_partitioned = 0;
+    mesh->partition = nullptr;
+
+    mesh->vertices = xalloc<Vertices_t>(1);
+    mesh->vertices->n_vertices = (int)(m.vertices.size() / 3);
+    mesh->vertices->x = xcopy(m.vertices);
+
+    const int N = m.n_cells(), F = m.n_faces(), B = m.n_boundaries();
+    Cells_t *c = xalloc<Cells_t>(1);
+    c->n_global_cells = c->n_local_cells = c->n_domain_cells = N;
+    c->max_cell_vertices = m.max_cell_vertices;
+    c->max_cell_faces = m.max_cell_faces;
+    c->id = xcopy(m.cell_region);
+    c->type = xcopy(m.cell_type);
+    c->n_vertices = xcopy(m.cell_nv);
+    c->vertices = xcopy(m.cell_vertices);
+    c->n_faces = xcopy(m.cell_nf);
+    c->faces = xcopy(m.cell_faces);
+    c->x = xcopy(m.cell_x);
+    c->volume = xcopy(m.cell_volume);
+    c->dx = xcopy(m.cell_dx);
+    c->stride = nullptr;
+    mesh->cells = c;
+
+    Faces_t *f = xalloc<Faces_t>(1);
+    f->n_global_faces = f->n_faces = F;
+    f->max_face_vertices = m.max_face_vertices;
+    f->type = xcopy(m.face_type);
+    f->n_vertices = xcopy(m.face_nv);
+    f->vertices = xcopy(m.face_vertices);
+    f->cells = xcopy(m.face_cells);
+    f->boundary = xcopy(m.face_boundary);
+    f->area = xcopy(m.face_area);
+    f->lambda = xcopy(m.face_lambda);
+    f->x = xcopy(m.face_x);
+    f->n = xcopy(m.face_n);
+    f->t1 = xcopy(m.face_t1);
+    f->t2 = xcopy(m.face_t2);
+    f->stride = nullptr;
+    mesh->faces = f;
+
+    Boundaries_t *b = xalloc<Boundaries_t>(1);
+    b->n_global_boundaries = b->n_boundaries = B;
+    b->max_boundary_vertices = m.max_boundary_vertices;
+    b->id = xcopy(m.bnd_region);
+    b->type = xcopy(m.bnd_type);
+    b->n_vertices = xcopy(m.bnd_nv);
+    b->vertices = xcopy(m.bnd_vertices);
+    b->face = xcopy(m.bnd_face);
+    b->distance = xcopy(m.bnd_distance);
+    b->n = xcopy(m.bnd_n);
+    b->t1 = xcopy(m.bnd_t1);
+    b->t2 = xcopy(m.bnd_t2);
+    b->stride = nullptr;
+    mesh->boundaries = b;
+
+    Regions_t *r = xalloc<Regions_t>(1);
+    r->n_regions = (int)m.region_names.size();
+    r->name = xalloc<string_t>(r->n_regions);
+    r->max_name_length = 0;
+    for (int i = 0; i < r->n_regions; ++i)
+    {
+        r->name[i] = xstrdup(m.region_names[i].c_str());
+        r->max_name_length =
+            std::max(r->max_name_length, (int)m.region_names[i].size() + 1);
+    }
+    r->is_boundary = xcopy(m.region_is_boundary);
+    r->type = nullptr;
+    r->function_id = nullptr;
+    r->phi_total = nullptr;
+    mesh->regions = r;
+
+    g_face_shift = std::move(m.face_shift);
+    if (g_face_shift.size() != (size_t)F * 3)
+        g_face_shift.assign((size_t)F * 3, 0.0);
+
+    calc_mesh_metrics(mesh);
+    solver_mesh = mesh;
+}
+
+// ---- built-in IC (fvh_set_ic) ----------------------------------------------
+static void builtin_exact(int id, double t, double *x, double *phi)
+{
+    (void)t;
+    const int nt = solver_variables->n_tot_variables;
+    if (g_ic_kind == 2)
+    {
+        // calc_ns_exact_function, navier_stokes.c:133-152
+        calc_ns_exact_function(id, t, x, phi);
+        return;
+    }
+    for (int j = 0; j < nt; ++j)
+        phi[j] = 0.0;
+    double rho, u, v, w, p;
+    if (g_ic_kind == 0)
+    {
+        const double a = g_ic_params[0], b = g_ic_params[1];
+        rho = a + b * std::sin(2.0 * M_PI * (x[0] + x[1] + x[2]));
+        u = v = w = g_ic_params[2];
+        p = g_ic_params[3];
+    }
+    else
+    {
+        const double p0 = g_ic_params[0], rho0 = g_ic_params[1],
+                     U0 = g_ic_params[2];
+        const double T0 = p0 / (rho0 * R_mix);
+        u = U0 * std::sin(x[0]) * std::cos(x[1]) * std::cos(x[2]);
+        v = -U0 * std::cos(x[0]) * std::sin(x[1]) * std::cos(x[2]);
+        w = 0.0;
+        p = p0 + rho0 * U0 * U0 / 16.0 * (std::cos(2.0 * x[0]) + std::cos(2.0 * x[1])) *
+                     (std::cos(2.0 * x[2]) + 2.0);
+        rho = p / (R_mix * T0);
+    }
+    phi[ic_rho] = rho;
+    phi[ip_u] = u;
+    phi[ip_v] = v;
+    phi[ip_w] = w;
+    phi[ip_p] = p;
+    phi[ip_T] = p / (rho * R_mix);
+    // prim_to_con, eos.c:120-127
+    phi[ic_rho_u] = phi[ip_u] * phi[ic_rho];
+    phi[ic_rho_v] = phi[ip_v] * phi[ic_rho];
+    phi[ic_rho_w] = phi[ip_w] * phi[ic_rho];
+    double d = 0.0;
+    d += phi[ip_u] * phi[ic_rho_u];
+    d += phi[ip_v] * phi[ic_rho_v];
+    d += phi[ip_w] * phi[ic_rho_w];
+    phi[ic_rho_e] = s_kappa_m1 * phi[ip_p] + 0.5 * d;
+}
+
+// ---- time loop ---------------------------------------------------------------
+namespace {
+
+void print_header()
+{
+    if (!g_verbose)
+        return;
+    if (g_is_explicit)
+        std::printf("%9s %12s %12s %1s %1s:", "iter", "time", "dt", "V", "O");
+    else
+        std::printf("%9s %12s %12s %1s %1s %6s %6s:", "iter", "time", "dt",
+                    "V", "O", "inner", "lsoe");
+    for (int i = 0; i < solver_variables->n_sol_variables; ++i)
+        std::printf(" %12s", solver_variables->sol_variables[i].name);
+    std::printf("\n");
+}
+
+void print_row(int iter, const double *row, bool out)
+{
+    if (!g_verbose)
+        return;
+    std::printf("%09d %12.5e %12.5e %c %c:", iter, row[0], row[1],
+                row[2] != 0.0 ? 'T' : 'F', out ? '*' : ' ');
+    for (int j = 0; j < 5; ++j)
+        std::printf(" %12.5e", row[3 + j]);
+    std::printf("\n");
+}
+
+// One explicit step (timestep_lserkw2, explicit.c:96-141) + bookkeeping.
+void enqueue_explicit_step(dev::DevScalars *sc)
+{
+    for (int s = 0; s < g_rk_stages; ++s)
+    {
+        const bool last = s == g_rk_stages - 1;
+        double *Uin = g_dev.U[g_dev.cur];
+        double *Uout = g_dev.U[1 - g_dev.cur];
+        enqueue_residual(Uin, Uout, last ? g_dev.R : nullptr, s, true, last,
+                         sc);
+        g_dev.cur = 1 - g_dev.cur;
+    }
+    dev::launch_finalize(g_dev.stream, sc, g_dev.partials,
+                         dev::cell_blocks(g_dev.m.N), phys(), 1, g_dev.log,
+                         g_dev.log_cap);
+}
+
+void init_scalars(int iter0, double t0, int max_iter, bool transient,
+                  double t_end)
+{
+    dev::DevScalars &h = *g_dev.sc_host;
+    std::memset(&h, 0, sizeof(h));
+    h.t = t0;
+    h.dt = 0.0;
+    h.t_end = transient ? t_end : DBL_MAX;
+    h.abort_residual = g_abort_residual;
+    h.vglob = g_dev.vglob;
+    h.iter = iter0;
+    h.max_iter_abs = iter0 + max_iter;
+    h.transient = transient;
+    h.log_base = iter0;
+    HIP_CHECK(hipMemcpyAsync(g_dev.sc, &h, sizeof(h), hipMemcpyHostToDevice,
+                             g_dev.stream));
+    // initial dt from the current state (primitives refreshed from U)
+    dev::launch_dt_partials(g_dev.stream, g_dev.m, phys(), g_dev.U[g_dev.cur],
+                            g_dev.partials);
+    dev::launch_finalize(g_dev.stream, g_dev.sc, g_dev.partials,
+                         dev::cell_blocks(g_dev.m.N), phys(), 0, nullptr, 0);
+}
+
+void read_scalars()
+{
+    HIP_CHECK(hipMemcpyAsync(g_dev.sc_host, g_dev.sc, sizeof(dev::DevScalars),
+                             hipMemcpyDeviceToHost, g_dev.stream));
+    HIP_CHECK(hipStreamSynchronize(g_dev.stream));
+}
+
+// timedisc.c:134-222, explicit branch, driven from device scalars.
+int run_explicit(int max_iter)
+{
+    require_ready();
+    if (g_rk_stages <= 0)
+        throw std::runtime_error("explicit scheme not initialised");
+    int iter0 = g_use_restart ? g_iter_restart : 0;
+    double t0 = g_use_restart ? g_t_restart : g_t_start;
+
+    print_header();
+    if (iter0 == 0)
+    {
+        enqueue_residual(g_dev.U[g_dev.cur], nullptr, g_dev.R, 0, false, false,
+                         nullptr);
+        g_dev.r_valid = true;
+    }
+    if (g_do_output && !g_use_restart)
+        write_output(iter0, t0);
+
+    if (g_dev.log_cap < max_iter)
+    {
+        g_dev.log_cap = max_iter;
+        g_dev.log = dmalloc<double>((size_t)g_dev.log_cap * 8);
+    }
+    g_log.assign((size_t)max_iter * 8, 0.0);
+    init_scalars(iter0, t0, max_iter, g_transient, g_t_end);
+
+    int printed = 0;
+    int status = 0;
+    while (true)
+    {
+        int chunk = std::max(1, g_sync_interval);
+        if (g_do_output && g_i_output_data > 0)
+        {
+            int it = iter0 + printed;
+            int to_next = g_i_output_data - (it % g_i_output_data);
+            chunk = std::min(chunk, to_next);
+        }
+        for (int i = 0; i < chunk; ++i)
+            enqueue_explicit_step(g_dev.sc);
+        g_dev.r_valid = true;
+        read_scalars();
+        const dev::DevScalars &h = *g_dev.sc_host;
+        int done_rows = h.iter - iter0;
+        if (done_rows > printed)
+        {
+            HIP_CHECK(hipMemcpy(&g_log[(size_t)printed * 8],
+                                &g_dev.log[(size_t)printed * 8],
+                                (size_t)(done_rows - printed) * 8 * sizeof(double),
+                                hipMemcpyDeviceToHost));
+            for (int r = printed; r < done_rows; ++r)
+            {
+                int it = iter0 + r + 1;
+                bool last = (r == done_rows - 1) && h.done != 0;
+                bool out = last || (g_i_output_data > 0 && it % g_i_output_data == 0);
+                print_row(it, &g_log[(size_t)r * 8], out);
+                for (int j = 0; j < 5; ++j)
+                    g_residual[j] = g_log[(size_t)r * 8 + 3 + j];
+                g_viscous_dt = g_log[(size_t)r * 8 + 2] != 0.0;
+            }
+            printed = done_rows;
+        }
+        if (h.done < 0)
+        {
+            status = FVH_ERR_NAN;
+            break;
+        }
+        if (g_do_output && h.done == 0 && g_i_output_data > 0 &&
+            (iter0 + printed) % g_i_output_data == 0)
+            write_output(iter0 + printed, h.t);
+        if (h.done > 0)
+        {
+            if (g_do_output)
+                write_output(iter0 + printed, h.t);
+            break;
+        }
+    }
+    g_last_iterations = printed;
+    g_last_status = status;
+    if (g_verbose)
+        std::fflush(stdout);
+    return status;
+}
+
+} // namespace
+} // namespace fvh
+
+using namespace fvh;
+
+// ===========================================================================
+// Solver API (reference names)
+// ===========================================================================
+extern "C" {
+
+// ---- solver.c -------------------------------------------------------------
+void init_solver(cstring_t title) { g_title = title ? title : "fv3d"; }
+cstring_t get_simulation_title(void) { return g_title.c_str(); }
+void free_solver(void) { g_title = "fv3d"; }
+
+// ---- equation.c:37-169 ------------------------------------------------------
+void init_equation(void)
+{
+    if (solver_variables)
+        free_equation();
+    solver_variables = xalloc<Variables_t>(1);
+}
+
+static void rebuild_tot_variables()
+{
+    Variables_t *v = solver_variables;
+    std::free(v->tot_variables);
+    v->n_tot_variables = v->n_sol_variables + v->n_dep_variables;
+    v->tot_variables = xalloc<Variable_t *>(v->n_tot_variables);
+    for (int i = 0; i < v->n_sol_variables; ++i)
+        v->tot_variables[i] = &v->sol_variables[i];
+    for (int i = 0; i < v->n_dep_variables; ++i)
+        v->tot_variables[v->n_sol_variables + i] = &v->dep_variables[i];
+}
+
+int add_sol_variable(cstring_t name)
+{
+    Variables_t *v = solver_variables;
+    if (!v)
+        fatal("add_sol_variable before init_equation");
+    v->sol_variables = (Variable_t *)std::realloc(
+        v->sol_variables, sizeof(Variable_t) * (v->n_sol_variables + 1));
+    v->sol_variables[v->n_sol_variables].name = xstrdup(name);
+    v->n_sol_variables += 1;
+    rebuild_tot_variables();
+    return v->n_sol_variables - 1;
+}
+
+int add_dep_variable(cstring_t name)
+{
+    Variables_t *v = solver_variables;
+    if (!v)
+        fatal("add_dep_variable before init_equation");
+    v->dep_variables = (Variable_t *)std::realloc(
+        v->dep_variables, sizeof(Variable_t) * (v->n_dep_variables + 1));
+    v->dep_variables[v->n_dep_variables].name = xstrdup(name);
+    v->n_dep_variables += 1;
+    rebuild_tot_variables();
+    // dependent variables follow all solution variables (equation.c:153-169)
+    return v->n_sol_variables + v->n_dep_variables - 1;
+}
+
+void free_equation(void)
+{
+    Variables_t *v = solver_variables;
+    if (!v)
+        return;
+    for (int i = 0; i < v->n_sol_variables; ++i)
+        std::free(v->sol_variables[i].name);
+    for (int i = 0; i < v->n_dep_variables; ++i)
+        std::free(v->dep_variables[i].name);
+    std::free(v->sol_variables);
+    std::free(v->dep_variables);
+    std::free(v->tot_variables);
+    std::free(v);
+    solver_variables = nullptr;
+}
+
+void print_variables(void)
+{
+    Variables_t *v = solver_variables;
+    if (!v)
+        return;
+    std::printf("Solution variables (%d):", v->n_sol_variables);
+    for (int i = 0; i < v->n_sol_variables; ++i)
+        std::printf(" %s", v->sol_variables[i].name);
+    std::printf("\nDependent variables (%d):", v->n_dep_variables);
+    for (int i = 0; i < v->n_dep_variables; ++i)
+        std::printf(" %s", v->dep_variables[i].name);
+    std::printf("\n");
+}
+
+// ---- mesh.c -------------------------------------------------------------------
+void init_mesh(cstring_t mesh_file)
+{
+    try
+    {
+        RawMesh m;
+        std::string p = mesh_file ? mesh_file : "";
+        int rc;
+        if (p.size() > 3 && p.substr(p.size() - 3) == ".h5")
+            rc = read_hdf5_mesh(m, p.c_str());
+        else
+            rc = read_raw_mesh(m, p.c_str());
+        if (rc != 0)
+            fatal("cannot read mesh file " + p + ": " + g_error);
+        install_mesh(std::move(m));
+    }
+    catch (const std::exception &e)
+    {
+        fatal(e.what());
+    }
+}
+
+void free_mesh(void)
+{
+    free_mesh_struct(solver_mesh);
+    solver_mesh = nullptr;
+    g_face_shift.clear();
+}
+
+void print_mesh_info(void)
+{
+    Mesh_t *m = solver_mesh;
+    if (!m)
+        return;
+    std::printf("Mesh: dimension %d, cells %d, faces %d (internal %d, "
+                "boundary %d), boundaries %d, regions %d, volume %.6e\n",
+                m->dimension, m->cells->n_domain_cells, m->faces->n_faces,
+                m->faces->n_internal_faces, m->faces->n_boundary_faces,
+                m->boundaries->n_boundaries, m->regions->n_regions,
+                m->global_volume);
+    for (int i = 0; i < m->regions->n_regions; ++i)
+        std::printf("  region %d: %s%s\n", i, m->regions->name[i],
+                    m->regions->is_boundary[i] ? " (boundary)" : "");
+}
+
+// ---- finite_volume.c ------------------------------------------------------------
+void set_calc_flux(void_calc_flux_ft fun_ptr) { g_calc_flux = fun_ptr; }
+void set_exact_function(void_calc_exact_ft fun_ptr) { g_calc_exact = fun_ptr; }
+void set_update(void_update_ft fun_ptr) { g_update = fun_ptr; }
+void set_update_gradients(void_update_gradients_ft fun_ptr)
+{
+    g_update_gradients = fun_ptr;
+}
+
+// finite_volume.c:23-55 (host mirror) + device buffers; finite_volume.c:189-201
+void init_finite_volume(void)
+{
+    try
+    {
+        if (!solver_mesh || !solver_variables)
+            fatal("init_finite_volume needs the mesh and the variables");
+        if (solver_variables->n_sol_variables != 5 ||
+            solver_variables->n_tot_variables != 10)
+            fatal("the device path implements the 5 + 5 Navier-Stokes variables");
+        if (!solver_mesh->regions->phi_total)
+            fatal("boundary regions not initialised (navier_stokes_initialize)");
+        if (g_dev.ready)
+            free_finite_volume();
+
+        const Cells_t *cells = solver_mesh->cells;
+        const Faces_t *faces = solver_mesh->faces;
+        const Boundaries_t *bnd = solver_mesh->boundaries;
+        const Regions_t *reg = solver_mesh->regions;
+        const int N = cells->n_domain_cells, F = faces->n_faces,
+                  B = bnd->n_boundaries, maxf = cells->max_cell_faces;
+
+        // host mirror: state rows only; gradients, face states and fluxes
+        // live on the device (they are recomputed per face there)
+        solver_data = xalloc<data_t>(1);
+        solver_data->phi_total = xalloc<double>((size_t)(N + B) * 10);
+        solver_data->phi_dt = xalloc<double>((size_t)(N + B) * 5);
+
+        HIP_CHECK(hipStreamCreateWithFlags(&g_dev.stream, hipStreamNonBlocking));
+        dev::DevMesh &m = g_dev.m;
+        m.N = N;
+        m.F = F;
+        m.B = B;
+        m.maxf = maxf;
+        {
+            std::vector<int> cf((size_t)maxf * N);
+            for (int c = 0; c < N; ++c)
+                for (int s = 0; s < maxf; ++s)
+                    cf[(size_t)s * N + c] = cells->faces[(size_t)c * maxf + s];
+            m.cf = dupload(cf.data(), cf.size());
+            m.ncf = dupload(cells->n_faces, N);
+        }
+        {
+            std::vector<int> a(F), b(F);
+            for (int f = 0; f < F; ++f)
+                a[f] = faces->cells[2 * f], b[f] = faces->cells[2 * f + 1];
+            m.fc0 = dupload(a.data(), F);
+            m.fc1 = dupload(b.data(), F);
+        }
+        auto soa3 = [&](const double *aos, int n) {
+            std::vector<double> v((size_t)3 * n);
+            for (int i = 0; i < n; ++i)
+                for (int d = 0; d < 3; ++d)
+                    v[(size_t)d * n + i] = aos[(size_t)i * 3 + d];
+            return dupload(v.data(), v.size());
+        };
+        m.fn = soa3(faces->n, F);
+        m.ft1 = soa3(faces->t1, F);
+        m.ft2 = soa3(faces->t2, F);
+        m.fd1 = soa3(faces->dist_cell_1, F);
+        m.fd2 = soa3(faces->dist_cell_2, F);
+        m.farea = dupload(faces->area, F);
+        m.flam = dupload(faces->lambda, F);
+        m.vol = dupload(cells->volume, N);
+        {
+            std::vector<double> sv(N);
+            for (int c = 0; c < N; ++c)
+                sv[c] = 1.0 / cells->volume[c];
+            m.svol = dupload(sv.data(), N);
+        }
+        m.cdx = soa3(cells->dx, N);
+        m.bnd_region = dupload(bnd->id, B);
+        m.bnd_face = dupload(bnd->face, B);
+        m.region_type = dupload(reg->type, reg->n_regions);
+        m.region_phi = dupload(reg->phi_total, (size_t)reg->n_regions * 10);
+
+        g_dev.U[0] = dmalloc<double>((size_t)5 * N);
+        g_dev.U[1] = dmalloc<double>((size_t)5 * N);
+        g_dev.k = dmalloc<double>((size_t)5 * N);
+        g_dev.R = dmalloc<double>((size_t)5 * N);
+        g_dev.partials = dmalloc<double>((size_t)dev::cell_blocks(N + B) *
+                                         dev::kPartialStride);
+        g_dev.sc = dmalloc<dev::DevScalars>(1);
+        HIP_CHECK(hipHostMalloc((void **)&g_dev.sc_host, sizeof(dev::DevScalars),
+                                hipHostMallocDefault));
+        g_dev.mirror_phi = dmalloc<double>((size_t)(N + B) * 10);
+        g_dev.mirror_dt = dmalloc<double>((size_t)N * 5);
+        g_dev.cur = 0;
+        g_dev.vglob = solver_mesh->global_volume;
+        g_dev.ready = true;
+
+        // set_solution (finite_volume.c:189-201): exact function per domain
+        // cell on the host, then the update hook's work on the device
+        if (!g_calc_exact)
+            fatal("no exact function set");
+        for (int i = 0; i < N; ++i)
+            g_calc_exact(0, 0.0, &cells->x[(size_t)i * DIM],
+                         &solver_data->phi_total[(size_t)i * 10]);
+        upload_state_from_mirror();
+        sync_mirror(false);
+    }
+    catch (const std::exception &e)
+    {
+        fatal(e.what());
+    }
+}
+
+void free_finite_volume(void)
+{
+    g_calc_flux = nullptr;
+    g_calc_exact = nullptr;
+    g_update = nullptr;
+    g_update_gradients = nullptr;
+    free_device();
+    if (solver_data)
+    {
+        std::free(solver_data->phi_total);
+        std::free(solver_data->phi_dt);
+        std::free(solver_data->grad_phi_total_x);
+        std::free(solver_data->grad_phi_total_y);
+        std::free(solver_data->grad_phi_total_z);
+        std::free(solver_data);
+        solver_data = nullptr;
+    }
+}
+
+// finite_volume.c:81-124 on the device; refreshes the host mirror
+void finite_volume_time_derivative(double time)
+{
+    (void)time;
+    try
+    {
+        require_ready();
+        enqueue_residual(g_dev.U[g_dev.cur], nullptr, g_dev.R, 0, false, false,
+                         nullptr);
+        g_dev.r_valid = true;
+        sync_mirror(true);
+    }
+    catch (const std::exception &e)
+    {
+        fatal(e.what());
+    }
+}
+
+// ---- reconstruction.c / limiter.c --------------------------------------------
+void init_reconstruction(reconstruction_type_t t)
+{
+    if (t != FirstOrder && t != Linear)
+        fatal("init_reconstruction: unknown type");
+    g_reconstruction = t;
+}
+void free_reconstruction(void) { g_reconstruction = -1; }
+void init_limiter(limiter_type_t t)
+{
+    if (t != BarthJespersenn && t != NoLimter)
+        fatal("init_limiter: unknown type");
+    g_limiter = t;
+}
+void free_limiter(void) { g_limiter = -1; }
+
+// ---- analyze.c ----------------------------------------------------------------
+void init_analyze(void)
+{
+    g_analyze_ready = true;
+    std::fill(g_residual.begin(), g_residual.end(), 0.0);
+}
+void free_analyze(void) { g_analyze_ready = false; }
+
+// analyze.c:37-61 from the last residual on the device
+void calc_global_residual(double dt)
+{
+    try
+    {
+        require_ready();
+        if (!g_dev.r_valid)
+            fatal("calc_global_residual: no residual evaluated yet");
+        std::vector<double> R((size_t)5 * g_dev.m.N);
+        HIP_CHECK(hipMemcpy(R.data(), g_dev.R, R.size() * 8,
+                            hipMemcpyDeviceToHost));
+        // ordered host sum over the device residual (a reporting path; the
+        // time loop uses the device reduction)
+        double tmp[5] = {0, 0, 0, 0, 0};
+        const int N = g_dev.m.N;
+        for (int i = 0; i < N; ++i)
+        {
+            double v = solver_mesh->cells->volume[i];
+            for (int j = 0; j < 5; ++j)
+            {
+                double r = R[(size_t)j * N + i];
+                tmp[j] += (r < 0 ? -r : r) * v;
+            }
+        }
+        for (int j = 0; j < 5; ++j)
+            g_residual[j] = tmp[j] * (dt / solver_mesh->global_volume);
+    }
+    catch (const std::exception &e)
+    {
+        fatal(e.what());
+    }
+}
+
+// ---- timedisc.c / explicit.c / implicit.c --------------------------------------
+void init_timedisc(timedisc_type_t type, int max_iter, bool_t is_transient,
+                   double abort_residual, double t_start, double t_end)
+{
+    if (type == Explicit)
+        g_is_explicit = true;
+    else if (type == Implicit)
+        g_is_explicit = false;
+    else
+        fatal("init_timedisc: unknown type");
+    g_max_iter = max_iter;
+    g_transient = is_transient != 0;
+    g_abort_residual = abort_residual;
+    g_t_start = t_start;
+    g_t_end = g_transient ? t_end : DBL_MAX;
+}
+
+void free_timedisc(void) { g_calc_timestep = nullptr; }
+bool_t is_explicit(void) { return g_is_explicit ? BC_TRUE : BC_FALSE; }
+void set_calc_timestep(double_calc_timestep_ft fun_ptr) { g_calc_timestep = fun_ptr; }
+void set_viscous_dt(bool_t v) { g_viscous_dt = v != 0; }
+
+void init_explicit(explicit_scheme_t scheme)
+{
+    switch (scheme)
+    {
+    case EulerExplicit:
+        g_rk_stages = 1, g_rk_a = rk_a_euler, g_rk_b = rk_b_euler,
+        g_rk_g = rk_g_euler;
+        break;
+    case RungeKutta33:
+        g_rk_stages = 3, g_rk_a = rk_a_rk33, g_rk_b = rk_b_rk33,
+        g_rk_g = rk_g_rk33;
+        break;
+    case RungeKutta45:
+        g_rk_stages = 5, g_rk_a = rk_a_rk45, g_rk_b = rk_b_rk45,
+        g_rk_g = rk_g_rk45;
+        break;
+    default:
+        fatal("init_explicit: unknown scheme");
+    }
+}
+
+void free_explicit(void)
+{
+    g_rk_stages = 0;
+    g_rk_a = g_rk_b = g_rk_g = nullptr;
+}
+
+void init_implicit(implicit_scheme_t implicit_scheme,
+                   implicit_solver_t implicit_solver, int max_iter_inner,
+                   double tolerance_lsoe, int max_iter_lsoe,
+                   int max_krylov_dims, int max_krylov_restarts)
+{
+    if (implicit_scheme != EulerImplicit && implicit_scheme != BDF2)
+        fatal("init_implicit: unknown scheme");
+    if (implicit_solver != BiCGStab && implicit_solver != GMRes)
+        fatal("init_implicit: unknown solver");
+    g_bdf_scheme = implicit_scheme;
+    g_implicit_solver = implicit_solver;
+    g_max_iter_inner = max_iter_inner;
+    g_tolerance_lsoe = tolerance_lsoe;
+    g_max_iter_lsoe = max_iter_lsoe;
+    g_max_krylov_dims = max_krylov_dims;
+    g_max_krylov_restarts = max_krylov_restarts;
+}
+
+void free_implicit(void) {}
+
+void timedisc(void)
+{
+    int rc;
+    try
+    {
+        if (!g_is_explicit)
+            fatal("timedisc: the implicit scheme is driven by the fvh_jfnk_* "
+                  "extension API");
+        rc = run_explicit(g_max_iter);
+    }
+    catch (const std::exception &e)
+    {
+        fatal(e.what());
+    }
+    if (rc == FVH_ERR_NAN)
+        fatal("timedisc: NaN or Inf residual");
+}
+
+// ---- output.c / restart.c --------------------------------------------------------
+void init_output(int i_output_data)
+{
+    g_i_output_data = i_output_data;
+    g_do_output = i_output_data != 0;
+}
+void free_output(void) {}
+
+void create_file_header(cstring_t file_name)
+{
+    try
+    {
+        RawWriter w(file_name);
+        w.attr_int("n_tot_variables", solver_variables->n_tot_variables);
+        w.attr_int("n_sol_variables", solver_variables->n_sol_variables);
+        std::vector<std::string> tot, sol;
+        for (int i = 0; i < solver_variables->n_tot_variables; ++i)
+            tot.push_back(solver_variables->tot_variables[i]->name);
+        for (int i = 0; i < solver_variables->n_sol_variables; ++i)
+            sol.push_back(solver_variables->sol_variables[i].name);
+        w.strings("tot_variables", tot);
+        w.strings("sol_variables", sol);
+    }
+    catch (const std::exception &e)
+    {
+        fatal(e.what());
+    }
+}
+
+// output.c:92-191: <title>.%09d.fvs with attrs iter, t and datasets
+// phi_total [N x 10], phi_dt [N x 5]
+void write_output(int iter, double t)
+{
+    try
+    {
+        sync_mirror(true);
+        char suffix[64];
+        std::snprintf(suffix, sizeof(suffix), ".%09d.fvs", iter);
+        std::string path = g_title + suffix;
+        RawWriter w(path.c_str());
+        w.attr_int("n_tot_variables", solver_variables->n_tot_variables);
+        w.attr_int("n_sol_variables", solver_variables->n_sol_variables);
+        std::vector<std::string> tot, sol;
+        for (int i = 0; i < solver_variables->n_tot_variables; ++i)
+            tot.push_back(solver_variables->tot_variables[i]->name);
+        for (int i = 0; i < solver_variables->n_sol_variables; ++i)
+            sol.push_back(solver_variables->sol_variables[i].name);
+        w.strings("tot_variables", tot);
+        w.strings("sol_variables", sol);
+        w.attr_int("iter", iter);
+        w.attr_double("t", t);
+        const int N = solver_mesh->cells->n_domain_cells;
+        w.doubles("phi_total", solver_data->phi_total, (size_t)N * 10, 10);
+        w.doubles("phi_dt", solver_data->phi_dt, (size_t)N * 5, 5);
+    }
+    catch (const std::exception &e)
+    {
+        fatal(e.what());
+    }
+}
+
+void init_restart(bool_t use_restart, cstring_t restart_file)
+{
+    g_use_restart = use_restart != 0;
+    g_restart_file = restart_file ? restart_file : "";
+    if (g_use_restart)
+        read_restart_data(g_restart_file.c_str());
+}
+void free_restart(void) { g_use_restart = false; }
+
+// restart.c:50-178 with the serial dataset names the writer uses
+void read_restart_data(cstring_t restart_file)
+{
+    try
+    {
+        RawReader r;
+        if (!r.open(restart_file))
+            fatal(std::string("cannot read restart file ") + restart_file);
+        g_iter_restart = r.ints("iter")[0];
+        g_t_restart = r.doubles("t")[0];
+        std::vector<double> phi = r.doubles("phi_total");
+        const int N = solver_mesh->cells->n_domain_cells;
+        if (phi.size() != (size_t)N * 10)
+            fatal("restart file does not match the mesh");
+        if (g_dev.ready)
+        {
+            std::memcpy(solver_data->phi_total, phi.data(), phi.size() * 8);
+            upload_state_from_mirror();
+            sync_mirror(false);
+        }
+    }
+    catch (const std::exception &e)
+    {
+        fatal(e.what());
+    }
+}
+
+// ---- Navier-Stokes plug-in built-ins ------------------------------------------
+// navier_stokes.c:85-124 without the parameter parsing
+void navier_stokes_initialize(void)
+{
+    kappa_m1 = kappa - 1.0;
+    kappa_p1 = kappa + 1.0;
+    s_kappa = 1.0 / kappa;
+    s_kappa_m1 = 1.0 / kappa_m1;
+    s_kappa_p1 = 1.0 / kappa_p1;
+    cv = R_mix * s_kappa_m1;
+    cp = kappa * cv;
+    kappa_pr = kappa / Pr;
+    lambda = mu_mix * cp / Pr;
+
+    init_equation();
+    ic_rho = add_sol_variable("rho");
+    ic_rho_u = add_sol_variable("rho_u");
+    ic_rho_v = add_sol_variable("rho_v");
+    ic_rho_w = add_sol_variable("rho_w");
+    ic_rho_e = add_sol_variable("rho_e");
+    ip_u = add_dep_variable("u");
+    ip_v = add_dep_variable("v");
+    ip_w = add_dep_variable("w");
+    ip_p = add_dep_variable("p");
+    ip_T = add_dep_variable("T");
+
+    set_calc_flux(calc_ns_flux);
+    set_calc_timestep(calc_ns_timestep);
+    set_exact_function(calc_ns_exact_function);
+    set_update(ns_update);
+    set_update_gradients(ns_update_gradients);
+
+    if (solver_mesh)
+    {
+        Regions_t *r = solver_mesh->regions;
+        std::free(r->type), std::free(r->function_id), std::free(r->phi_total);
+        r->type = xalloc<int>(r->n_regions);
+        r->function_id = xalloc<int>(r->n_regions);
+        r->phi_total = xalloc<double>((size_t)r->n_regions * 10);
+    }
+}
+
+// boundary.c:76-121 + parse_primitive_state boundary.c:144-183
+void boundary_set_region(int region, int type, const double *prim5)
+{
+    if (!solver_mesh || !solver_mesh->regions->phi_total)
+        fatal("boundary_set_region before navier_stokes_initialize");
+    Regions_t *r = solver_mesh->regions;
+    if (region < 0 || region >= r->n_regions)
+        fatal("boundary_set_region: region out of range");
+    if (type == BoundaryFunction)
+        fatal("boundary type 'function' is not reachable in the reference "
+              "(boundary.c:127 vs navier_stokes.c:142-150)");
+    if (type < 0 || type >= BoundaryTypeMax)
+        fatal("boundary_set_region: unknown type");
+    if (type == BoundaryFlow && region != r->flow_region)
+        fatal("the flow state belongs to the flow region");
+    r->type[region] = type;
+    double *phi = &r->phi_total[(size_t)region * 10];
+    if (type == BoundaryFlow || type == BoundaryInflow || type == BoundaryState)
+    {
+        for (int j = 0; j < 10; ++j)
+            phi[j] = 0.0;
+        phi[ip_u] = prim5[0];
+        phi[ip_v] = prim5[1];
+        phi[ip_w] = prim5[2];
+        phi[ip_p] = prim5[3];
+        phi[ip_T] = prim5[4];
+        phi[ic_rho] = phi[ip_p] / (R_mix * phi[ip_T]);
+        phi[ic_rho_u] = phi[ip_u] * phi[ic_rho];
+        phi[ic_rho_v] = phi[ip_v] * phi[ic_rho];
+        phi[ic_rho_w] = phi[ip_w] * phi[ic_rho];
+        double d = 0.0;
+        d += phi[ip_u] * phi[ic_rho_u];
+        d += phi[ip_v] * phi[ic_rho_v];
+        d += phi[ip_w] * phi[ic_rho_w];
+        phi[ic_rho_e] = s_kappa_m1 * phi[ip_p] + 0.5 * d;
+    }
+    else if (type == BoundaryIsothermalWall)
+    {
+        phi[ip_T] = prim5[4];
+    }
+}
+
+// flux.c:295-312
+void flux_initialize_scheme(cstring_t name)
+{
+    std::string s = name ? name : "";
+    if (s == "Rusanov")
+        g_flux = dev::FluxRusanov;
+    else if (s == "AUSM")
+        g_flux = dev::FluxAUSM;
+    else
+        fatal("unknown flux scheme " + s);
+}
+
+// The hooks below are the identities the solver dispatches on; the work of
+// each is fused into the device residual kernels.
+void calc_ns_flux(double time) { (void)time; }
+void ns_update(double time) { (void)time; }
+void ns_update_gradients(double time) { (void)time; }
+
+// navier_stokes.c:159-193 on the device (current primitives)
+double calc_ns_timestep(double time)
+{
+    (void)time;
+    try
+    {
+        require_ready();
+        dev::launch_dt_partials(g_dev.stream, g_dev.m, phys(),
+                                g_dev.U[g_dev.cur], g_dev.partials);
+        dev::DevScalars &h = *g_dev.sc_host;
+        std::memset(&h, 0, sizeof(h));
+        h.t_end = DBL_MAX;
+        HIP_CHECK(hipMemcpyAsync(g_dev.sc, &h, sizeof(h), hipMemcpyHostToDevice,
+                                 g_dev.stream));
+        dev::launch_finalize(g_dev.stream, g_dev.sc, g_dev.partials,
+                             dev::cell_blocks(g_dev.m.N), phys(), 0, nullptr, 0);
+        read_scalars();
+        set_viscous_dt(h.viscous);
+        return h.dt;
+    }
+    catch (const std::exception &e)
+    {
+        fatal(e.what());
+    }
+}
+
+// navier_stokes.c:133-152
+void calc_ns_exact_function(int id, double t, double *x, double *phi)
+{
+    (void)t;
+    (void)x;
+    if (id != BoundaryFlow)
+        fatal("calc_ns_exact_function: only the flow region state is defined");
+    Regions_t *r = solver_mesh->regions;
+    for (int j = 0; j < 10; ++j)
+        phi[j] = r->phi_total[(size_t)r->flow_region * 10 + j];
+}
+
+// ===========================================================================
+// fvh_* extension API
+// ===========================================================================
+const char *fvh_last_error(void) { return g_error.c_str(); }
+
+#define FVH_TRY(...)                                                           \
+    try                                                                        \
+    {                                                                          \
+        __VA_ARGS__;                                                           \
+    }                                                                          \
+    catch (const std::exception &e)                                            \
+    {                                                                          \
+        set_error(e.what());                                                   \
+        return FVH_ERR_STATE;                                                  \
+    }
+
+int fvh_mesh_box(int nx, int ny, int nz, double x0, double y0, double z0,
+                 double lx, double ly, double lz, int tets, int periodic,
+                 int tile)
+{
+    FVH_TRY({
+        RawMesh m;
+        generate_box(m, nx, ny, nz, x0, y0, z0, lx, ly, lz, tets != 0,
+                     periodic, tile);
+        install_mesh(std::move(m));
+    });
+    return FVH_OK;
+}
+
+int fvh_mesh_channel(int nx, int ny, int nz, double jitter,
+                     unsigned long long seed)
+{
+    FVH_TRY({
+        RawMesh m;
+        generate_channel(m, nx, ny, nz, jitter, seed);
+        install_mesh(std::move(m));
+    });
+    return FVH_OK;
+}
+
+int fvh_mesh_write(cstring_t path)
+{
+    FVH_TRY({
+        if (!solver_mesh)
+            throw std::runtime_error("no mesh");
+        // reconstruct the file layout (second face cell -1 on boundaries)
+        const Mesh_t *mm = solver_mesh;
+        RawMesh m;
+        m.dimension = mm->dimension;
+        const int N = mm->cells->n_domain_cells, F = mm->faces->n_faces,
+                  B = mm->boundaries->n_boundaries;
+        m.vertices.assign(mm->vertices->x, mm->vertices->x + 3 * mm->vertices->n_vertices);
+        m.max_cell_vertices = mm->cells->max_cell_vertices;
+        m.max_cell_faces = mm->cells->max_cell_faces;
+        m.max_face_vertices = mm->faces->max_face_vertices;
+        m.max_boundary_vertices = mm->boundaries->max_boundary_vertices;
+        auto vi = [](const int *p, size_t n) { return std::vector<int>(p, p + n); };
+        auto vd = [](const double *p, size_t n) { return std::vector<double>(p, p + n); };
+        m.cell_region = vi(mm->cells->id, N);
+        m.cell_type = vi(mm->cells->type, N);
+        m.cell_nv = vi(mm->cells->n_vertices, N);
+        m.cell_vertices = vi(mm->cells->vertices, (size_t)N * m.max_cell_vertices);
+        m.cell_nf = vi(mm->cells->n_faces, N);
+        m.cell_faces = vi(mm->cells->faces, (size_t)N * m.max_cell_faces);
+        m.cell_volume = vd(mm->cells->volume, N);
+        m.cell_x = vd(mm->cells->x, (size_t)N * 3);
+        m.cell_dx = vd(mm->cells->dx, (size_t)N * 3);
+        m.face_type = vi(mm->faces->type, F);
+        m.face_nv = vi(mm->faces->n_vertices, F);
+        m.face_vertices = vi(mm->faces->vertices, (size_t)F * m.max_face_vertices);
+        m.face_cells = vi(mm->faces->cells, (size_t)F * 2);
+        for (int f = 0; f < F; ++f)
+            if (mm->faces->boundary[f] >= 0)
+                m.face_cells[2 * f + 1] = -1;
+        m.face_boundary = vi(mm->faces->boundary, F);
+        m.face_x = vd(mm->faces->x, (size_t)F * 3);
+        m.face_n = vd(mm->faces->n, (size_t)F * 3);
+        m.face_t1 = vd(mm->faces->t1, (size_t)F * 3);
+        m.face_t2 = vd(mm->faces->t2, (size_t)F * 3);
+        m.face_area = vd(mm->faces->area, F);
+        m.face_lambda = vd(mm->faces->lambda, F);
+        m.face_shift = g_face_shift;
+        m.bnd_region = vi(mm->boundaries->id, B);
+        m.bnd_type = vi(mm->boundaries->type, B);
+        m.bnd_nv = vi(mm->boundaries->n_vertices, B);
+        m.bnd_vertices = vi(mm->boundaries->vertices, (size_t)B * m.max_boundary_vertices);
+        m.bnd_face = vi(mm->boundaries->face, B);
+        m.bnd_n = vd(mm->boundaries->n, (size_t)B * 3);
+        m.bnd_t1 = vd(mm->boundaries->t1, (size_t)B * 3);
+        m.bnd_t2 = vd(mm->boundaries->t2, (size_t)B * 3);
+        m.bnd_distance = vd(mm->boundaries->distance, B);
+        for (int i = 0; i < mm->regions->n_regions; ++i)
+            m.region_names.push_back(mm->regions->name[i]);
+        m.region_is_boundary = vi(mm->regions->is_boundary, mm->regions->n_regions);
+        if (write_raw_mesh(m, path) != 0)
+            throw std::runtime_error("cannot write mesh");
+    });
+    return FVH_OK;
+}
+
+int fvh_get_face_shift(double *shift)
+{
+    if (!solver_mesh)
+    {
+        set_error("no mesh");
+        return FVH_ERR_STATE;
+    }
+    std::memcpy(shift, g_face_shift.data(), g_face_shift.size() * sizeof(double));
+    return FVH_OK;
+}
+
+int fvh_set_visc_dt_mode(int mode)
+{
+    if (mode != 0 && mode != 1)
+        return FVH_ERR_ARG;
+    g_visc_dt_mode = mode;
+    return FVH_OK;
+}
+
+int fvh_set_ic(int kind, const double *params, int n_params)
+{
+    if (kind < 0 || kind > 2 || n_params < 0 || n_params > 8)
+        return FVH_ERR_ARG;
+    g_ic_kind = kind;
+    for (int i = 0; i < n_params; ++i)
+        g_ic_params[i] = params[i];
+    return FVH_OK;
+}
+
+void fvh_exact_function_builtin(int id, double t, double *x, double *phi)
+{
+    builtin_exact(id, t, x, phi);
+}
+
+int fvh_get_state(double *U)
+{
+    FVH_TRY({
+        if (!g_dev.ready)
+            throw std::runtime_error("no device state");
+        const int N = g_dev.m.N;
+        std::vector<double> soa((size_t)5 * N);
+        HIP_CHECK(hipMemcpy(soa.data(), g_dev.U[g_dev.cur], soa.size() * 8,
+                            hipMemcpyDeviceToHost));
+        for (int c = 0; c < N; ++c)
+            for (int j = 0; j < 5; ++j)
+                U[(size_t)c * 5 + j] = soa[(size_t)j * N + c];
+    });
+    return FVH_OK;
+}
+
+int fvh_set_state(const double *U)
+{
+    FVH_TRY({
+        if (!g_dev.ready)
+            throw std::runtime_error("no device state");
+        const int N = g_dev.m.N;
+        std::vector<double> soa((size_t)5 * N);
+        for (int c = 0; c < N; ++c)
+            for (int j = 0; j < 5; ++j)
+                soa[(size_t)j * N + c] = U[(size_t)c * 5 + j];
+        HIP_CHECK(hipMemcpy(g_dev.U[g_dev.cur], soa.data(), soa.size() * 8,
+                            hipMemcpyHostToDevice));
+        g_dev.r_valid = false;
+    });
+    return FVH_OK;
+}
+
+int fvh_get_phi_dt(double *R)
+{
+    FVH_TRY({
+        if (!g_dev.ready || !g_dev.r_valid)
+            throw std::runtime_error("no residual on the device");
+        const int N = g_dev.m.N;
+        std::vector<double> soa((size_t)5 * N);
+        HIP_CHECK(hipMemcpy(soa.data(), g_dev.R, soa.size() * 8,
+                            hipMemcpyDeviceToHost));
+        for (int c = 0; c < N; ++c)
+            for (int j = 0; j < 5; ++j)
+                R[(size_t)c * 5 + j] = soa[(size_t)j * N + c];
+    });
+    return FVH_OK;
+}
+
+int fvh_get_field(int what, double *out)
+{
+    FVH_TRY({
+        if (!g_dev.ready)
+            throw std::runtime_error("no device state");
+        const int N = g_dev.m.N;
+        const double *src = what == 0 ? g_dev.grad : (what == 1 ? g_dev.mnmx : nullptr);
+        const int rows = what == 0 ? 30 : 20;
+        if (!src)
+            throw std::runtime_error("field not computed by the current scheme");
+        HIP_CHECK(hipStreamSynchronize(g_dev.stream));
+        std::vector<double> soa((size_t)rows * N);
+        HIP_CHECK(hipMemcpy(soa.data(), src, soa.size() * 8, hipMemcpyDeviceToHost));
+        for (int c = 0; c < N; ++c)
+            for (int q = 0; q < rows; ++q)
+                out[(size_t)c * rows + q] = soa[(size_t)q * N + c];
+    });
+    return FVH_OK;
+}
+
+int fvh_sync_host(void)
+{
+    FVH_TRY({ sync_mirror(true); });
+    return FVH_OK;
+}
+
+int fvh_set_verbose(int verbose)
+{
+    g_verbose = verbose;
+    return FVH_OK;
+}
+
+int fvh_set_sync_interval(int steps)
+{
+    if (steps < 1)
+        return FVH_ERR_ARG;
+    g_sync_interval = steps;
+    return FVH_OK;
+}
+
+int fvh_get_log(double *rows, int max_rows)
+{
+    int n = std::min(max_rows, g_last_iterations);
+    if (n > 0)
+        std::memcpy(rows, g_log.data(), (size_t)n * 8 * sizeof(double));
+    return n;
+}
+
+int fvh_last_status(int *iterations)
+{
+    if (iterations)
+        *iterations = g_last_iterations;
+    return g_last_status;
+}
+
+int fvh_timedisc(void)
+{
+    FVH_TRY({ return run_explicit(g_max_iter); });
+}
+
+int fvh_bench_explicit(int n_steps, int n_warmup, double *seconds,
+                       double *k_us)
+{
+    FVH_TRY({
+        require_ready();
+        if (g_rk_stages <= 0)
+            throw std::runtime_error("explicit scheme not initialised");
+        const int total = n_steps + n_warmup;
+        if (g_dev.log_cap < total)
+        {
+            g_dev.log_cap = total;
+            g_dev.log = dmalloc<double>((size_t)total * 8);
+        }
+        init_scalars(0, 0.0, 1 << 30, true, DBL_MAX);
+        for (int i = 0; i < n_warmup; ++i)
+            enqueue_explicit_step(g_dev.sc);
+        hipEvent_t e0, e1;
+        HIP_CHECK(hipEventCreate(&e0));
+        HIP_CHECK(hipEventCreate(&e1));
+        HIP_CHECK(hipStreamSynchronize(g_dev.stream));
+        HIP_CHECK(hipEventRecord(e0, g_dev.stream));
+        for (int i = 0; i < n_steps; ++i)
+            enqueue_explicit_step(g_dev.sc);
+        HIP_CHECK(hipEventRecord(e1, g_dev.stream));
+        HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        *seconds = ms * 1e-3;
+        read_scalars();
+        if (g_dev.sc_host->done < 0)
+            throw std::runtime_error("NaN/Inf residual during the benchmark");
+
+        // instrumented pass: events around each kernel of n_steps steps
+        if (k_us)
+        {
+            const int ev_steps = std::max(1, std::min(n_steps, 20));
+            double acc[3] = {0, 0, 0};
+            int cnt[3] = {0, 0, 0};
+            hipEvent_t a, b;
+            HIP_CHECK(hipEventCreate(&a));
+            HIP_CHECK(hipEventCreate(&b));
+            const dev::PhysDev P = phys();
+            for (int st = 0; st < ev_steps; ++st)
+            {
+                for (int s = 0; s < g_rk_stages; ++s)
+                {
+                    const bool last = s == g_rk_stages - 1;
+                    double *Uin = g_dev.U[g_dev.cur];
+                    double *Uout = g_dev.U[1 - g_dev.cur];
+                    ensure_gradient_buffers();
+                    if (needs_gradients())
+                    {
+                        HIP_CHECK(hipEventRecord(a, g_dev.stream));
+                        dev::launch_gradient(g_dev.stream, g_dev.m, P, Uin,
+                                             g_dev.grad, g_dev.mnmx, P.bj, g_dev.sc);
+                        HIP_CHECK(hipEventRecord(b, g_dev.stream));
+                        HIP_CHECK(hipEventSynchronize(b));
+                        float t = 0;
+                        HIP_CHECK(hipEventElapsedTime(&t, a, b));
+                        acc[0] += t * 1e3, cnt[0]++;
+                    }
+                    dev::StageParams sp{};
+                    sp.U_in = Uin;
+                    sp.U_out = Uout;
+                    sp.k = g_dev.k;
+                    sp.R_out = last ? g_dev.R : nullptr;
+                    sp.grad = g_dev.grad;
+                    sp.mnmx = g_dev.mnmx;
+                    sp.rk_a = g_rk_a[s];
+                    sp.rk_g = g_rk_g[s];
+                    sp.stage0 = s == 0;
+                    sp.do_update = 1;
+                    sp.do_reduce = last;
+                    sp.sc = g_dev.sc;
+                    sp.partials = g_dev.partials;
+                    HIP_CHECK(hipEventRecord(a, g_dev.stream));
+                    dev::launch_residual(g_dev.stream, g_dev.m, P, sp, g_flux,
+                                         is_linear(), is_viscous());
+                    HIP_CHECK(hipEventRecord(b, g_dev.stream));
+                    HIP_CHECK(hipEventSynchronize(b));
+                    float t = 0;
+                    HIP_CHECK(hipEventElapsedTime(&t, a, b));
+                    acc[1] += t * 1e3, cnt[1]++;
+                    g_dev.cur = 1 - g_dev.cur;
+                }
+                HIP_CHECK(hipEventRecord(a, g_dev.stream));
+                dev::launch_finalize(g_dev.stream, g_dev.sc, g_dev.partials,
+                                     dev::cell_blocks(g_dev.m.N), P, 1, g_dev.log,
+                                     g_dev.log_cap);
+                HIP_CHECK(hipEventRecord(b, g_dev.stream));
+                HIP_CHECK(hipEventSynchronize(b));
+                float t = 0;
+                HIP_CHECK(hipEventElapsedTime(&t, a, b));
+                acc[2] += t * 1e3, cnt[2]++;
+            }
+            for (int q = 0; q < 3; ++q)
+                k_us[q] = cnt[q] ? acc[q] / cnt[q] : 0.0;
+            hipEventDestroy(a);
+            hipEventDestroy(b);
+        }
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        g_dev.r_valid = true;
+    });
+    return FVH_OK;
+}
+
+int fvh_device_info(char *name, int name_len, int *cu_count,
+                    double *hbm_gbps_peak)
+{
+    FVH_TRY({
+        int d = 0;
+        HIP_CHECK(hipGetDevice(&d));
+        hipDeviceProp_t p;
+        HIP_CHECK(hipGetDeviceProperties(&p, d));
+        if (name && name_len > 0)
+        {
+            std::snprintf(name, name_len, "%s (%s)", p.name, p.gcnArchName);
+        }
+        if (cu_count)
+            *cu_count = p.multiProcessorCount;
+        if (hbm_gbps_peak)
+            *hbm_gbps_peak = 8000.0; // MI355X published HBM3E peak
+    });
+    return FVH_OK;
+}
+
+} // extern "C"

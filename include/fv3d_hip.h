@@ -1,0 +1,113 @@
+/*
+ * fv3d_hip.h — extension API (fvh_*) of libfv3d_hip.so.
+ *
+ * The Solver API of fv3d_solver.h is the drop-in boundary; the functions
+ * here have no counterpart in the reference and serve the things the
+ * reference does not have: mesh generation with periodic pairing, direct
+ * device state access for tests, device-side stepping control and the
+ * measurement hooks.  All return 0 on success and a negative status on
+ * error (fvh_last_error() gives the message), unlike the reference-named
+ * functions, which abort on error as the reference does.
+ */
+#ifndef FV3D_HIP_H
+#define FV3D_HIP_H
+
+#include "fv3d_solver.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FVH_OK 0
+#define FVH_ERR_ARG -1
+#define FVH_ERR_STATE -2
+#define FVH_ERR_HIP -3
+#define FVH_ERR_IO -4
+#define FVH_ERR_NAN -5
+
+const char *fvh_last_error(void);
+
+/* ---- mesh generation (replaces reading a pyMeshFV3D file) ------------- */
+/* Structured box [x0,x0+lx]x[y0,y0+ly]x[z0,z0+lz] with nx*ny*nz hexahedra,
+ * optionally Kuhn-split into 6 tetrahedra each (tets != 0).  periodic is a
+ * bit mask (1 = x, 2 = y, 4 = z); non-periodic sides become boundary
+ * regions "xmin","xmax","ymin","ymax","zmin","zmax" (in that order, after
+ * the flow region "flow").  tile > 0 orders the hexahedra in tile^3 blocks
+ * for cache locality (0 = lexicographic x-fastest).  Builds solver_mesh
+ * exactly as init_mesh does (reader layout + calc_mesh_metrics). */
+int fvh_mesh_box(int nx, int ny, int nz, double x0, double y0, double z0,
+                 double lx, double ly, double lz, int tets, int periodic,
+                 int tile);
+
+/* Bump channel x in [-2,2], y in [h(x),1], h = 0.1*exp(-25 x^2),
+ * z in [0,0.5]; nx*ny*nz hexahedra Kuhn-split into tetrahedra; interior
+ * nodes jittered uniformly by +-jitter*spacing per axis (splitmix64 with
+ * the given seed).  Regions: "flow","inflow","outflow","bottom","top",
+ * "zmin","zmax". */
+int fvh_mesh_channel(int nx, int ny, int nz, double jitter,
+                     unsigned long long seed);
+
+/* Write / read the mesh in a raw little-endian container (.fvm) with the
+ * same groups and datasets as the reference HDF5 mesh file.  init_mesh()
+ * accepts .fvm files and, when libhdf5 can be loaded, .h5 files. */
+int fvh_mesh_write(cstring_t path);
+/* Periodic translation of each face's cells[1] image, [F*3] (zeros on
+ * non-periodic faces).  Faces_t has no field for it. */
+int fvh_get_face_shift(double *shift);
+
+/* ---- physics parameters not carried by the reference's globals -------- */
+/* visc_dt_mode 0: reference dt formula verbatim (dt = 0 when mu_mix = 0);
+ * 1: skip the viscous limit when mu_mix == 0. */
+int fvh_set_visc_dt_mode(int mode);
+/* Built-in initial conditions usable with set_exact_function:
+ * kind 0 = density wave rho = a + b*sin(2*pi*(x+y+z)), u=v=w=c, p = d
+ *          (params a, b, c, d);
+ * kind 1 = Taylor-Green vortex on [0,2pi]^3 (params p0, rho0, U0) with
+ *          rho = p/(R_mix*T0), T0 = p0/(rho0*R_mix);
+ * kind 2 = uniform "flow" region state (the reference's exact function). */
+int fvh_set_ic(int kind, const double *params, int n_params);
+void fvh_exact_function_builtin(int id, double t, double *x, double *phi);
+
+/* ---- state access ------------------------------------------------------ */
+/* Conservative state of the domain cells, AoS [N*5]. */
+int fvh_get_state(double *U);
+int fvh_set_state(const double *U);
+/* Last residual (phi_dt) of the domain cells, AoS [N*5]. */
+int fvh_get_phi_dt(double *R);
+/* Device fields for tests: what 0 = gradients [N*30] as (dim, var) =
+ * g[c*30 + d*10 + j]; 1 = neighbourhood min/max [N*20] (min j, max j). */
+int fvh_get_field(int what, double *out);
+/* Refresh the host mirror solver_data->phi_total/phi_dt from the device. */
+int fvh_sync_host(void);
+
+/* ---- stepping ---------------------------------------------------------- */
+/* Print per-step residual lines from timedisc() (default 1). */
+int fvh_set_verbose(int verbose);
+/* Steps per host synchronisation inside timedisc() (default 16). */
+int fvh_set_sync_interval(int steps);
+/* timedisc() returning a status instead of aborting: FVH_OK, or
+ * FVH_ERR_NAN when the residual became NaN/Inf (the reference aborts). */
+int fvh_timedisc(void);
+/* Per-step log of the last timedisc(): rows of (t, dt, viscous, res[5]);
+ * returns the number of rows copied (<= max_rows). */
+int fvh_get_log(double *rows, int max_rows);
+/* Status of the last timedisc(): iterations done and 0 / FVH_ERR_NAN. */
+int fvh_last_status(int *iterations);
+
+/* Time the explicit step loop: runs n_steps after n_warmup, returns the
+ * elapsed seconds of the timed steps (device events on the library's
+ * stream) and the per-kernel average durations in microseconds
+ * (k_us[0] gradient kernel, k_us[1] residual/stage kernel, k_us[2]
+ * finalize), measured with HIP events in a separate, instrumented pass. */
+int fvh_bench_explicit(int n_steps, int n_warmup, double *seconds,
+                       double *k_us);
+
+/* ---- device information ----------------------------------------------- */
+int fvh_device_info(char *name, int name_len, int *cu_count,
+                    double *hbm_gbps_peak);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FV3D_HIP_H */
