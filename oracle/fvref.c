@@ -152,7 +152,8 @@ static double calc_ig_T(double p, double rho, double R)
     return p / (rho * R);
 }
 
-/* eos.c:55-75 */
+/* eos.c:55-75.  The compression branch uses A = 2/((kappa+1) u_n) where
+ * the textbook shock relation has 2/((kappa+1) rho); kept for parity. */
 static double calc_riemann_p(fvref_t *s, const double *phi)
 {
     if (phi[IP_U] <= 0.0)
@@ -888,6 +889,35 @@ static void timestep_lserkw2(fvref_t *s, double *k, int n_stages,
             for (int j = 0; j < NSOL; ++j)
                 s->phi_total[i * NTOT + j] += k[i * NSOL + j] * dt * rk_g[st];
     }
+}
+
+/* Amplification factor of the 2N-storage loop of timestep_lserkw2 on the
+ * scalar ODE y' = lambda*y with z = lambda*dt (pins the tables and the loop
+ * form k = R + a*k, y += k*dt*g). */
+double fvref_lserk_amplification(int scheme, double z)
+{
+    const double *a, *g;
+    int ns;
+    switch (scheme)
+    {
+    case 0:
+        a = rk_a_euler, g = rk_g_euler, ns = 1;
+        break;
+    case 1:
+        a = rk_a_rk33, g = rk_g_rk33, ns = 3;
+        break;
+    default:
+        a = rk_a_rk45, g = rk_g_rk45, ns = 5;
+        break;
+    }
+    double y = 1.0, k = 0.0;
+    for (int st = 0; st < ns; ++st)
+    {
+        double r = z * y; /* dt * R(y) with dt = 1 */
+        k = st == 0 ? r : r + k * a[st];
+        y += k * 1.0 * g[st];
+    }
+    return y;
 }
 
 /* src/Solver/src/timedisc/timedisc.c:134-222 (explicit branch) */

@@ -88,6 +88,9 @@ int fvref_timedisc_explicit(fvref_t *s, int scheme, int max_iter,
                             int transient, double abort_residual,
                             double t_start, double t_end, double *log);
 
+/* Amplification factor of the LSERK loop on y' = lambda*y, z = lambda*dt. */
+double fvref_lserk_amplification(int scheme, double z);
+
 /* Reference-semantics implicit step (timestep_newton): BDF1/BDF2 with the
  * lumped finite-difference block Jacobian; the block-diagonal linear system
  * is solved exactly per cell (the reference hands it to an external Krylov

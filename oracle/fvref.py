@@ -62,6 +62,8 @@ def lib():
     L.fvref_riemann_p.restype = c_double
     L.fvref_riemann_p.argtypes = [c_void_p, PD]
     L.fvref_riemann.argtypes = [c_void_p, c_int, PD, PD, PD]
+    L.fvref_lserk_amplification.restype = c_double
+    L.fvref_lserk_amplification.argtypes = [c_int, c_double]
     _L = L
     return L
 
