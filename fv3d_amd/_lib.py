@@ -102,7 +102,8 @@ DATA_SYMBOLS = [
 
 
 def lib_path():
-    return os.path.join(_HERE, "libfv3d_hip.so")
+    """The in-tree build, or an alternate build named by FV3D_HIP_LIB."""
+    return os.environ.get("FV3D_HIP_LIB") or os.path.join(_HERE, "libfv3d_hip.so")
 
 
 def lib():

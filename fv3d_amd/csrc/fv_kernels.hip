@@ -14,8 +14,33 @@
 // as the reference's x86-64 build.
 #include <cfloat>
 #include <cmath>
+#include <stdexcept>
 
 #include "fv_device.h"
+
+// Minimum resident waves per SIMD requested from the register allocator for
+// the two stencil kernels (occupancy against spilling).
+#ifndef FVH_K1_MINW
+#define FVH_K1_MINW 1
+#endif
+#ifndef FVH_K2_MINW
+#define FVH_K2_MINW 1
+#endif
+// Face loops are unrolled to the element's face count (4 for tetrahedra, 6
+// for hexahedra), so the loads of all of a cell's faces can be in flight
+// together; FVH_FACE_UNROLL=0 keeps them rolled.
+#ifndef FVH_FACE_UNROLL
+#define FVH_FACE_UNROLL 1
+#endif
+// FVH_K2_GROUP selects the lane-group residual kernel (one face per lane).
+#ifndef FVH_K2_GROUP
+#define FVH_K2_GROUP 1
+#endif
+#if FVH_FACE_UNROLL
+#define FVH_FACE_PRAGMA _Pragma("unroll")
+#else
+#define FVH_FACE_PRAGMA _Pragma("unroll 1")
+#endif
 
 namespace fvh {
 namespace dev {
@@ -188,18 +213,97 @@ __device__ __forceinline__ void load_cons(const double *__restrict__ U, int N,
         phi[j] = U[j * N + c];
 }
 
+// Workgroup -> cell tile mapping.  The hardware hands consecutive
+// workgroups to the 8 XCDs in turn; this gives each XCD one contiguous range
+// of cell tiles so that the neighbour data a tile gathers is mostly in that
+// XCD's own L2.  A bijection on [0, nblk) for any nblk.
+__device__ __forceinline__ int xcd_tile(int nblk)
+{
+    const int b = blockIdx.x;
+    const int x = b & 7, i = b >> 3;
+    const int q = nblk >> 3, r = nblk & 7;
+    return x * q + (x < r ? x : r) + i;
+}
+
+// Viscous gradient subset: momentum (vars 1,2,3) and T (9) per direction,
+// v[d*4 + q] = d/dx_d of var {1,2,3,9}[q].
+struct ViscGrad
+{
+    double v[12];
+};
+
+__device__ __forceinline__ int visc_var(int q) { return q < 3 ? q + 1 : 9; }
+
+// Limited linear extrapolation to a face (reconstruction.c:193-200 with
+// limiter.c:86-105: min(1, y) of the per-face-side Barth-Jespersen ratio).
+__device__ __forceinline__ double limited(double phi, double gx, double gy,
+                                          double gz, const double *r, double mn,
+                                          double mx, int bj)
+{
+    const double slope = r[0] * gx + r[1] * gy + r[2] * gz;
+    double lim = 1.0;
+    if (bj)
+    {
+        double y = 1.0;
+        if (slope > 0)
+            y = (mx - phi) / slope;
+        else if (slope < 0)
+            y = (mn - phi) / slope;
+        lim = rmin(1.0, y);
+    }
+    return phi + lim * slope;
+}
+
 // ---------------------------------------------------------------------------
-// K1: Green-Gauss gradients of all 10 variables + neighbourhood min/max.
-// reconstruction.c:20-93 (gather form) and limiter.c:57-82.
+// Ghost rows N+b (boundary.c:188-320 and the update hook ns_update,
+// navier_stokes.c:199-209): the boundary-condition state of every boundary
+// face from its interior cell, as primitives + conservatives [10][B]; with
+// face states on, also side 1 of the boundary faces.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void fv_gradient_kernel(
-    DevMesh m, PhysDev P, const double *__restrict__ U, double *__restrict__ grad,
-    double *__restrict__ mnmx, int want_minmax, const DevScalars *sc)
+__global__ __launch_bounds__(kBlock) void fv_ghost_kernel(
+    DevMesh m, PhysDev P, const double *__restrict__ U, double *__restrict__ gphi,
+    double *__restrict__ fs, const DevScalars *sc)
 {
     if (sc && sc->done)
         return;
-    const int N = m.N;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= m.B)
+        return;
+    const int f = m.bnd_face[b];
+    const int c = m.fc0[f];
+    double pc[10], g[10];
+    load_cons(U, m.N, c, pc);
+    con_to_prim(P, pc);
+    Frame fr;
+    load_frame(m, f, fr);
+    ghost_state(m, P, b, fr, pc, g);
+#pragma unroll
+    for (int j = 0; j < 10; ++j)
+        gphi[(size_t)j * m.B + b] = g[j];
+    if (fs)
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            fs[(size_t)(8 + j) * m.F + f] = g[j];
+}
+
+// ---------------------------------------------------------------------------
+// K1: Green-Gauss gradients of all 10 variables and neighbourhood min/max of
+// one cell (reconstruction.c:20-93 in gather form, limiter.c:57-82), then
+// the cell's reconstructed state on each of its faces
+// (reconstruction.c:168-255 / 142-163) and the ghost state on its boundary
+// faces (boundary.c:188-320).  Writes the face states by face id and side,
+// and the gradient subset the viscous flux needs.
+// ---------------------------------------------------------------------------
+template <int MAXF>
+__global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
+    DevMesh m, PhysDev P, const double *__restrict__ U,
+    const double *__restrict__ gphi, GradOut out, int linear,
+    const DevScalars *sc)
+{
+    if (sc && sc->done)
+        return;
+    const int N = m.N, F = m.F;
+    const int c = xcd_tile(gridDim.x) * kBlock + threadIdx.x;
     if (c >= N)
         return;
 
@@ -219,15 +323,17 @@ __global__ __launch_bounds__(kBlock) void fv_gradient_kernel(
     }
 
     const int nf = m.ncf[c];
-    for (int s = 0; s < nf; ++s)
+    FVH_FACE_PRAGMA
+    for (int s = 0; s < MAXF; ++s)
     {
+        if (s >= nf)
+            break;
         const int f = m.cf[s * N + c];
         const int c0 = m.fc0[f];
         const int c1 = m.fc1[f];
         const bool own0 = c0 == c;
         const int nb = own0 ? c1 : c0;
-        Frame fr;
-        load_frame(m, f, fr);
+        const double n0 = m.fn[f], n1 = m.fn[F + f], n2 = m.fn[2 * F + f];
         const double lam = m.flam[f];
         const double area = m.farea[f];
 
@@ -239,7 +345,11 @@ __global__ __launch_bounds__(kBlock) void fv_gradient_kernel(
         }
         else
         {
-            ghost_state(m, P, nb - N, fr, pc, pn);
+            // boundary face: the ghost row written by fv_ghost_kernel
+            const int b = nb - N;
+#pragma unroll
+            for (int j = 0; j < 10; ++j)
+                pn[j] = gphi[(size_t)j * m.B + b];
         }
 
 #pragma unroll
@@ -248,9 +358,9 @@ __global__ __launch_bounds__(kBlock) void fv_gradient_kernel(
             const double p0 = own0 ? pc[j] : pn[j];
             const double p1 = own0 ? pn[j] : pc[j];
             const double pm = p0 + lam * (p1 - p0);
-            const double ax = pm * fr.n[0] * area;
-            const double ay = pm * fr.n[1] * area;
-            const double az = pm * fr.n[2] * area;
+            const double ax = pm * n0 * area;
+            const double ay = pm * n1 * area;
+            const double az = pm * n2 * area;
             if (own0)
             {
                 gx[j] += ax;
@@ -272,17 +382,68 @@ __global__ __launch_bounds__(kBlock) void fv_gradient_kernel(
 #pragma unroll
     for (int j = 0; j < 10; ++j)
     {
-        grad[(0 * 10 + j) * (size_t)N + c] = gx[j] * sv;
-        grad[(1 * 10 + j) * (size_t)N + c] = gy[j] * sv;
-        grad[(2 * 10 + j) * (size_t)N + c] = gz[j] * sv;
+        gx[j] *= sv;
+        gy[j] *= sv;
+        gz[j] *= sv;
     }
-    if (want_minmax)
+
+    if (out.vg)
+    {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+        {
+            const int j = visc_var(q);
+            out.vg[(size_t)(0 + q) * N + c] = gx[j];
+            out.vg[(size_t)(4 + q) * N + c] = gy[j];
+            out.vg[(size_t)(8 + q) * N + c] = gz[j];
+        }
+    }
+    if (out.grad)
     {
 #pragma unroll
         for (int j = 0; j < 10; ++j)
         {
-            mnmx[j * (size_t)N + c] = mn[j];
-            mnmx[(10 + j) * (size_t)N + c] = mx[j];
+            out.grad[(size_t)(0 + j) * N + c] = gx[j];
+            out.grad[(size_t)(10 + j) * N + c] = gy[j];
+            out.grad[(size_t)(20 + j) * N + c] = gz[j];
+        }
+    }
+    if (out.mnmx)
+    {
+#pragma unroll
+        for (int j = 0; j < 10; ++j)
+        {
+            out.mnmx[(size_t)j * N + c] = mn[j];
+            out.mnmx[(size_t)(10 + j) * N + c] = mx[j];
+        }
+    }
+
+    if (!out.fs)
+        return;
+    // this cell's reconstructed state on each of its faces
+    FVH_FACE_PRAGMA
+    for (int s = 0; s < MAXF; ++s)
+    {
+        if (s >= nf)
+            break;
+        const int f = m.cf[s * N + c];
+        const bool own0 = m.fc0[f] == c;
+        const int side = own0 ? 0 : 1;
+        double *dst = out.fs + (size_t)side * 8 * F + f;
+        if (linear)
+        {
+            const double *dd = own0 ? m.fd1 : m.fd2;
+            const double r[3] = {dd[f], dd[F + f], dd[2 * F + f]};
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                dst[(size_t)j * F] = limited(pc[j], gx[j], gy[j], gz[j], r, mn[j],
+                                             mx[j], P.bj);
+        }
+        else
+        {
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                dst[(size_t)j * F] = pc[j];
         }
     }
 }
@@ -371,13 +532,6 @@ __device__ __forceinline__ void riemann_ausm(const PhysDev &P, const double *l,
     f[4] += mm * c_r * r[0] * H_r;
 }
 
-// Gradient subset used by the viscous flux: momentum (vars 1,2,3) and T (9)
-// per direction: v[d*4 + {0,1,2,3}] = grad_d of vars {1,2,3,9}.
-struct ViscGrad
-{
-    double v[12];
-};
-
 // flux.c:215-263 with the reference's use of momentum gradients as velocity
 // gradients; phi holds the face state (reconstructed primitives at 5..7).
 __device__ __forceinline__ void viscous_1d(const PhysDev &P, const double *phi,
@@ -462,26 +616,6 @@ __device__ __forceinline__ void face_flux(const PhysDev &P, const Frame &fr,
     }
 }
 
-// Limited linear extrapolation to a face (reconstruction.c:193-200 with
-// limiter.c:86-105: min(1, y) of the per-face-side Barth-Jespersen ratio).
-__device__ __forceinline__ double limited(double phi, double gx, double gy,
-                                          double gz, const double *r, double mn,
-                                          double mx, int bj)
-{
-    const double slope = r[0] * gx + r[1] * gy + r[2] * gz;
-    double lim = 1.0;
-    if (bj)
-    {
-        double y = 1.0;
-        if (slope > 0)
-            y = (mx - phi) / slope;
-        else if (slope < 0)
-            y = (mn - phi) / slope;
-        lim = rmin(1.0, y);
-    }
-    return phi + lim * slope;
-}
-
 __device__ __forceinline__ void block_sum(double *sh, double &v)
 {
     const int t = threadIdx.x;
@@ -515,19 +649,21 @@ __device__ __forceinline__ void block_min(double *sh, double &v)
 }
 
 // ---------------------------------------------------------------------------
-// K2: residual dU/dt = -(sum_f +-flux_f A_f)/V of one cell, optionally fused
-// with the 2N-storage RK stage (explicit.c:113-140) and, in the last stage,
-// the residual-norm and time-step partial reductions (analyze.c:46-55,
-// navier_stokes.c:171-187).
+// K2: residual dU/dt = -(sum_f +-flux_f A_f)/V of one cell
+// (finite_volume.c:96-123), optionally fused with the 2N-storage RK stage
+// (explicit.c:113-140) and, in the last stage, the residual-norm and
+// time-step partial reductions (analyze.c:46-55, navier_stokes.c:171-187).
+// FS: face states from K1 (reconstructed path); otherwise first order, the
+// face states are the cell values taken from U.
 // ---------------------------------------------------------------------------
-template <int FLUX, bool LINEAR, bool VISC>
-__global__ __launch_bounds__(kBlock) void fv_residual_kernel(DevMesh m,
+template <int FLUX, bool FS, bool VISC, int MAXF>
+__global__ __launch_bounds__(kBlock, FVH_K2_MINW) void fv_residual_kernel(DevMesh m,
                                                              PhysDev P,
                                                              StageParams sp)
 {
     __shared__ double sh[kBlock];
-    const int N = m.N;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int N = m.N, F = m.F;
+    const int c = xcd_tile(gridDim.x) * kBlock + threadIdx.x;
 
     if (sp.sc && sp.sc->done)
     {
@@ -547,37 +683,21 @@ __global__ __launch_bounds__(kBlock) void fv_residual_kernel(DevMesh m,
         load_cons(sp.U_in, N, c, pc);
         con_to_prim(P, pc);
 
-        double gc[30];
-        double mnc[10], mxc[10];
-        if (LINEAR || VISC)
+        ViscGrad go;
+        if (VISC)
         {
 #pragma unroll
-            for (int q = 0; q < 30; ++q)
-                gc[q] = sp.grad[q * (size_t)N + c];
-        }
-        if (LINEAR && P.bj)
-        {
-#pragma unroll
-            for (int j = 0; j < 10; ++j)
-            {
-                mnc[j] = sp.mnmx[j * (size_t)N + c];
-                mxc[j] = sp.mnmx[(10 + j) * (size_t)N + c];
-            }
-        }
-        else
-        {
-#pragma unroll
-            for (int j = 0; j < 10; ++j)
-            {
-                mnc[j] = 0.0;
-                mxc[j] = 0.0;
-            }
+            for (int q = 0; q < 12; ++q)
+                go.v[q] = sp.vg[(size_t)q * N + c];
         }
 
         double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         const int nf = m.ncf[c];
-        for (int s = 0; s < nf; ++s)
+        FVH_FACE_PRAGMA
+        for (int s = 0; s < MAXF; ++s)
         {
+            if (s >= nf)
+                break;
             const int f = m.cf[s * N + c];
             const int c0 = m.fc0[f];
             const int c1 = m.fc1[f];
@@ -587,116 +707,57 @@ __global__ __launch_bounds__(kBlock) void fv_residual_kernel(DevMesh m,
             Frame fr;
             load_frame(m, f, fr);
 
-            // own side face state
-            double fo[10];
-            if (LINEAR)
-            {
-                double r[3];
-                const double *dd = own0 ? m.fd1 : m.fd2;
-#pragma unroll
-                for (int d = 0; d < 3; ++d)
-                    r[d] = dd[d * m.F + f];
-#pragma unroll
-                for (int j = 0; j < 10; ++j)
-                    fo[j] = limited(pc[j], gc[j], gc[10 + j], gc[20 + j], r,
-                                    mnc[j], mxc[j], P.bj);
-            }
-            else
+            double L[10], R[10];
+            if (FS)
             {
 #pragma unroll
-                for (int j = 0; j < 10; ++j)
-                    fo[j] = pc[j];
-            }
-
-            // neighbour side face state and gradient subset
-            double fn[10];
-            ViscGrad gn;
-            if (ghost)
-            {
-                ghost_state(m, P, nb - N, fr, pc, fn);
-                if (VISC)
+                for (int j = 0; j < 8; ++j)
                 {
-                    const int type = m.region_type[m.bnd_region[nb - N]];
-                    const int vars[4] = {1, 2, 3, 9};
-#pragma unroll
-                    for (int q = 0; q < 4; ++q)
-                    {
-                        const int j = vars[q];
-                        ghost_grad(type, fr, gc[j], gc[10 + j], gc[20 + j],
-                                   gn.v[q], gn.v[4 + q], gn.v[8 + q]);
-                    }
+                    L[j] = sp.fs[(size_t)j * F + f];
+                    R[j] = sp.fs[(size_t)(8 + j) * F + f];
                 }
+                L[8] = L[9] = R[8] = R[9] = 0.0;
             }
             else
             {
                 double pn[10];
-                load_cons(sp.U_in, N, nb, pn);
-                con_to_prim(P, pn);
-                if (LINEAR)
+                if (ghost)
                 {
-                    double r[3];
-                    const double *dd = own0 ? m.fd2 : m.fd1;
-#pragma unroll
-                    for (int d = 0; d < 3; ++d)
-                        r[d] = dd[d * m.F + f];
 #pragma unroll
                     for (int j = 0; j < 10; ++j)
-                    {
-                        const double gxj = sp.grad[(0 + j) * (size_t)N + nb];
-                        const double gyj = sp.grad[(10 + j) * (size_t)N + nb];
-                        const double gzj = sp.grad[(20 + j) * (size_t)N + nb];
-                        double mnj = 0.0, mxj = 0.0;
-                        if (P.bj)
-                        {
-                            mnj = sp.mnmx[j * (size_t)N + nb];
-                            mxj = sp.mnmx[(10 + j) * (size_t)N + nb];
-                        }
-                        fn[j] = limited(pn[j], gxj, gyj, gzj, r, mnj, mxj, P.bj);
-                    }
+                        pn[j] = sp.gphi[(size_t)j * m.B + (nb - N)];
+                }
+                else
+                {
+                    load_cons(sp.U_in, N, nb, pn);
+                    con_to_prim(P, pn);
+                }
+#pragma unroll
+                for (int j = 0; j < 10; ++j)
+                {
+                    L[j] = own0 ? pc[j] : pn[j];
+                    R[j] = own0 ? pn[j] : pc[j];
+                }
+            }
+
+            ViscGrad gL, gR;
+            if (VISC)
+            {
+                ViscGrad gn;
+                if (ghost)
+                {
+                    const int type = m.region_type[m.bnd_region[nb - N]];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        ghost_grad(type, fr, go.v[q], go.v[4 + q], go.v[8 + q],
+                                   gn.v[q], gn.v[4 + q], gn.v[8 + q]);
                 }
                 else
                 {
 #pragma unroll
-                    for (int j = 0; j < 10; ++j)
-                        fn[j] = pn[j];
+                    for (int q = 0; q < 12; ++q)
+                        gn.v[q] = sp.vg[(size_t)q * N + nb];
                 }
-                if (VISC)
-                {
-                    const int vars[4] = {1, 2, 3, 9};
-#pragma unroll
-                    for (int q = 0; q < 4; ++q)
-                    {
-                        const int j = vars[q];
-                        gn.v[q] = sp.grad[(0 + j) * (size_t)N + nb];
-                        gn.v[4 + q] = sp.grad[(10 + j) * (size_t)N + nb];
-                        gn.v[8 + q] = sp.grad[(20 + j) * (size_t)N + nb];
-                    }
-                }
-            }
-
-            ViscGrad go;
-            if (VISC)
-            {
-                const int vars[4] = {1, 2, 3, 9};
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                {
-                    go.v[q] = gc[vars[q]];
-                    go.v[4 + q] = gc[10 + vars[q]];
-                    go.v[8 + q] = gc[20 + vars[q]];
-                }
-            }
-
-            double L[10], R[10];
-            ViscGrad gL, gR;
-#pragma unroll
-            for (int j = 0; j < 10; ++j)
-            {
-                L[j] = own0 ? fo[j] : fn[j];
-                R[j] = own0 ? fn[j] : fo[j];
-            }
-            if (VISC)
-            {
 #pragma unroll
                 for (int q = 0; q < 12; ++q)
                 {
@@ -752,6 +813,205 @@ __global__ __launch_bounds__(kBlock) void fv_residual_kernel(DevMesh m,
                     part[j] = rabs(R5[j]) * vol;
                 // time step with the new density and the primitives of
                 // this stage's input (navier_stokes.c:171-187)
+                const double dx0 = m.cdx[c], dx1 = m.cdx[N + c],
+                             dx2 = m.cdx[2 * (size_t)N + c];
+                const double s_rho = 1.0 / unew[0];
+                const double cs = sqrt(P.kappa * pc[8] * s_rho);
+                const double ds1 = (rabs(pc[5]) + cs) * dx0 +
+                                   (rabs(pc[6]) + cs) * dx1 +
+                                   (rabs(pc[7]) + cs) * dx2;
+                part[5] = vol / ds1;
+                const double ds2 = dot3(dx0, dx1, dx2, dx0, dx1, dx2);
+                part[6] = s_rho * ds2 / vol;
+            }
+        }
+    }
+
+    if (sp.do_reduce)
+    {
+#pragma unroll
+        for (int q = 0; q < 5; ++q)
+            block_sum(sh, part[q]);
+        block_min(sh, part[5]);
+        block_min(sh, part[6]);
+        if (threadIdx.x == 0)
+        {
+#pragma unroll
+            for (int q = 0; q < 7; ++q)
+                sp.partials[blockIdx.x * kPartialStride + q] = part[q];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K2 in lane groups: LANES consecutive lanes per cell, one face per lane
+// (LANES = 4 for tetrahedra, 8 for hexahedra).  Each lane evaluates one face
+// flux; the group's contributions are summed in ascending face order with
+// lane shuffles (acc - x is acc + (-x) exactly, so the sum is bit-identical
+// to the face-ordered scatter of finite_volume.c:100-112), and the group's
+// first lane runs the cell epilogue.  Same results as fv_residual_kernel
+// with a quarter of the per-lane state, so more waves hide the gathers.
+// ---------------------------------------------------------------------------
+template <int FLUX, bool FS, bool VISC, int LANES>
+__global__ __launch_bounds__(kBlock, FVH_K2_MINW) void fv_residual_group_kernel(
+    DevMesh m, PhysDev P, StageParams sp)
+{
+    __shared__ double sh[kBlock];
+    const int N = m.N, F = m.F;
+    const int tid = xcd_tile(gridDim.x) * kBlock + threadIdx.x;
+    const int c = tid / LANES;
+    const int s = tid & (LANES - 1);
+    const bool cell_ok = c < N;
+
+    if (sp.sc && sp.sc->done)
+    {
+        if (sp.do_update && cell_ok && s == 0)
+#pragma unroll
+            for (int j = 0; j < 5; ++j)
+                sp.U_out[j * (size_t)N + c] = sp.U_in[j * (size_t)N + c];
+        return;
+    }
+
+    const int nf = cell_ok ? m.ncf[c] : 0;
+    double contrib[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (s < nf)
+    {
+        const int f = m.cf[s * N + c];
+        const int c0 = m.fc0[f];
+        const int c1 = m.fc1[f];
+        const bool own0 = c0 == c;
+        const int nb = own0 ? c1 : c0;
+        const bool ghost = nb >= N;
+        Frame fr;
+        load_frame(m, f, fr);
+
+        double L[10], R[10];
+        if (FS)
+        {
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+            {
+                L[j] = sp.fs[(size_t)j * F + f];
+                R[j] = sp.fs[(size_t)(8 + j) * F + f];
+            }
+            L[8] = L[9] = R[8] = R[9] = 0.0;
+        }
+        else
+        {
+            double pc[10], pn[10];
+            load_cons(sp.U_in, N, c, pc);
+            con_to_prim(P, pc);
+            if (ghost)
+            {
+#pragma unroll
+                for (int j = 0; j < 10; ++j)
+                    pn[j] = sp.gphi[(size_t)j * m.B + (nb - N)];
+            }
+            else
+            {
+                load_cons(sp.U_in, N, nb, pn);
+                con_to_prim(P, pn);
+            }
+#pragma unroll
+            for (int j = 0; j < 10; ++j)
+            {
+                L[j] = own0 ? pc[j] : pn[j];
+                R[j] = own0 ? pn[j] : pc[j];
+            }
+        }
+
+        ViscGrad gL, gR;
+        if (VISC)
+        {
+            ViscGrad go, gn;
+#pragma unroll
+            for (int q = 0; q < 12; ++q)
+                go.v[q] = sp.vg[(size_t)q * N + c];
+            if (ghost)
+            {
+                const int type = m.region_type[m.bnd_region[nb - N]];
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    ghost_grad(type, fr, go.v[q], go.v[4 + q], go.v[8 + q],
+                               gn.v[q], gn.v[4 + q], gn.v[8 + q]);
+            }
+            else
+            {
+#pragma unroll
+                for (int q = 0; q < 12; ++q)
+                    gn.v[q] = sp.vg[(size_t)q * N + nb];
+            }
+#pragma unroll
+            for (int q = 0; q < 12; ++q)
+            {
+                gL.v[q] = own0 ? go.v[q] : gn.v[q];
+                gR.v[q] = own0 ? gn.v[q] : go.v[q];
+            }
+        }
+
+        double flux[5];
+        face_flux<FLUX, VISC>(P, fr, L, R, gL, gR, flux);
+        const double area = m.farea[f];
+#pragma unroll
+        for (int j = 0; j < 5; ++j)
+        {
+            const double v = flux[j] * area;
+            contrib[j] = own0 ? v : -v;
+        }
+    }
+
+    // ordered sum over the group's faces
+    double acc[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+    {
+        double a = 0.0;
+#pragma unroll
+        for (int k = 0; k < LANES; ++k)
+        {
+            const double v = __shfl(contrib[j], k, LANES);
+            if (k < nf)
+                a = a + v;
+        }
+        acc[j] = a;
+    }
+
+    double part[7] = {0.0, 0.0, 0.0, 0.0, 0.0, DBL_MAX, DBL_MAX};
+    if (cell_ok && s == 0)
+    {
+        const double vol = m.vol[c];
+        double R5[5];
+#pragma unroll
+        for (int j = 0; j < 5; ++j)
+            R5[j] = -acc[j] / vol;
+        if (sp.R_out)
+        {
+#pragma unroll
+            for (int j = 0; j < 5; ++j)
+                sp.R_out[j * (size_t)N + c] = R5[j];
+        }
+        if (sp.do_update)
+        {
+            double pc[10];
+            load_cons(sp.U_in, N, c, pc);
+            const double dt = sp.sc->dt;
+            double unew[5];
+#pragma unroll
+            for (int j = 0; j < 5; ++j)
+            {
+                double kj = R5[j];
+                if (!sp.stage0)
+                    kj = R5[j] + sp.k[j * (size_t)N + c] * sp.rk_a;
+                sp.k[j * (size_t)N + c] = kj;
+                unew[j] = pc[j] + kj * dt * sp.rk_g;
+                sp.U_out[j * (size_t)N + c] = unew[j];
+            }
+            if (sp.do_reduce)
+            {
+                con_to_prim(P, pc);
+#pragma unroll
+                for (int j = 0; j < 5; ++j)
+                    part[j] = rabs(R5[j]) * vol;
                 const double dx0 = m.cdx[c], dx1 = m.cdx[N + c],
                              dx2 = m.cdx[2 * (size_t)N + c];
                 const double s_rho = 1.0 / unew[0];
@@ -966,25 +1226,72 @@ void launch_mirror(hipStream_t s, const DevMesh &m, const PhysDev &P,
                        phi_dt_aos);
 }
 
-void launch_gradient(hipStream_t s, const DevMesh &m, const PhysDev &P,
-                     const double *U, double *grad, double *mnmx,
-                     int want_minmax, const DevScalars *sc)
+void launch_ghost(hipStream_t s, const DevMesh &m, const PhysDev &P,
+                  const double *U, double *gphi, double *fs,
+                  const DevScalars *sc)
 {
-    hipLaunchKernelGGL(fv_gradient_kernel, dim3(cell_blocks(m.N)), dim3(kBlock),
-                       0, s, m, P, U, grad, mnmx, want_minmax, sc);
+    if (m.B == 0)
+        return;
+    hipLaunchKernelGGL(fv_ghost_kernel, dim3(cell_blocks(m.B)), dim3(kBlock), 0,
+                       s, m, P, U, gphi, fs, sc);
 }
 
-template <int FLUX, bool LINEAR, bool VISC>
+void launch_gradient(hipStream_t s, const DevMesh &m, const PhysDev &P,
+                     const double *U, const double *gphi, const GradOut &out,
+                     int linear, const DevScalars *sc)
+{
+    if (m.maxf > 6)
+        throw std::runtime_error("cells with more than 6 faces are not supported");
+    if (m.maxf <= 4)
+        hipLaunchKernelGGL(fv_gradient_kernel<4>, dim3(cell_blocks(m.N)),
+                           dim3(kBlock), 0, s, m, P, U, gphi, out, linear, sc);
+    else
+        hipLaunchKernelGGL(fv_gradient_kernel<6>, dim3(cell_blocks(m.N)),
+                           dim3(kBlock), 0, s, m, P, U, gphi, out, linear, sc);
+}
+
+int residual_grid(const DevMesh &m)
+{
+    if (FVH_K2_GROUP)
+        return cell_blocks(m.N * (m.maxf <= 4 ? 4 : 8));
+    return cell_blocks(m.N);
+}
+
+int dt_partials_grid(const DevMesh &m) { return cell_blocks(m.N); }
+
+template <int FLUX, bool FS, bool VISC>
 static void launch_res_t(hipStream_t s, const DevMesh &m, const PhysDev &P,
                          const StageParams &sp)
 {
-    hipLaunchKernelGGL((fv_residual_kernel<FLUX, LINEAR, VISC>),
-                       dim3(cell_blocks(m.N)), dim3(kBlock), 0, s, m, P, sp);
+    if (FVH_K2_GROUP)
+    {
+        if (m.maxf <= 4)
+            hipLaunchKernelGGL((fv_residual_group_kernel<FLUX, FS, VISC, 4>),
+                               dim3(residual_grid(m)), dim3(kBlock), 0, s, m, P,
+                               sp);
+        else
+            hipLaunchKernelGGL((fv_residual_group_kernel<FLUX, FS, VISC, 8>),
+                               dim3(residual_grid(m)), dim3(kBlock), 0, s, m, P,
+                               sp);
+        return;
+    }
+    if (m.maxf <= 4)
+        hipLaunchKernelGGL((fv_residual_kernel<FLUX, FS, VISC, 4>),
+                           dim3(cell_blocks(m.N)), dim3(kBlock), 0, s, m, P, sp);
+    else
+        hipLaunchKernelGGL((fv_residual_kernel<FLUX, FS, VISC, 6>),
+                           dim3(cell_blocks(m.N)), dim3(kBlock), 0, s, m, P, sp);
 }
 
 void launch_residual(hipStream_t s, const DevMesh &m, const PhysDev &P,
-                     const StageParams &sp, int flux, bool linear, bool viscous)
+                     const StageParams &sp, int flux, bool face_states,
+                     bool viscous)
 {
+    if (m.maxf > 6)
+        throw std::runtime_error("cells with more than 6 faces are not supported");
+    if (sp.do_reduce && residual_grid(m) > sp.partials_cap)
+        throw std::runtime_error("reduction partials buffer too small");
+    const bool linear = face_states;
     if (flux == FluxRusanov)
     {
         if (linear)
@@ -1006,8 +1313,10 @@ void launch_residual(hipStream_t s, const DevMesh &m, const PhysDev &P,
 }
 
 void launch_dt_partials(hipStream_t s, const DevMesh &m, const PhysDev &P,
-                        const double *U, double *partials)
+                        const double *U, double *partials, int partials_cap)
 {
+    if (dt_partials_grid(m) > partials_cap)
+        throw std::runtime_error("reduction partials buffer too small");
     hipLaunchKernelGGL(fv_dt_partials_kernel, dim3(cell_blocks(m.N)),
                        dim3(kBlock), 0, s, m, P, U, partials);
 }

@@ -147,9 +147,13 @@ struct Device
     int cur = 0;
     double *k = nullptr;
     double *R = nullptr;
-    double *grad = nullptr;
-    double *mnmx = nullptr;
+    double *gphi = nullptr; // [10*B] ghost rows
+    double *vg = nullptr;   // [12*N] viscous gradient subset
+    double *fs = nullptr;   // [16*F] face states
+    double *grad = nullptr; // [30*N] all gradients (tests only)
+    double *mnmx = nullptr; // [20*N] min/max (tests only)
     double *partials = nullptr;
+    int partials_cap = 0; // rows of partials
     dev::DevScalars *sc = nullptr;
     dev::DevScalars *sc_host = nullptr; // pinned
     double *log = nullptr;
@@ -259,11 +263,19 @@ void require_ready()
 
 void ensure_gradient_buffers()
 {
-    size_t N = (size_t)g_dev.m.N;
-    if (needs_gradients() && !g_dev.grad)
-        g_dev.grad = dmalloc<double>(30 * N);
-    if (is_linear() && g_limiter == BarthJespersenn && !g_dev.mnmx)
-        g_dev.mnmx = dmalloc<double>(20 * N);
+    size_t N = (size_t)g_dev.m.N, F = (size_t)g_dev.m.F;
+    if (is_viscous() && !g_dev.vg)
+        g_dev.vg = dmalloc<double>(12 * N);
+    if (is_linear() && !g_dev.fs)
+        g_dev.fs = dmalloc<double>(16 * F);
+}
+
+dev::GradOut grad_out()
+{
+    dev::GradOut o{};
+    o.vg = is_viscous() ? g_dev.vg : nullptr;
+    o.fs = is_linear() ? g_dev.fs : nullptr;
+    return o;
 }
 
 // Residual of U_in into R (and, for a stage, into U_out/k).
@@ -272,16 +284,19 @@ void enqueue_residual(const double *U_in, double *U_out, double *R_out,
 {
     const dev::PhysDev P = phys();
     ensure_gradient_buffers();
+    dev::launch_ghost(g_dev.stream, g_dev.m, P, U_in, g_dev.gphi,
+                      is_linear() ? g_dev.fs : nullptr, sc);
     if (needs_gradients())
-        dev::launch_gradient(g_dev.stream, g_dev.m, P, U_in, g_dev.grad,
-                             g_dev.mnmx, P.bj, sc);
+        dev::launch_gradient(g_dev.stream, g_dev.m, P, U_in, g_dev.gphi,
+                             grad_out(), is_linear(), sc);
     dev::StageParams sp{};
     sp.U_in = U_in;
     sp.U_out = U_out;
     sp.k = g_dev.k;
     sp.R_out = R_out;
-    sp.grad = g_dev.grad;
-    sp.mnmx = g_dev.mnmx;
+    sp.vg = g_dev.vg;
+    sp.fs = g_dev.fs;
+    sp.gphi = g_dev.gphi;
     sp.rk_a = update ? g_rk_a[stage] : 0.0;
     sp.rk_g = update ? g_rk_g[stage] : 0.0;
     sp.stage0 = stage == 0;
@@ -289,6 +304,7 @@ void enqueue_residual(const double *U_in, double *U_out, double *R_out,
     sp.do_reduce = reduce;
     sp.sc = sc;
     sp.partials = g_dev.partials;
+    sp.partials_cap = g_dev.partials_cap;
     dev::launch_residual(g_dev.stream, g_dev.m, P, sp, g_flux, is_linear(),
                          is_viscous());
 }
@@ -612,7 +628,7 @@ void enqueue_explicit_step(dev::DevScalars *sc)
         g_dev.cur = 1 - g_dev.cur;
     }
     dev::launch_finalize(g_dev.stream, sc, g_dev.partials,
-                         dev::cell_blocks(g_dev.m.N), phys(), 1, g_dev.log,
+                         dev::residual_grid(g_dev.m), phys(), 1, g_dev.log,
                          g_dev.log_cap);
 }
 
@@ -634,9 +650,9 @@ void init_scalars(int iter0, double t0, int max_iter, bool transient,
                              g_dev.stream));
     // initial dt from the current state (primitives refreshed from U)
     dev::launch_dt_partials(g_dev.stream, g_dev.m, phys(), g_dev.U[g_dev.cur],
-                            g_dev.partials);
+                            g_dev.partials, g_dev.partials_cap);
     dev::launch_finalize(g_dev.stream, g_dev.sc, g_dev.partials,
-                         dev::cell_blocks(g_dev.m.N), phys(), 0, nullptr, 0);
+                         dev::dt_partials_grid(g_dev.m), phys(), 0, nullptr, 0);
 }
 
 void read_scalars()
@@ -956,8 +972,10 @@ void init_finite_volume(void)
         g_dev.U[1] = dmalloc<double>((size_t)5 * N);
         g_dev.k = dmalloc<double>((size_t)5 * N);
         g_dev.R = dmalloc<double>((size_t)5 * N);
-        g_dev.partials = dmalloc<double>((size_t)dev::cell_blocks(N + B) *
+        g_dev.partials_cap = std::max(dev::residual_grid(m), dev::dt_partials_grid(m));
+        g_dev.partials = dmalloc<double>((size_t)g_dev.partials_cap *
                                          dev::kPartialStride);
+        g_dev.gphi = dmalloc<double>((size_t)10 * B);
         g_dev.sc = dmalloc<dev::DevScalars>(1);
         HIP_CHECK(hipHostMalloc((void **)&g_dev.sc_host, sizeof(dev::DevScalars),
                                 hipHostMallocDefault));
@@ -1371,14 +1389,15 @@ double calc_ns_timestep(double time)
     {
         require_ready();
         dev::launch_dt_partials(g_dev.stream, g_dev.m, phys(),
-                                g_dev.U[g_dev.cur], g_dev.partials);
+                                g_dev.U[g_dev.cur], g_dev.partials,
+                                g_dev.partials_cap);
         dev::DevScalars &h = *g_dev.sc_host;
         std::memset(&h, 0, sizeof(h));
         h.t_end = DBL_MAX;
         HIP_CHECK(hipMemcpyAsync(g_dev.sc, &h, sizeof(h), hipMemcpyHostToDevice,
                                  g_dev.stream));
         dev::launch_finalize(g_dev.stream, g_dev.sc, g_dev.partials,
-                             dev::cell_blocks(g_dev.m.N), phys(), 0, nullptr, 0);
+                             dev::dt_partials_grid(g_dev.m), phys(), 0, nullptr, 0);
         read_scalars();
         set_viscous_dt(h.viscous);
         return h.dt;
@@ -1590,10 +1609,24 @@ int fvh_get_field(int what, double *out)
         if (!g_dev.ready)
             throw std::runtime_error("no device state");
         const int N = g_dev.m.N;
-        const double *src = what == 0 ? g_dev.grad : (what == 1 ? g_dev.mnmx : nullptr);
+        if (what != 0 && what != 1)
+            throw std::runtime_error("unknown field");
+        // recompute the gradients and min/max of the current state with the
+        // test outputs switched on
+        if (!g_dev.grad)
+            g_dev.grad = dmalloc<double>((size_t)30 * N);
+        if (!g_dev.mnmx)
+            g_dev.mnmx = dmalloc<double>((size_t)20 * N);
+        ensure_gradient_buffers();
+        dev::GradOut o = grad_out();
+        o.grad = g_dev.grad;
+        o.mnmx = g_dev.mnmx;
+        dev::launch_ghost(g_dev.stream, g_dev.m, phys(), g_dev.U[g_dev.cur],
+                          g_dev.gphi, o.fs, nullptr);
+        dev::launch_gradient(g_dev.stream, g_dev.m, phys(), g_dev.U[g_dev.cur],
+                             g_dev.gphi, o, is_linear(), nullptr);
+        const double *src = what == 0 ? g_dev.grad : g_dev.mnmx;
         const int rows = what == 0 ? 30 : 20;
-        if (!src)
-            throw std::runtime_error("field not computed by the current scheme");
         HIP_CHECK(hipStreamSynchronize(g_dev.stream));
         std::vector<double> soa((size_t)rows * N);
         HIP_CHECK(hipMemcpy(soa.data(), src, soa.size() * 8, hipMemcpyDeviceToHost));
@@ -1694,11 +1727,14 @@ int fvh_bench_explicit(int n_steps, int n_warmup, double *seconds,
                     double *Uin = g_dev.U[g_dev.cur];
                     double *Uout = g_dev.U[1 - g_dev.cur];
                     ensure_gradient_buffers();
+                    dev::launch_ghost(g_dev.stream, g_dev.m, P, Uin, g_dev.gphi,
+                                      is_linear() ? g_dev.fs : nullptr, g_dev.sc);
                     if (needs_gradients())
                     {
                         HIP_CHECK(hipEventRecord(a, g_dev.stream));
                         dev::launch_gradient(g_dev.stream, g_dev.m, P, Uin,
-                                             g_dev.grad, g_dev.mnmx, P.bj, g_dev.sc);
+                                             g_dev.gphi, grad_out(), is_linear(),
+                                             g_dev.sc);
                         HIP_CHECK(hipEventRecord(b, g_dev.stream));
                         HIP_CHECK(hipEventSynchronize(b));
                         float t = 0;
@@ -1710,8 +1746,9 @@ int fvh_bench_explicit(int n_steps, int n_warmup, double *seconds,
                     sp.U_out = Uout;
                     sp.k = g_dev.k;
                     sp.R_out = last ? g_dev.R : nullptr;
-                    sp.grad = g_dev.grad;
-                    sp.mnmx = g_dev.mnmx;
+                    sp.vg = g_dev.vg;
+                    sp.fs = g_dev.fs;
+                    sp.gphi = g_dev.gphi;
                     sp.rk_a = g_rk_a[s];
                     sp.rk_g = g_rk_g[s];
                     sp.stage0 = s == 0;
@@ -1719,6 +1756,7 @@ int fvh_bench_explicit(int n_steps, int n_warmup, double *seconds,
                     sp.do_reduce = last;
                     sp.sc = g_dev.sc;
                     sp.partials = g_dev.partials;
+                    sp.partials_cap = g_dev.partials_cap;
                     HIP_CHECK(hipEventRecord(a, g_dev.stream));
                     dev::launch_residual(g_dev.stream, g_dev.m, P, sp, g_flux,
                                          is_linear(), is_viscous());
@@ -1731,7 +1769,7 @@ int fvh_bench_explicit(int n_steps, int n_warmup, double *seconds,
                 }
                 HIP_CHECK(hipEventRecord(a, g_dev.stream));
                 dev::launch_finalize(g_dev.stream, g_dev.sc, g_dev.partials,
-                                     dev::cell_blocks(g_dev.m.N), P, 1, g_dev.log,
+                                     dev::residual_grid(g_dev.m), P, 1, g_dev.log,
                                      g_dev.log_cap);
                 HIP_CHECK(hipEventRecord(b, g_dev.stream));
                 HIP_CHECK(hipEventSynchronize(b));
