@@ -41,9 +41,14 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=96, help="hexahedra per side")
-    ap.add_argument("--cpu-sample-n", type=int, default=40)
-    ap.add_argument("--cpu-sample-steps", type=int, default=2)
+    ap.add_argument("--tile", type=int, default=8, help="hexahedra per side of a "
+                    "cell-ordering tile of the mesh")
+    ap.add_argument("--cpu-sample-n", type=int, default=64)
+    ap.add_argument("--cpu-sample-steps", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 1 GiB device copy (torch) whose HBM "
+                         "byte counters calibrate FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -55,7 +60,7 @@ def main():
     from fv3d_amd import solver as S
 
     t0 = time.time()
-    cfg = configs.taylor_green(n=args.n, steps=args.steps + args.warmup + 100)
+    cfg = configs.taylor_green(n=args.n, steps=args.steps + args.warmup + 100, tile=args.tile)
     case = S.Case(**cfg)
     print("mesh + init: %.1f s, %d cells, %d faces" % (time.time() - t0, case.n_cells,
                                                     case.n_faces), file=sys.stderr, flush=True)
@@ -79,6 +84,24 @@ def main():
              "step": {"achieved": round(step_gbs, 1), "bytes_per_cell_stage": B_CELL_STAGE,
                       "frac": round(step_gbs / PEAK_HBM_GBS, 4)}}
     case.close()
+
+    if args.copy_calibration:
+        import torch
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(5):
+            dst.copy_(src)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 5
+        print("device copy 1 GiB: %.3f ms, %.0f GB/s read+write" % (ms, 2 * (1 << 30) / ms / 1e6),
+              file=sys.stderr, flush=True)
+        del src, dst
 
     cpu = None
     if not args.no_cpu_baseline:

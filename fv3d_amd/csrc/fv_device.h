@@ -1,21 +1,25 @@
 // Device-side data structures and kernel launchers of libfv3d_hip.so
 // (internal; the public C-ABI is include/fv3d_solver.h + fv3d_hip.h).
 //
-// HBM layout (structure of arrays, FP64 values, int32 indices):
-//   cell arrays [N]   : U[5][N] (ping-pong pair), k[5][N] (RK register),
-//                       vg[12][N] (gradients of rho*u, rho*v, rho*w and T
-//                       per direction, row d*4 + q), vol, svol = 1/vol,
-//                       dx[3][N], ncf, cf[maxf][N] (face ids per slot,
-//                       ascending, -1 padded)
-//   face-state array  : fs[2][8][F]: the reconstructed (limited) state of
-//                       variables 0..7 on side 0 (cells[0]) and side 1
-//                       (cells[1] or the ghost) of every face
-//   ghost rows        : gphi[10][B] (boundary-condition states)
-//   test-only outputs : grad[30][N] ((dim*10 + var) rows), mnmx[20][N]
-//   face arrays [F]   : fc0, fc1 (fc1 >= N is ghost row N + boundary),
-//                       n, t1, t2, d1 = x_f - x_c0, d2 = x_f - x_c1 image
-//                       ([3][F] each), area, lambda
-//   boundary arrays   : bnd_region[B]; region_type[R], region_phi[R*10]
+// HBM layout.  Everything a stencil kernel gathers through an index is a
+// per-element record, so one gather touches one or two cache lines; FP64
+// values, int32 indices.
+//   cell records      : U[N][5] (ping-pong pair), k[N][5] (RK register),
+//                       R[N][5] (dU/dt), vg[N][12] (gradients of rho*u,
+//                       rho*v, rho*w and T per direction, d*4 + q),
+//                       cdx[N][3], cx[N][3] (centroids), cf[N][maxf]
+//                       (face ids, ascending, -1
+//                       padded); vol[N], svol[N] = 1/vol, ncf[N]
+//   face records      : fs[F][16] (reconstructed state of variables 0..7
+//                       on side 0 = cells[0], then side 1 = cells[1] or the
+//                       ghost), fgA[F][10] (n, t1, t2, area: flux kernel),
+//                       fgB[F][8] (n, area, lambda, face centre x_f:
+//                       gradient kernel), fsh[F] (index into shtab[][3],
+//                       the exact periodic shifts; 0 = none), fcell[F][2]
+//                       (cells[1] >= N is the ghost row N + boundary)
+//   boundary records  : gphi[B][10] (ghost rows), bnd_region[B],
+//                       bnd_face[B]; region_type[R], region_phi[R][10]
+//   test-only outputs : grad[N][30] (d*10 + var), mnmx[N][20]
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -23,23 +27,23 @@
 namespace fvh {
 namespace dev {
 
+constexpr int kFgA = 10; // doubles per flux-kernel face record
+constexpr int kFgB = 8;  // doubles per gradient-kernel face record
+
 struct DevMesh
 {
     int N, F, B, maxf;
-    const int *__restrict__ cf;
-    const int *__restrict__ ncf;
-    const int *__restrict__ fc0;
-    const int *__restrict__ fc1;
-    const double *__restrict__ fn;  // [3*F]
-    const double *__restrict__ ft1; // [3*F]
-    const double *__restrict__ ft2; // [3*F]
-    const double *__restrict__ fd1; // [3*F]
-    const double *__restrict__ fd2; // [3*F]
-    const double *__restrict__ farea;
-    const double *__restrict__ flam;
+    const int *__restrict__ cf;     // [N*maxf]
+    const int *__restrict__ ncf;    // [N]
+    const int *__restrict__ fcell;  // [F*2]
+    const double *__restrict__ fgA; // [F*kFgA]
+    const double *__restrict__ fgB; // [F*kFgB]
+    const int *__restrict__ fsh;    // [F]
+    const double *__restrict__ shtab; // [nsh*3]
+    const double *__restrict__ cx;  // [N*3]
     const double *__restrict__ vol;
     const double *__restrict__ svol;
-    const double *__restrict__ cdx; // [3*N]
+    const double *__restrict__ cdx; // [N*3]
     const int *__restrict__ bnd_region;
     const int *__restrict__ bnd_face;
     const int *__restrict__ region_type;
@@ -66,10 +70,10 @@ struct DevScalars
 // Outputs of the gradient/reconstruction kernel (null pointers are skipped).
 struct GradOut
 {
-    double *vg;   // [12*N] viscous gradient subset
-    double *fs;   // [16*F] face states
-    double *grad; // [30*N] all gradients (tests)
-    double *mnmx; // [20*N] neighbourhood min/max (tests)
+    double *vg;   // [N*12] viscous gradient subset
+    double *fs;   // [F*16] face states
+    double *grad; // [N*30] all gradients (tests)
+    double *mnmx; // [N*20] neighbourhood min/max (tests)
 };
 
 struct StageParams
@@ -78,9 +82,9 @@ struct StageParams
     double *U_out;
     double *k;
     double *R_out;         // may be null
-    const double *vg;      // [12*N] when viscous
-    const double *fs;      // [16*F] face states (reconstructed path)
-    const double *gphi;    // [10*B] ghost rows (first-order path)
+    const double *vg;      // [N*12] when viscous
+    const double *fs;      // [F*16] face states (reconstructed path)
+    const double *gphi;    // [B*10] ghost rows (first-order path)
     double rk_a, rk_g;
     int stage0;            // first stage: k = R
     int do_update;         // RK stage update into U_out / k
@@ -106,7 +110,7 @@ enum FluxKind
     FluxRusanov = 1
 };
 
-// Ghost rows gphi[10][B] of the current state (and side 1 of the boundary
+// Ghost rows gphi[B][10] of the current state (and side 1 of the boundary
 // faces in fs when fs is not null).  No-op without boundaries.
 void launch_ghost(hipStream_t s, const DevMesh &m, const PhysDev &P,
                   const double *U, double *gphi, double *fs,
@@ -127,10 +131,13 @@ void launch_dt_partials(hipStream_t s, const DevMesh &m, const PhysDev &P,
 void launch_mirror(hipStream_t s, const DevMesh &m, const PhysDev &P,
                    const double *U, const double *R, double *phi_total_aos,
                    double *phi_dt_aos);
-// mode 0: initial dt only; mode 1: end of step bookkeeping + next dt
+// Rows of the first-level fold of n_partials partials rows.
+int fold_rows(int n_partials);
+// Folds the partials (into fold, fold_cap rows) and runs the time-loop
+// bookkeeping.  mode 0: initial dt only; mode 1: end of step + next dt.
 void launch_finalize(hipStream_t s, DevScalars *sc, const double *partials,
-                     int n_partials, const PhysDev &P, int mode,
-                     double *log, int log_cap);
+                     int n_partials, double *fold, int fold_cap,
+                     const PhysDev &P, int mode, double *log, int log_cap);
 
 } // namespace dev
 } // namespace fvh

@@ -1,13 +1,25 @@
 // CDNA4 kernels of the residual and explicit time-stepping path.
 //
-// One thread per cell; each cell gathers over its faces in ascending face id
-// (the order of the reference's face-ordered scatter,
-// src/Solver/src/finite_volume/finite_volume.c:100-112), so the per-cell
-// sums match the reference's exactly and no atomics are used: results are
-// bitwise reproducible run to run.  Every face flux is evaluated by both of
-// its cells (same inputs, same operations, so both get the same bits); this
-// trades FP64 ALU work, which is plentiful, for the flux[F][5] and face-state
-// arrays the reference streams through memory.
+// Per RK stage (or residual evaluation):
+//   fv_ghost_kernel     ghost rows of the boundary faces (B threads)
+//   fv_gradient_kernel  one thread per cell: Green-Gauss gradients and
+//                       neighbourhood min/max, then the cell's limited state
+//                       on each of its faces, written by face id and side
+//   fv_residual_group_kernel
+//                       LANES lanes per cell, one face per lane: the face
+//                       flux from the two stored face states, summed in
+//                       ascending face order, then the fused RK stage and,
+//                       in the last stage, the residual-norm and time-step
+//                       partials
+//   fv_finalize_kernel  one block: the time loop's bookkeeping
+//
+// Every cell gathers over its own faces in ascending face id (the order of
+// the reference's face-ordered scatter, src/Solver/src/finite_volume/
+// finite_volume.c:100-112), so its sums equal the reference's exactly and
+// no atomics are used: results are bitwise reproducible run to run.  Each
+// face flux is evaluated by both of its cells (same inputs, same
+// operations, same bits), which keeps the flux[F][5] array of the reference
+// out of HBM.
 //
 // Arithmetic follows the reference operation by operation and is compiled
 // with -ffp-contract=off, so IEEE-rounded +,-,*,/ and sqrt give the same bits
@@ -26,20 +38,10 @@
 #ifndef FVH_K2_MINW
 #define FVH_K2_MINW 1
 #endif
-// Face loops are unrolled to the element's face count (4 for tetrahedra, 6
-// for hexahedra), so the loads of all of a cell's faces can be in flight
-// together; FVH_FACE_UNROLL=0 keeps them rolled.
-#ifndef FVH_FACE_UNROLL
-#define FVH_FACE_UNROLL 1
-#endif
-// FVH_K2_GROUP selects the lane-group residual kernel (one face per lane).
-#ifndef FVH_K2_GROUP
-#define FVH_K2_GROUP 1
-#endif
-#if FVH_FACE_UNROLL
-#define FVH_FACE_PRAGMA _Pragma("unroll")
-#else
-#define FVH_FACE_PRAGMA _Pragma("unroll 1")
+// Dynamic LDS reserved per gradient-kernel workgroup; a non-zero value caps
+// the resident workgroups per CU (occupancy experiments).
+#ifndef FVH_K1_LDS
+#define FVH_K1_LDS 0
 #endif
 
 namespace fvh {
@@ -96,7 +98,7 @@ __device__ __forceinline__ void prim_to_con(const PhysDev &P, double *phi)
              0.5 * dot3(phi[5], phi[6], phi[7], phi[1], phi[2], phi[3]);
 }
 
-// eos.c:55-75
+// eos.c:55-75 (compression branch with the reference's A = 2/((k+1) u_n))
 __device__ __forceinline__ double riemann_p(const PhysDev &P, const double *phi)
 {
     if (phi[5] <= 0.0)
@@ -114,18 +116,17 @@ __device__ __forceinline__ double riemann_p(const PhysDev &P, const double *phi)
 struct Frame
 {
     double n[3], t1[3], t2[3];
+    double area;
 };
 
 __device__ __forceinline__ void load_frame(const DevMesh &m, int f, Frame &fr)
 {
-    const int F = m.F;
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-    {
-        fr.n[d] = m.fn[d * F + f];
-        fr.t1[d] = m.ft1[d * F + f];
-        fr.t2[d] = m.ft2[d * F + f];
-    }
+    const double2 *g = reinterpret_cast<const double2 *>(m.fgA + (size_t)f * kFgA);
+    const double2 a = g[0], b = g[1], c = g[2], d = g[3], e = g[4];
+    fr.n[0] = a.x, fr.n[1] = a.y, fr.n[2] = b.x;
+    fr.t1[0] = b.y, fr.t1[1] = c.x, fr.t1[2] = c.y;
+    fr.t2[0] = d.x, fr.t2[1] = d.y, fr.t2[2] = e.x;
+    fr.area = e.y;
 }
 
 __device__ __forceinline__ void rotate_back(const Frame &fr, double *g)
@@ -182,8 +183,8 @@ __device__ __forceinline__ void ghost_state(const DevMesh &m, const PhysDev &P,
     prim_to_con(P, g);
 }
 
-// Ghost gradient of variable j (boundary.c:361-406): a copy of the interior
-// gradient, with the normal component removed for slip walls and symmetry.
+// Ghost gradient (boundary.c:361-406): a copy of the interior gradient, with
+// the normal component removed for slip walls and symmetry.
 __device__ __forceinline__ void ghost_grad(int type, const Frame &fr, double gx,
                                            double gy, double gz, double &ox,
                                            double &oy, double &oz)
@@ -205,12 +206,13 @@ __device__ __forceinline__ void ghost_grad(int type, const Frame &fr, double gx,
     }
 }
 
-__device__ __forceinline__ void load_cons(const double *__restrict__ U, int N,
-                                          int c, double *phi)
+__device__ __forceinline__ void load_cons(const double *__restrict__ U, int c,
+                                          double *phi)
 {
+    const double *p = U + (size_t)c * 5;
 #pragma unroll
     for (int j = 0; j < 5; ++j)
-        phi[j] = U[j * N + c];
+        phi[j] = p[j];
 }
 
 // Workgroup -> cell tile mapping.  The hardware hands consecutive
@@ -233,6 +235,19 @@ struct ViscGrad
 };
 
 __device__ __forceinline__ int visc_var(int q) { return q < 3 ? q + 1 : 9; }
+
+__device__ __forceinline__ void load_vg(const double *__restrict__ vg, int c,
+                                        ViscGrad &g)
+{
+    const double2 *p = reinterpret_cast<const double2 *>(vg + (size_t)c * 12);
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+    {
+        const double2 v = p[q];
+        g.v[2 * q] = v.x;
+        g.v[2 * q + 1] = v.y;
+    }
+}
 
 // Limited linear extrapolation to a face (reconstruction.c:193-200 with
 // limiter.c:86-105: min(1, y) of the per-face-side Barth-Jespersen ratio).
@@ -257,8 +272,8 @@ __device__ __forceinline__ double limited(double phi, double gx, double gy,
 // ---------------------------------------------------------------------------
 // Ghost rows N+b (boundary.c:188-320 and the update hook ns_update,
 // navier_stokes.c:199-209): the boundary-condition state of every boundary
-// face from its interior cell, as primitives + conservatives [10][B]; with
-// face states on, also side 1 of the boundary faces.
+// face from its interior cell, primitives + conservatives [B][10]; with face
+// states on, also side 1 of the boundary faces.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void fv_ghost_kernel(
     DevMesh m, PhysDev P, const double *__restrict__ U, double *__restrict__ gphi,
@@ -270,29 +285,28 @@ __global__ __launch_bounds__(kBlock) void fv_ghost_kernel(
     if (b >= m.B)
         return;
     const int f = m.bnd_face[b];
-    const int c = m.fc0[f];
+    const int c = m.fcell[2 * f];
     double pc[10], g[10];
-    load_cons(U, m.N, c, pc);
+    load_cons(U, c, pc);
     con_to_prim(P, pc);
     Frame fr;
     load_frame(m, f, fr);
     ghost_state(m, P, b, fr, pc, g);
 #pragma unroll
     for (int j = 0; j < 10; ++j)
-        gphi[(size_t)j * m.B + b] = g[j];
+        gphi[(size_t)b * 10 + j] = g[j];
     if (fs)
 #pragma unroll
         for (int j = 0; j < 8; ++j)
-            fs[(size_t)(8 + j) * m.F + f] = g[j];
+            fs[(size_t)f * 16 + 8 + j] = g[j];
 }
 
 // ---------------------------------------------------------------------------
 // K1: Green-Gauss gradients of all 10 variables and neighbourhood min/max of
 // one cell (reconstruction.c:20-93 in gather form, limiter.c:57-82), then
 // the cell's reconstructed state on each of its faces
-// (reconstruction.c:168-255 / 142-163) and the ghost state on its boundary
-// faces (boundary.c:188-320).  Writes the face states by face id and side,
-// and the gradient subset the viscous flux needs.
+// (reconstruction.c:168-255 / 142-163).  Writes the face states by face id
+// and side, and the gradient subset the viscous flux needs.
 // ---------------------------------------------------------------------------
 template <int MAXF>
 __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
@@ -302,13 +316,13 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
 {
     if (sc && sc->done)
         return;
-    const int N = m.N, F = m.F;
+    const int N = m.N;
     const int c = xcd_tile(gridDim.x) * kBlock + threadIdx.x;
     if (c >= N)
         return;
 
     double pc[10];
-    load_cons(U, N, c, pc);
+    load_cons(U, c, pc);
     con_to_prim(P, pc);
 
     double gx[10], gy[10], gz[10], mn[10], mx[10];
@@ -323,33 +337,34 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
     }
 
     const int nf = m.ncf[c];
-    FVH_FACE_PRAGMA
+    const int *cfc = m.cf + (size_t)c * m.maxf;
+#pragma unroll
     for (int s = 0; s < MAXF; ++s)
     {
         if (s >= nf)
             break;
-        const int f = m.cf[s * N + c];
-        const int c0 = m.fc0[f];
-        const int c1 = m.fc1[f];
-        const bool own0 = c0 == c;
-        const int nb = own0 ? c1 : c0;
-        const double n0 = m.fn[f], n1 = m.fn[F + f], n2 = m.fn[2 * F + f];
-        const double lam = m.flam[f];
-        const double area = m.farea[f];
+        const int f = cfc[s];
+        const int2 fc = reinterpret_cast<const int2 *>(m.fcell)[f];
+        const bool own0 = fc.x == c;
+        const int nb = own0 ? fc.y : fc.x;
+        const double2 *g = reinterpret_cast<const double2 *>(m.fgB + (size_t)f * kFgB);
+        const double2 g0 = g[0], g1 = g[1], g2 = g[2];
+        const double n0 = g0.x, n1 = g0.y, n2 = g1.x;
+        const double area = g1.y, lam = g2.x;
 
         double pn[10];
         if (nb < N)
         {
-            load_cons(U, N, nb, pn);
+            load_cons(U, nb, pn);
             con_to_prim(P, pn);
         }
         else
         {
             // boundary face: the ghost row written by fv_ghost_kernel
-            const int b = nb - N;
+            const double *gp = gphi + (size_t)(nb - N) * 10;
 #pragma unroll
             for (int j = 0; j < 10; ++j)
-                pn[j] = gphi[(size_t)j * m.B + b];
+                pn[j] = gp[j];
         }
 
 #pragma unroll
@@ -389,62 +404,84 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
 
     if (out.vg)
     {
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-        {
-            const int j = visc_var(q);
-            out.vg[(size_t)(0 + q) * N + c] = gx[j];
-            out.vg[(size_t)(4 + q) * N + c] = gy[j];
-            out.vg[(size_t)(8 + q) * N + c] = gz[j];
-        }
+        double2 *d = reinterpret_cast<double2 *>(out.vg + (size_t)c * 12);
+        d[0] = make_double2(gx[1], gx[2]);
+        d[1] = make_double2(gx[3], gx[9]);
+        d[2] = make_double2(gy[1], gy[2]);
+        d[3] = make_double2(gy[3], gy[9]);
+        d[4] = make_double2(gz[1], gz[2]);
+        d[5] = make_double2(gz[3], gz[9]);
     }
     if (out.grad)
     {
+        double *d = out.grad + (size_t)c * 30;
 #pragma unroll
         for (int j = 0; j < 10; ++j)
         {
-            out.grad[(size_t)(0 + j) * N + c] = gx[j];
-            out.grad[(size_t)(10 + j) * N + c] = gy[j];
-            out.grad[(size_t)(20 + j) * N + c] = gz[j];
+            d[j] = gx[j];
+            d[10 + j] = gy[j];
+            d[20 + j] = gz[j];
         }
     }
     if (out.mnmx)
     {
+        double *d = out.mnmx + (size_t)c * 20;
 #pragma unroll
         for (int j = 0; j < 10; ++j)
         {
-            out.mnmx[(size_t)j * N + c] = mn[j];
-            out.mnmx[(size_t)(10 + j) * N + c] = mx[j];
+            d[j] = mn[j];
+            d[10 + j] = mx[j];
         }
     }
 
     if (!out.fs)
         return;
-    // this cell's reconstructed state on each of its faces
-    FVH_FACE_PRAGMA
+    // this cell's reconstructed state on each of its faces; the distance
+    // vectors of mesh.c:44-56 (dist_cell_1 = x_f - x_c0, dist_cell_2 =
+    // x_f - x_c1 through the periodic image) from the face and cell centres
+    const double xc0 = m.cx[(size_t)c * 3], xc1 = m.cx[(size_t)c * 3 + 1],
+                 xc2 = m.cx[(size_t)c * 3 + 2];
+#pragma unroll
     for (int s = 0; s < MAXF; ++s)
     {
         if (s >= nf)
             break;
-        const int f = m.cf[s * N + c];
-        const bool own0 = m.fc0[f] == c;
-        const int side = own0 ? 0 : 1;
-        double *dst = out.fs + (size_t)side * 8 * F + f;
+        const int f = cfc[s];
+        const bool own0 = m.fcell[2 * f] == c;
+        double st[8];
         if (linear)
         {
-            const double *dd = own0 ? m.fd1 : m.fd2;
-            const double r[3] = {dd[f], dd[F + f], dd[2 * F + f]};
+            const double2 *g = reinterpret_cast<const double2 *>(m.fgB + (size_t)f * kFgB);
+            const double2 g2 = g[2], g3 = g[3];
+            double r[3];
+            if (own0)
+            {
+                r[0] = g2.y - xc0;
+                r[1] = g3.x - xc1;
+                r[2] = g3.y - xc2;
+            }
+            else
+            {
+                const double *sh = m.shtab + (size_t)m.fsh[f] * 3;
+                r[0] = g2.y - (xc0 + sh[0]);
+                r[1] = g3.x - (xc1 + sh[1]);
+                r[2] = g3.y - (xc2 + sh[2]);
+            }
 #pragma unroll
             for (int j = 0; j < 8; ++j)
-                dst[(size_t)j * F] = limited(pc[j], gx[j], gy[j], gz[j], r, mn[j],
-                                             mx[j], P.bj);
+                st[j] = limited(pc[j], gx[j], gy[j], gz[j], r, mn[j], mx[j], P.bj);
         }
         else
         {
 #pragma unroll
             for (int j = 0; j < 8; ++j)
-                dst[(size_t)j * F] = pc[j];
+                st[j] = pc[j];
         }
+        double2 *d = reinterpret_cast<double2 *>(out.fs + (size_t)f * 16 + (own0 ? 0 : 8));
+        d[0] = make_double2(st[0], st[1]);
+        d[1] = make_double2(st[2], st[3]);
+        d[2] = make_double2(st[4], st[5]);
+        d[3] = make_double2(st[6], st[7]);
     }
 }
 
@@ -649,215 +686,23 @@ __device__ __forceinline__ void block_min(double *sh, double &v)
 }
 
 // ---------------------------------------------------------------------------
-// K2: residual dU/dt = -(sum_f +-flux_f A_f)/V of one cell
-// (finite_volume.c:96-123), optionally fused with the 2N-storage RK stage
+// K2: residual dU/dt = -(sum_f +-flux_f A_f)/V (finite_volume.c:96-123) in
+// lane groups: LANES consecutive lanes per cell (4 for tetrahedra, 8 for
+// hexahedra), one face per lane.  The group's contributions are summed in
+// ascending face order with lane shuffles (acc - x is acc + (-x) exactly,
+// so the sum is bit-identical to the face-ordered scatter), and the group's
+// first lane runs the cell epilogue: the 2N-storage RK stage
 // (explicit.c:113-140) and, in the last stage, the residual-norm and
 // time-step partial reductions (analyze.c:46-55, navier_stokes.c:171-187).
 // FS: face states from K1 (reconstructed path); otherwise first order, the
-// face states are the cell values taken from U.
-// ---------------------------------------------------------------------------
-template <int FLUX, bool FS, bool VISC, int MAXF>
-__global__ __launch_bounds__(kBlock, FVH_K2_MINW) void fv_residual_kernel(DevMesh m,
-                                                             PhysDev P,
-                                                             StageParams sp)
-{
-    __shared__ double sh[kBlock];
-    const int N = m.N, F = m.F;
-    const int c = xcd_tile(gridDim.x) * kBlock + threadIdx.x;
-
-    if (sp.sc && sp.sc->done)
-    {
-        // the run has stopped: carry the state through this stage unchanged
-        if (sp.do_update && c < N)
-#pragma unroll
-            for (int j = 0; j < 5; ++j)
-                sp.U_out[j * (size_t)N + c] = sp.U_in[j * (size_t)N + c];
-        return;
-    }
-
-    double part[7] = {0.0, 0.0, 0.0, 0.0, 0.0, DBL_MAX, DBL_MAX};
-
-    if (c < N)
-    {
-        double pc[10];
-        load_cons(sp.U_in, N, c, pc);
-        con_to_prim(P, pc);
-
-        ViscGrad go;
-        if (VISC)
-        {
-#pragma unroll
-            for (int q = 0; q < 12; ++q)
-                go.v[q] = sp.vg[(size_t)q * N + c];
-        }
-
-        double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        const int nf = m.ncf[c];
-        FVH_FACE_PRAGMA
-        for (int s = 0; s < MAXF; ++s)
-        {
-            if (s >= nf)
-                break;
-            const int f = m.cf[s * N + c];
-            const int c0 = m.fc0[f];
-            const int c1 = m.fc1[f];
-            const bool own0 = c0 == c;
-            const int nb = own0 ? c1 : c0;
-            const bool ghost = nb >= N;
-            Frame fr;
-            load_frame(m, f, fr);
-
-            double L[10], R[10];
-            if (FS)
-            {
-#pragma unroll
-                for (int j = 0; j < 8; ++j)
-                {
-                    L[j] = sp.fs[(size_t)j * F + f];
-                    R[j] = sp.fs[(size_t)(8 + j) * F + f];
-                }
-                L[8] = L[9] = R[8] = R[9] = 0.0;
-            }
-            else
-            {
-                double pn[10];
-                if (ghost)
-                {
-#pragma unroll
-                    for (int j = 0; j < 10; ++j)
-                        pn[j] = sp.gphi[(size_t)j * m.B + (nb - N)];
-                }
-                else
-                {
-                    load_cons(sp.U_in, N, nb, pn);
-                    con_to_prim(P, pn);
-                }
-#pragma unroll
-                for (int j = 0; j < 10; ++j)
-                {
-                    L[j] = own0 ? pc[j] : pn[j];
-                    R[j] = own0 ? pn[j] : pc[j];
-                }
-            }
-
-            ViscGrad gL, gR;
-            if (VISC)
-            {
-                ViscGrad gn;
-                if (ghost)
-                {
-                    const int type = m.region_type[m.bnd_region[nb - N]];
-#pragma unroll
-                    for (int q = 0; q < 4; ++q)
-                        ghost_grad(type, fr, go.v[q], go.v[4 + q], go.v[8 + q],
-                                   gn.v[q], gn.v[4 + q], gn.v[8 + q]);
-                }
-                else
-                {
-#pragma unroll
-                    for (int q = 0; q < 12; ++q)
-                        gn.v[q] = sp.vg[(size_t)q * N + nb];
-                }
-#pragma unroll
-                for (int q = 0; q < 12; ++q)
-                {
-                    gL.v[q] = own0 ? go.v[q] : gn.v[q];
-                    gR.v[q] = own0 ? gn.v[q] : go.v[q];
-                }
-            }
-
-            double flux[5];
-            face_flux<FLUX, VISC>(P, fr, L, R, gL, gR, flux);
-            const double area = m.farea[f];
-#pragma unroll
-            for (int j = 0; j < 5; ++j)
-            {
-                if (own0)
-                    acc[j] += flux[j] * area;
-                else
-                    acc[j] -= flux[j] * area;
-            }
-        }
-
-        const double vol = m.vol[c];
-        double R5[5];
-#pragma unroll
-        for (int j = 0; j < 5; ++j)
-            R5[j] = -acc[j] / vol;
-
-        if (sp.R_out)
-        {
-#pragma unroll
-            for (int j = 0; j < 5; ++j)
-                sp.R_out[j * (size_t)N + c] = R5[j];
-        }
-
-        if (sp.do_update)
-        {
-            const double dt = sp.sc->dt;
-            double unew[5];
-#pragma unroll
-            for (int j = 0; j < 5; ++j)
-            {
-                double kj = R5[j];
-                if (!sp.stage0)
-                    kj = R5[j] + sp.k[j * (size_t)N + c] * sp.rk_a;
-                sp.k[j * (size_t)N + c] = kj;
-                unew[j] = pc[j] + kj * dt * sp.rk_g;
-                sp.U_out[j * (size_t)N + c] = unew[j];
-            }
-            if (sp.do_reduce)
-            {
-#pragma unroll
-                for (int j = 0; j < 5; ++j)
-                    part[j] = rabs(R5[j]) * vol;
-                // time step with the new density and the primitives of
-                // this stage's input (navier_stokes.c:171-187)
-                const double dx0 = m.cdx[c], dx1 = m.cdx[N + c],
-                             dx2 = m.cdx[2 * (size_t)N + c];
-                const double s_rho = 1.0 / unew[0];
-                const double cs = sqrt(P.kappa * pc[8] * s_rho);
-                const double ds1 = (rabs(pc[5]) + cs) * dx0 +
-                                   (rabs(pc[6]) + cs) * dx1 +
-                                   (rabs(pc[7]) + cs) * dx2;
-                part[5] = vol / ds1;
-                const double ds2 = dot3(dx0, dx1, dx2, dx0, dx1, dx2);
-                part[6] = s_rho * ds2 / vol;
-            }
-        }
-    }
-
-    if (sp.do_reduce)
-    {
-#pragma unroll
-        for (int q = 0; q < 5; ++q)
-            block_sum(sh, part[q]);
-        block_min(sh, part[5]);
-        block_min(sh, part[6]);
-        if (threadIdx.x == 0)
-        {
-#pragma unroll
-            for (int q = 0; q < 7; ++q)
-                sp.partials[blockIdx.x * kPartialStride + q] = part[q];
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// K2 in lane groups: LANES consecutive lanes per cell, one face per lane
-// (LANES = 4 for tetrahedra, 8 for hexahedra).  Each lane evaluates one face
-// flux; the group's contributions are summed in ascending face order with
-// lane shuffles (acc - x is acc + (-x) exactly, so the sum is bit-identical
-// to the face-ordered scatter of finite_volume.c:100-112), and the group's
-// first lane runs the cell epilogue.  Same results as fv_residual_kernel
-// with a quarter of the per-lane state, so more waves hide the gathers.
+// face states are the cell values of U.
 // ---------------------------------------------------------------------------
 template <int FLUX, bool FS, bool VISC, int LANES>
 __global__ __launch_bounds__(kBlock, FVH_K2_MINW) void fv_residual_group_kernel(
     DevMesh m, PhysDev P, StageParams sp)
 {
     __shared__ double sh[kBlock];
-    const int N = m.N, F = m.F;
+    const int N = m.N;
     const int tid = xcd_tile(gridDim.x) * kBlock + threadIdx.x;
     const int c = tid / LANES;
     const int s = tid & (LANES - 1);
@@ -865,10 +710,11 @@ __global__ __launch_bounds__(kBlock, FVH_K2_MINW) void fv_residual_group_kernel(
 
     if (sp.sc && sp.sc->done)
     {
+        // the run has stopped: carry the state through this stage unchanged
         if (sp.do_update && cell_ok && s == 0)
 #pragma unroll
             for (int j = 0; j < 5; ++j)
-                sp.U_out[j * (size_t)N + c] = sp.U_in[j * (size_t)N + c];
+                sp.U_out[(size_t)c * 5 + j] = sp.U_in[(size_t)c * 5 + j];
         return;
     }
 
@@ -876,11 +722,10 @@ __global__ __launch_bounds__(kBlock, FVH_K2_MINW) void fv_residual_group_kernel(
     double contrib[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     if (s < nf)
     {
-        const int f = m.cf[s * N + c];
-        const int c0 = m.fc0[f];
-        const int c1 = m.fc1[f];
-        const bool own0 = c0 == c;
-        const int nb = own0 ? c1 : c0;
+        const int f = m.cf[(size_t)c * m.maxf + s];
+        const int2 fc = reinterpret_cast<const int2 *>(m.fcell)[f];
+        const bool own0 = fc.x == c;
+        const int nb = own0 ? fc.y : fc.x;
         const bool ghost = nb >= N;
         Frame fr;
         load_frame(m, f, fr);
@@ -888,28 +733,33 @@ __global__ __launch_bounds__(kBlock, FVH_K2_MINW) void fv_residual_group_kernel(
         double L[10], R[10];
         if (FS)
         {
+            const double2 *p = reinterpret_cast<const double2 *>(sp.fs + (size_t)f * 16);
 #pragma unroll
-            for (int j = 0; j < 8; ++j)
+            for (int q = 0; q < 4; ++q)
             {
-                L[j] = sp.fs[(size_t)j * F + f];
-                R[j] = sp.fs[(size_t)(8 + j) * F + f];
+                const double2 a = p[q], b = p[4 + q];
+                L[2 * q] = a.x;
+                L[2 * q + 1] = a.y;
+                R[2 * q] = b.x;
+                R[2 * q + 1] = b.y;
             }
             L[8] = L[9] = R[8] = R[9] = 0.0;
         }
         else
         {
             double pc[10], pn[10];
-            load_cons(sp.U_in, N, c, pc);
+            load_cons(sp.U_in, c, pc);
             con_to_prim(P, pc);
             if (ghost)
             {
+                const double *gp = sp.gphi + (size_t)(nb - N) * 10;
 #pragma unroll
                 for (int j = 0; j < 10; ++j)
-                    pn[j] = sp.gphi[(size_t)j * m.B + (nb - N)];
+                    pn[j] = gp[j];
             }
             else
             {
-                load_cons(sp.U_in, N, nb, pn);
+                load_cons(sp.U_in, nb, pn);
                 con_to_prim(P, pn);
             }
 #pragma unroll
@@ -924,9 +774,7 @@ __global__ __launch_bounds__(kBlock, FVH_K2_MINW) void fv_residual_group_kernel(
         if (VISC)
         {
             ViscGrad go, gn;
-#pragma unroll
-            for (int q = 0; q < 12; ++q)
-                go.v[q] = sp.vg[(size_t)q * N + c];
+            load_vg(sp.vg, c, go);
             if (ghost)
             {
                 const int type = m.region_type[m.bnd_region[nb - N]];
@@ -937,9 +785,7 @@ __global__ __launch_bounds__(kBlock, FVH_K2_MINW) void fv_residual_group_kernel(
             }
             else
             {
-#pragma unroll
-                for (int q = 0; q < 12; ++q)
-                    gn.v[q] = sp.vg[(size_t)q * N + nb];
+                load_vg(sp.vg, nb, gn);
             }
 #pragma unroll
             for (int q = 0; q < 12; ++q)
@@ -951,11 +797,10 @@ __global__ __launch_bounds__(kBlock, FVH_K2_MINW) void fv_residual_group_kernel(
 
         double flux[5];
         face_flux<FLUX, VISC>(P, fr, L, R, gL, gR, flux);
-        const double area = m.farea[f];
 #pragma unroll
         for (int j = 0; j < 5; ++j)
         {
-            const double v = flux[j] * area;
+            const double v = flux[j] * fr.area;
             contrib[j] = own0 ? v : -v;
         }
     }
@@ -988,12 +833,12 @@ __global__ __launch_bounds__(kBlock, FVH_K2_MINW) void fv_residual_group_kernel(
         {
 #pragma unroll
             for (int j = 0; j < 5; ++j)
-                sp.R_out[j * (size_t)N + c] = R5[j];
+                sp.R_out[(size_t)c * 5 + j] = R5[j];
         }
         if (sp.do_update)
         {
             double pc[10];
-            load_cons(sp.U_in, N, c, pc);
+            load_cons(sp.U_in, c, pc);
             const double dt = sp.sc->dt;
             double unew[5];
 #pragma unroll
@@ -1001,19 +846,21 @@ __global__ __launch_bounds__(kBlock, FVH_K2_MINW) void fv_residual_group_kernel(
             {
                 double kj = R5[j];
                 if (!sp.stage0)
-                    kj = R5[j] + sp.k[j * (size_t)N + c] * sp.rk_a;
-                sp.k[j * (size_t)N + c] = kj;
+                    kj = R5[j] + sp.k[(size_t)c * 5 + j] * sp.rk_a;
+                sp.k[(size_t)c * 5 + j] = kj;
                 unew[j] = pc[j] + kj * dt * sp.rk_g;
-                sp.U_out[j * (size_t)N + c] = unew[j];
+                sp.U_out[(size_t)c * 5 + j] = unew[j];
             }
             if (sp.do_reduce)
             {
+                // time step with the new density and the primitives of this
+                // stage's input (navier_stokes.c:171-187)
                 con_to_prim(P, pc);
 #pragma unroll
                 for (int j = 0; j < 5; ++j)
                     part[j] = rabs(R5[j]) * vol;
-                const double dx0 = m.cdx[c], dx1 = m.cdx[N + c],
-                             dx2 = m.cdx[2 * (size_t)N + c];
+                const double dx0 = m.cdx[(size_t)c * 3], dx1 = m.cdx[(size_t)c * 3 + 1],
+                             dx2 = m.cdx[(size_t)c * 3 + 2];
                 const double s_rho = 1.0 / unew[0];
                 const double cs = sqrt(P.kappa * pc[8] * s_rho);
                 const double ds1 = (rabs(pc[5]) + cs) * dx0 +
@@ -1055,10 +902,10 @@ __global__ __launch_bounds__(kBlock) void fv_dt_partials_kernel(
     if (c < N)
     {
         double pc[10];
-        load_cons(U, N, c, pc);
+        load_cons(U, c, pc);
         con_to_prim(P, pc);
-        const double dx0 = m.cdx[c], dx1 = m.cdx[N + c],
-                     dx2 = m.cdx[2 * (size_t)N + c];
+        const double dx0 = m.cdx[(size_t)c * 3], dx1 = m.cdx[(size_t)c * 3 + 1],
+                     dx2 = m.cdx[(size_t)c * 3 + 2];
         const double vol = m.vol[c];
         const double s_rho = 1.0 / pc[0];
         const double cs = sqrt(P.kappa * pc[8] * s_rho);
@@ -1075,6 +922,42 @@ __global__ __launch_bounds__(kBlock) void fv_dt_partials_kernel(
         partials[blockIdx.x * kPartialStride + 5] = lc;
         partials[blockIdx.x * kPartialStride + 6] = lv;
     }
+}
+
+// First level of the reduction of the per-workgroup partials: each block
+// folds kFoldRows consecutive rows (sums of 0..4, minima of 5..6) in a fixed
+// order into one row of `fold`.
+constexpr int kFoldRows = 2048;
+
+__global__ __launch_bounds__(kBlock) void fv_fold_partials_kernel(
+    const double *__restrict__ partials, int n_partials, int mode,
+    double *__restrict__ fold)
+{
+    __shared__ double sh[kBlock];
+    const int t = threadIdx.x;
+    const int lo = blockIdx.x * kFoldRows;
+    const int hi = min(lo + kFoldRows, n_partials);
+    double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, DBL_MAX, DBL_MAX};
+    for (int i = lo + t; i < hi; i += kBlock)
+    {
+        const double *p = &partials[(size_t)i * kPartialStride];
+        if (mode == 1)
+#pragma unroll
+            for (int q = 0; q < 5; ++q)
+                v[q] = v[q] + p[q];
+        v[5] = rmin(v[5], p[5]);
+        v[6] = rmin(v[6], p[6]);
+    }
+    if (mode == 1)
+#pragma unroll
+        for (int q = 0; q < 5; ++q)
+            block_sum(sh, v[q]);
+    block_min(sh, v[5]);
+    block_min(sh, v[6]);
+    if (t == 0)
+#pragma unroll
+        for (int q = 0; q < 7; ++q)
+            fold[(size_t)blockIdx.x * kPartialStride + q] = v[q];
 }
 
 // Time-loop bookkeeping of timedisc.c:164-207 on the device: residual norm
@@ -1190,20 +1073,20 @@ __global__ __launch_bounds__(kBlock) void fv_mirror_kernel(
     double pc[10];
     if (row < N)
     {
-        load_cons(U, N, row, pc);
+        load_cons(U, row, pc);
         con_to_prim(P, pc);
         if (R && dt_aos)
 #pragma unroll
             for (int j = 0; j < 5; ++j)
-                dt_aos[(size_t)row * 5 + j] = R[j * (size_t)N + row];
+                dt_aos[(size_t)row * 5 + j] = R[(size_t)row * 5 + j];
     }
     else
     {
         const int b = row - N;
         const int f = m.bnd_face[b];
-        const int c = m.fc0[f];
+        const int c = m.fcell[2 * f];
         double pi[10];
-        load_cons(U, N, c, pi);
+        load_cons(U, c, pi);
         con_to_prim(P, pi);
         Frame fr;
         load_frame(m, f, fr);
@@ -1217,14 +1100,18 @@ __global__ __launch_bounds__(kBlock) void fv_mirror_kernel(
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-void launch_mirror(hipStream_t s, const DevMesh &m, const PhysDev &P,
-                   const double *U, const double *R, double *phi_total_aos,
-                   double *phi_dt_aos)
+static void check_faces(const DevMesh &m)
 {
-    hipLaunchKernelGGL(fv_mirror_kernel, dim3(cell_blocks(m.N + m.B)),
-                       dim3(kBlock), 0, s, m, P, U, R, phi_total_aos,
-                       phi_dt_aos);
+    if (m.maxf > 6)
+        throw std::runtime_error("cells with more than 6 faces are not supported");
 }
+
+int residual_grid(const DevMesh &m)
+{
+    return cell_blocks(m.N * (m.maxf <= 4 ? 4 : 8));
+}
+
+int dt_partials_grid(const DevMesh &m) { return cell_blocks(m.N); }
 
 void launch_ghost(hipStream_t s, const DevMesh &m, const PhysDev &P,
                   const double *U, double *gphi, double *fs,
@@ -1240,61 +1127,39 @@ void launch_gradient(hipStream_t s, const DevMesh &m, const PhysDev &P,
                      const double *U, const double *gphi, const GradOut &out,
                      int linear, const DevScalars *sc)
 {
-    if (m.maxf > 6)
-        throw std::runtime_error("cells with more than 6 faces are not supported");
+    check_faces(m);
     if (m.maxf <= 4)
         hipLaunchKernelGGL(fv_gradient_kernel<4>, dim3(cell_blocks(m.N)),
-                           dim3(kBlock), 0, s, m, P, U, gphi, out, linear, sc);
+                           dim3(kBlock), FVH_K1_LDS, s, m, P, U, gphi, out, linear,
+                           sc);
     else
         hipLaunchKernelGGL(fv_gradient_kernel<6>, dim3(cell_blocks(m.N)),
-                           dim3(kBlock), 0, s, m, P, U, gphi, out, linear, sc);
+                           dim3(kBlock), FVH_K1_LDS, s, m, P, U, gphi, out, linear,
+                           sc);
 }
-
-int residual_grid(const DevMesh &m)
-{
-    if (FVH_K2_GROUP)
-        return cell_blocks(m.N * (m.maxf <= 4 ? 4 : 8));
-    return cell_blocks(m.N);
-}
-
-int dt_partials_grid(const DevMesh &m) { return cell_blocks(m.N); }
 
 template <int FLUX, bool FS, bool VISC>
 static void launch_res_t(hipStream_t s, const DevMesh &m, const PhysDev &P,
                          const StageParams &sp)
 {
-    if (FVH_K2_GROUP)
-    {
-        if (m.maxf <= 4)
-            hipLaunchKernelGGL((fv_residual_group_kernel<FLUX, FS, VISC, 4>),
-                               dim3(residual_grid(m)), dim3(kBlock), 0, s, m, P,
-                               sp);
-        else
-            hipLaunchKernelGGL((fv_residual_group_kernel<FLUX, FS, VISC, 8>),
-                               dim3(residual_grid(m)), dim3(kBlock), 0, s, m, P,
-                               sp);
-        return;
-    }
     if (m.maxf <= 4)
-        hipLaunchKernelGGL((fv_residual_kernel<FLUX, FS, VISC, 4>),
-                           dim3(cell_blocks(m.N)), dim3(kBlock), 0, s, m, P, sp);
+        hipLaunchKernelGGL((fv_residual_group_kernel<FLUX, FS, VISC, 4>),
+                           dim3(residual_grid(m)), dim3(kBlock), 0, s, m, P, sp);
     else
-        hipLaunchKernelGGL((fv_residual_kernel<FLUX, FS, VISC, 6>),
-                           dim3(cell_blocks(m.N)), dim3(kBlock), 0, s, m, P, sp);
+        hipLaunchKernelGGL((fv_residual_group_kernel<FLUX, FS, VISC, 8>),
+                           dim3(residual_grid(m)), dim3(kBlock), 0, s, m, P, sp);
 }
 
 void launch_residual(hipStream_t s, const DevMesh &m, const PhysDev &P,
                      const StageParams &sp, int flux, bool face_states,
                      bool viscous)
 {
-    if (m.maxf > 6)
-        throw std::runtime_error("cells with more than 6 faces are not supported");
+    check_faces(m);
     if (sp.do_reduce && residual_grid(m) > sp.partials_cap)
         throw std::runtime_error("reduction partials buffer too small");
-    const bool linear = face_states;
     if (flux == FluxRusanov)
     {
-        if (linear)
+        if (face_states)
             viscous ? launch_res_t<FluxRusanov, true, true>(s, m, P, sp)
                     : launch_res_t<FluxRusanov, true, false>(s, m, P, sp);
         else
@@ -1303,7 +1168,7 @@ void launch_residual(hipStream_t s, const DevMesh &m, const PhysDev &P,
     }
     else
     {
-        if (linear)
+        if (face_states)
             viscous ? launch_res_t<FluxAUSM, true, true>(s, m, P, sp)
                     : launch_res_t<FluxAUSM, true, false>(s, m, P, sp);
         else
@@ -1317,16 +1182,32 @@ void launch_dt_partials(hipStream_t s, const DevMesh &m, const PhysDev &P,
 {
     if (dt_partials_grid(m) > partials_cap)
         throw std::runtime_error("reduction partials buffer too small");
-    hipLaunchKernelGGL(fv_dt_partials_kernel, dim3(cell_blocks(m.N)),
+    hipLaunchKernelGGL(fv_dt_partials_kernel, dim3(dt_partials_grid(m)),
                        dim3(kBlock), 0, s, m, P, U, partials);
 }
 
-void launch_finalize(hipStream_t s, DevScalars *sc, const double *partials,
-                     int n_partials, const PhysDev &P, int mode, double *log,
-                     int log_cap)
+void launch_mirror(hipStream_t s, const DevMesh &m, const PhysDev &P,
+                   const double *U, const double *R, double *phi_total_aos,
+                   double *phi_dt_aos)
 {
+    hipLaunchKernelGGL(fv_mirror_kernel, dim3(cell_blocks(m.N + m.B)),
+                       dim3(kBlock), 0, s, m, P, U, R, phi_total_aos,
+                       phi_dt_aos);
+}
+
+int fold_rows(int n_partials) { return (n_partials + kFoldRows - 1) / kFoldRows; }
+
+void launch_finalize(hipStream_t s, DevScalars *sc, const double *partials,
+                     int n_partials, double *fold, int fold_cap,
+                     const PhysDev &P, int mode, double *log, int log_cap)
+{
+    const int nf = fold_rows(n_partials);
+    if (nf > fold_cap)
+        throw std::runtime_error("reduction fold buffer too small");
+    hipLaunchKernelGGL(fv_fold_partials_kernel, dim3(nf), dim3(kBlock), 0, s,
+                       partials, n_partials, mode, fold);
     hipLaunchKernelGGL(fv_finalize_kernel, dim3(1), dim3(kBlock), 0, s, sc,
-                       partials, n_partials, P, mode, log, log_cap);
+                       fold, nf, P, mode, log, log_cap);
 }
 
 } // namespace dev

@@ -154,6 +154,8 @@ struct Device
     double *mnmx = nullptr; // [20*N] min/max (tests only)
     double *partials = nullptr;
     int partials_cap = 0; // rows of partials
+    double *fold = nullptr;
+    int fold_cap = 0;
     dev::DevScalars *sc = nullptr;
     dev::DevScalars *sc_host = nullptr; // pinned
     double *log = nullptr;
@@ -328,11 +330,11 @@ void sync_mirror(bool with_dt)
 void upload_state_from_mirror()
 {
     const int N = g_dev.m.N;
-    std::vector<double> soa((size_t)5 * N);
+    std::vector<double> aos((size_t)5 * N);
     for (int c = 0; c < N; ++c)
         for (int j = 0; j < 5; ++j)
-            soa[(size_t)j * N + c] = solver_data->phi_total[(size_t)c * 10 + j];
-    HIP_CHECK(hipMemcpy(g_dev.U[g_dev.cur], soa.data(), soa.size() * 8,
+            aos[(size_t)c * 5 + j] = solver_data->phi_total[(size_t)c * 10 + j];
+    HIP_CHECK(hipMemcpy(g_dev.U[g_dev.cur], aos.data(), aos.size() * 8,
                         hipMemcpyHostToDevice));
     g_dev.r_valid = false;
 }
@@ -627,8 +629,7 @@ void enqueue_explicit_step(dev::DevScalars *sc)
                          sc);
         g_dev.cur = 1 - g_dev.cur;
     }
-    dev::launch_finalize(g_dev.stream, sc, g_dev.partials,
-                         dev::residual_grid(g_dev.m), phys(), 1, g_dev.log,
+    dev::launch_finalize(g_dev.stream, sc, g_dev.partials, dev::residual_grid(g_dev.m), g_dev.fold, g_dev.fold_cap, phys(), 1, g_dev.log,
                          g_dev.log_cap);
 }
 
@@ -651,8 +652,7 @@ void init_scalars(int iter0, double t0, int max_iter, bool transient,
     // initial dt from the current state (primitives refreshed from U)
     dev::launch_dt_partials(g_dev.stream, g_dev.m, phys(), g_dev.U[g_dev.cur],
                             g_dev.partials, g_dev.partials_cap);
-    dev::launch_finalize(g_dev.stream, g_dev.sc, g_dev.partials,
-                         dev::dt_partials_grid(g_dev.m), phys(), 0, nullptr, 0);
+    dev::launch_finalize(g_dev.stream, g_dev.sc, g_dev.partials, dev::dt_partials_grid(g_dev.m), g_dev.fold, g_dev.fold_cap, phys(), 0, nullptr, 0);
 }
 
 void read_scalars()
@@ -926,35 +926,50 @@ void init_finite_volume(void)
         m.F = F;
         m.B = B;
         m.maxf = maxf;
+        // cell face lists and face cells keep the reference's AoS layout
+        m.cf = dupload(cells->faces, (size_t)N * maxf);
+        m.ncf = dupload(cells->n_faces, N);
+        m.fcell = dupload(faces->cells, (size_t)F * 2);
         {
-            std::vector<int> cf((size_t)maxf * N);
-            for (int c = 0; c < N; ++c)
-                for (int s = 0; s < maxf; ++s)
-                    cf[(size_t)s * N + c] = cells->faces[(size_t)c * maxf + s];
-            m.cf = dupload(cf.data(), cf.size());
-            m.ncf = dupload(cells->n_faces, N);
-        }
-        {
-            std::vector<int> a(F), b(F);
+            // face records; periodic shifts as indices into a table of the
+            // distinct exact shift vectors (entry 0 = no shift)
+            std::vector<double> a((size_t)F * dev::kFgA), b((size_t)F * dev::kFgB);
+            std::vector<int> fsh(F, 0);
+            std::vector<double> shtab = {0.0, 0.0, 0.0};
             for (int f = 0; f < F; ++f)
-                a[f] = faces->cells[2 * f], b[f] = faces->cells[2 * f + 1];
-            m.fc0 = dupload(a.data(), F);
-            m.fc1 = dupload(b.data(), F);
-        }
-        auto soa3 = [&](const double *aos, int n) {
-            std::vector<double> v((size_t)3 * n);
-            for (int i = 0; i < n; ++i)
+            {
+                double *ra = &a[(size_t)f * dev::kFgA];
+                double *rb = &b[(size_t)f * dev::kFgB];
                 for (int d = 0; d < 3; ++d)
-                    v[(size_t)d * n + i] = aos[(size_t)i * 3 + d];
-            return dupload(v.data(), v.size());
-        };
-        m.fn = soa3(faces->n, F);
-        m.ft1 = soa3(faces->t1, F);
-        m.ft2 = soa3(faces->t2, F);
-        m.fd1 = soa3(faces->dist_cell_1, F);
-        m.fd2 = soa3(faces->dist_cell_2, F);
-        m.farea = dupload(faces->area, F);
-        m.flam = dupload(faces->lambda, F);
+                {
+                    ra[d] = faces->n[3 * f + d];
+                    ra[3 + d] = faces->t1[3 * f + d];
+                    ra[6 + d] = faces->t2[3 * f + d];
+                    rb[d] = faces->n[3 * f + d];
+                    rb[5 + d] = faces->x[3 * f + d];
+                }
+                ra[9] = faces->area[f];
+                rb[3] = faces->area[f];
+                rb[4] = faces->lambda[f];
+                const double *sh = &g_face_shift[(size_t)3 * f];
+                if (sh[0] != 0.0 || sh[1] != 0.0 || sh[2] != 0.0)
+                {
+                    int k = 1;
+                    const int nsh = (int)shtab.size() / 3;
+                    for (; k < nsh; ++k)
+                        if (std::memcmp(&shtab[3 * k], sh, 3 * sizeof(double)) == 0)
+                            break;
+                    if (k == nsh)
+                        shtab.insert(shtab.end(), sh, sh + 3);
+                    fsh[f] = k;
+                }
+            }
+            m.fgA = dupload(a.data(), a.size());
+            m.fgB = dupload(b.data(), b.size());
+            m.fsh = dupload(fsh.data(), fsh.size());
+            m.shtab = dupload(shtab.data(), shtab.size());
+        }
+        m.cx = dupload(cells->x, (size_t)N * 3);
         m.vol = dupload(cells->volume, N);
         {
             std::vector<double> sv(N);
@@ -962,7 +977,7 @@ void init_finite_volume(void)
                 sv[c] = 1.0 / cells->volume[c];
             m.svol = dupload(sv.data(), N);
         }
-        m.cdx = soa3(cells->dx, N);
+        m.cdx = dupload(cells->dx, (size_t)N * 3);
         m.bnd_region = dupload(bnd->id, B);
         m.bnd_face = dupload(bnd->face, B);
         m.region_type = dupload(reg->type, reg->n_regions);
@@ -975,6 +990,8 @@ void init_finite_volume(void)
         g_dev.partials_cap = std::max(dev::residual_grid(m), dev::dt_partials_grid(m));
         g_dev.partials = dmalloc<double>((size_t)g_dev.partials_cap *
                                          dev::kPartialStride);
+        g_dev.fold_cap = dev::fold_rows(g_dev.partials_cap);
+        g_dev.fold = dmalloc<double>((size_t)g_dev.fold_cap * dev::kPartialStride);
         g_dev.gphi = dmalloc<double>((size_t)10 * B);
         g_dev.sc = dmalloc<dev::DevScalars>(1);
         HIP_CHECK(hipHostMalloc((void **)&g_dev.sc_host, sizeof(dev::DevScalars),
@@ -1082,7 +1099,7 @@ void calc_global_residual(double dt)
             double v = solver_mesh->cells->volume[i];
             for (int j = 0; j < 5; ++j)
             {
-                double r = R[(size_t)j * N + i];
+                double r = R[(size_t)i * 5 + j];
                 tmp[j] += (r < 0 ? -r : r) * v;
             }
         }
@@ -1396,8 +1413,7 @@ double calc_ns_timestep(double time)
         h.t_end = DBL_MAX;
         HIP_CHECK(hipMemcpyAsync(g_dev.sc, &h, sizeof(h), hipMemcpyHostToDevice,
                                  g_dev.stream));
-        dev::launch_finalize(g_dev.stream, g_dev.sc, g_dev.partials,
-                             dev::dt_partials_grid(g_dev.m), phys(), 0, nullptr, 0);
+        dev::launch_finalize(g_dev.stream, g_dev.sc, g_dev.partials, dev::dt_partials_grid(g_dev.m), g_dev.fold, g_dev.fold_cap, phys(), 0, nullptr, 0);
         read_scalars();
         set_viscous_dt(h.viscous);
         return h.dt;
@@ -1559,13 +1575,9 @@ int fvh_get_state(double *U)
     FVH_TRY({
         if (!g_dev.ready)
             throw std::runtime_error("no device state");
-        const int N = g_dev.m.N;
-        std::vector<double> soa((size_t)5 * N);
-        HIP_CHECK(hipMemcpy(soa.data(), g_dev.U[g_dev.cur], soa.size() * 8,
+        HIP_CHECK(hipStreamSynchronize(g_dev.stream));
+        HIP_CHECK(hipMemcpy(U, g_dev.U[g_dev.cur], (size_t)5 * g_dev.m.N * 8,
                             hipMemcpyDeviceToHost));
-        for (int c = 0; c < N; ++c)
-            for (int j = 0; j < 5; ++j)
-                U[(size_t)c * 5 + j] = soa[(size_t)j * N + c];
     });
     return FVH_OK;
 }
@@ -1575,12 +1587,8 @@ int fvh_set_state(const double *U)
     FVH_TRY({
         if (!g_dev.ready)
             throw std::runtime_error("no device state");
-        const int N = g_dev.m.N;
-        std::vector<double> soa((size_t)5 * N);
-        for (int c = 0; c < N; ++c)
-            for (int j = 0; j < 5; ++j)
-                soa[(size_t)j * N + c] = U[(size_t)c * 5 + j];
-        HIP_CHECK(hipMemcpy(g_dev.U[g_dev.cur], soa.data(), soa.size() * 8,
+        HIP_CHECK(hipStreamSynchronize(g_dev.stream));
+        HIP_CHECK(hipMemcpy(g_dev.U[g_dev.cur], U, (size_t)5 * g_dev.m.N * 8,
                             hipMemcpyHostToDevice));
         g_dev.r_valid = false;
     });
@@ -1592,13 +1600,9 @@ int fvh_get_phi_dt(double *R)
     FVH_TRY({
         if (!g_dev.ready || !g_dev.r_valid)
             throw std::runtime_error("no residual on the device");
-        const int N = g_dev.m.N;
-        std::vector<double> soa((size_t)5 * N);
-        HIP_CHECK(hipMemcpy(soa.data(), g_dev.R, soa.size() * 8,
+        HIP_CHECK(hipStreamSynchronize(g_dev.stream));
+        HIP_CHECK(hipMemcpy(R, g_dev.R, (size_t)5 * g_dev.m.N * 8,
                             hipMemcpyDeviceToHost));
-        for (int c = 0; c < N; ++c)
-            for (int j = 0; j < 5; ++j)
-                R[(size_t)c * 5 + j] = soa[(size_t)j * N + c];
     });
     return FVH_OK;
 }
@@ -1628,11 +1632,7 @@ int fvh_get_field(int what, double *out)
         const double *src = what == 0 ? g_dev.grad : g_dev.mnmx;
         const int rows = what == 0 ? 30 : 20;
         HIP_CHECK(hipStreamSynchronize(g_dev.stream));
-        std::vector<double> soa((size_t)rows * N);
-        HIP_CHECK(hipMemcpy(soa.data(), src, soa.size() * 8, hipMemcpyDeviceToHost));
-        for (int c = 0; c < N; ++c)
-            for (int q = 0; q < rows; ++q)
-                out[(size_t)c * rows + q] = soa[(size_t)q * N + c];
+        HIP_CHECK(hipMemcpy(out, src, (size_t)rows * N * 8, hipMemcpyDeviceToHost));
     });
     return FVH_OK;
 }
@@ -1768,8 +1768,7 @@ int fvh_bench_explicit(int n_steps, int n_warmup, double *seconds,
                     g_dev.cur = 1 - g_dev.cur;
                 }
                 HIP_CHECK(hipEventRecord(a, g_dev.stream));
-                dev::launch_finalize(g_dev.stream, g_dev.sc, g_dev.partials,
-                                     dev::residual_grid(g_dev.m), P, 1, g_dev.log,
+                dev::launch_finalize(g_dev.stream, g_dev.sc, g_dev.partials, dev::residual_grid(g_dev.m), g_dev.fold, g_dev.fold_cap, P, 1, g_dev.log,
                                      g_dev.log_cap);
                 HIP_CHECK(hipEventRecord(b, g_dev.stream));
                 HIP_CHECK(hipEventSynchronize(b));
