@@ -9,6 +9,24 @@ c_int, c_double, c_char_p, c_void_p = ctypes.c_int, ctypes.c_double, ctypes.c_ch
 P_DOUBLE = ctypes.POINTER(ctypes.c_double)
 P_INT = ctypes.POINTER(ctypes.c_int)
 
+
+
+class JfnkOptions(ctypes.Structure):
+    """fvh_jfnk_options_t (include/fv3d_hip.h)."""
+    _fields_ = [("max_newton", c_int), ("tol", c_double), ("cfl0", c_double),
+                ("cfl_growth", c_double), ("cfl_cut", c_double), ("cfl_max", c_double),
+                ("krylov_dims", c_int), ("max_restarts", c_int), ("rtol", c_double),
+                ("fd_eps", c_double), ("max_change", c_double), ("precision", c_int),
+                ("precond", c_int), ("verbose", c_int)]
+
+
+class JfnkStats(ctypes.Structure):
+    """fvh_jfnk_stats_t (include/fv3d_hip.h)."""
+    _fields_ = [("status", c_int), ("newton_steps", c_int), ("gmres_iterations", c_int),
+                ("residual_evals", c_int), ("seconds", c_double), ("norm0", c_double),
+                ("norm", c_double), ("l2_drop", c_double)]
+
+
 EXACT_FT = ctypes.CFUNCTYPE(None, c_int, c_double, P_DOUBLE, P_DOUBLE)
 VOID_T_FT = ctypes.CFUNCTYPE(None, c_double)
 DOUBLE_T_FT = ctypes.CFUNCTYPE(c_double, c_double)
@@ -88,6 +106,10 @@ SIGNATURES = [
     ("fvh_last_status", c_int, [P_INT]),
     ("fvh_bench_explicit", c_int, [c_int, c_int, P_DOUBLE, P_DOUBLE]),
     ("fvh_device_info", c_int, [c_char_p, c_int, P_INT, P_DOUBLE]),
+    ("fvh_jfnk_default_options", c_int, [ctypes.POINTER(JfnkOptions)]),
+    ("fvh_jfnk_solve", c_int, [ctypes.POINTER(JfnkOptions), ctypes.POINTER(JfnkStats)]),
+    ("fvh_jfnk_history", c_int, [P_DOUBLE, c_int]),
+    ("fvh_jfnk_blocks", c_int, [c_double, c_int, P_DOUBLE, P_DOUBLE]),
 ]
 
 # exported data symbols (globals) of fv3d_solver.h

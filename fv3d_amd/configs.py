@@ -70,6 +70,21 @@ def bump_channel(nx=160, ny=40, nz=13, mach=0.5, jitter=0.05, seed=20261203,
     )
 
 
+def bump_channel_steady(nx=160, ny=40, nz=13, mach=0.5, jitter=0.05, seed=20261203,
+                        flux="AUSM", reconstruction="Linear"):
+    """The bump channel set up for the steady Newton-Krylov solve: the outlet
+    imposes the free-stream state (region type "state") instead of the
+    zero-gradient copy, and the slopes are unlimited.  With the zero-gradient
+    outlet next to a full-state inlet the steady Jacobian is near-singular
+    (an eigenvalue of 2.7e-3 against a spectral radius of 6e2 on the 32x8x3
+    mesh) and Newton stalls; the Barth-Jespersen limiter's min/max switches
+    stall it below 1e-6 as well."""
+    cfg = bump_channel(nx=nx, ny=ny, nz=nz, mach=mach, jitter=jitter, seed=seed,
+                       reconstruction=reconstruction, limiter="None", flux=flux)
+    cfg["regions"]["outflow"] = ("state", cfg["regions"]["inflow"][1])
+    return cfg
+
+
 def all_boundaries_box(n=(6, 5, 4), tets=True, mu=1e-3, flux="AUSM",
                        reconstruction="Linear", limiter="Barth-Jespersenn", steps=10):
     """Non-periodic box whose six sides carry six different boundary types

@@ -28,7 +28,7 @@
 #include <cmath>
 #include <stdexcept>
 
-#include "fv_device.h"
+#include "fv_physics.h"
 
 // Minimum resident waves per SIMD requested from the register allocator for
 // the two stencil kernels (occupancy against spilling).
@@ -46,228 +46,6 @@
 
 namespace fvh {
 namespace dev {
-
-// reference BM_MIN / BM_MAX / BM_ABS semantics (ternaries, not fmin/fmax)
-__device__ __forceinline__ double rmin(double a, double b) { return a < b ? a : b; }
-__device__ __forceinline__ double rmax(double a, double b) { return a > b ? a : b; }
-__device__ __forceinline__ double rabs(double a) { return a < 0 ? -a : a; }
-
-// basec dot_n: sequential sum from 0.0
-__device__ __forceinline__ double dot3(double a0, double a1, double a2,
-                                       double b0, double b1, double b2)
-{
-    double s = 0.0;
-    s += a0 * b0;
-    s += a1 * b1;
-    s += a2 * b2;
-    return s;
-}
-
-enum
-{
-    BC_FLOW = 0,
-    BC_INFLOW = 1,
-    BC_OUTFLOW = 2,
-    BC_WALL_ADIABATIC = 3,
-    BC_WALL_ISOTHERMAL = 4,
-    BC_WALL_SLIP = 5,
-    BC_SYMMETRY = 6,
-    BC_STATE = 7
-};
-
-// eos.c:81-92
-__device__ __forceinline__ void con_to_prim(const PhysDev &P, double *phi)
-{
-    phi[5] = phi[1] / phi[0];
-    phi[6] = phi[2] / phi[0];
-    phi[7] = phi[3] / phi[0];
-    double p = P.kappa_m1 * (phi[4] - 0.5 * dot3(phi[1], phi[2], phi[3],
-                                                 phi[5], phi[6], phi[7]));
-    p = rmax(1.00E-10, p);
-    phi[8] = p;
-    phi[9] = p / (phi[0] * P.R);
-}
-
-// eos.c:120-127
-__device__ __forceinline__ void prim_to_con(const PhysDev &P, double *phi)
-{
-    phi[1] = phi[5] * phi[0];
-    phi[2] = phi[6] * phi[0];
-    phi[3] = phi[7] * phi[0];
-    phi[4] = P.s_kappa_m1 * phi[8] +
-             0.5 * dot3(phi[5], phi[6], phi[7], phi[1], phi[2], phi[3]);
-}
-
-// eos.c:55-75 (compression branch with the reference's A = 2/((k+1) u_n))
-__device__ __forceinline__ double riemann_p(const PhysDev &P, const double *phi)
-{
-    if (phi[5] <= 0.0)
-    {
-        double c = sqrt(P.kappa * phi[8] / phi[0]);
-        return phi[8] * pow(rmax(1e-4, 1. + 0.5 * P.kappa_m1 * phi[5] / c),
-                            2 * P.kappa * P.s_kappa_m1);
-    }
-    double ar = 2 * P.s_kappa_p1 / phi[5];
-    double br = P.kappa_m1 * P.s_kappa_p1 * phi[8];
-    return phi[8] + phi[5] / ar * 0.5 *
-                        (phi[5] + sqrt(phi[5] * phi[5] + 4. * ar * (phi[8] + br)));
-}
-
-struct Frame
-{
-    double n[3], t1[3], t2[3];
-    double area;
-};
-
-__device__ __forceinline__ void load_frame(const DevMesh &m, int f, Frame &fr)
-{
-    const double2 *g = reinterpret_cast<const double2 *>(m.fgA + (size_t)f * kFgA);
-    const double2 a = g[0], b = g[1], c = g[2], d = g[3], e = g[4];
-    fr.n[0] = a.x, fr.n[1] = a.y, fr.n[2] = b.x;
-    fr.t1[0] = b.y, fr.t1[1] = c.x, fr.t1[2] = c.y;
-    fr.t2[0] = d.x, fr.t2[1] = d.y, fr.t2[2] = e.x;
-    fr.area = e.y;
-}
-
-__device__ __forceinline__ void rotate_back(const Frame &fr, double *g)
-{
-    double tu = g[5] * fr.n[0] + g[6] * fr.t1[0] + g[7] * fr.t2[0];
-    double tv = g[5] * fr.n[1] + g[6] * fr.t1[1] + g[7] * fr.t2[1];
-    double tw = g[5] * fr.n[2] + g[6] * fr.t1[2] + g[7] * fr.t2[2];
-    g[5] = tu;
-    g[6] = tv;
-    g[7] = tw;
-}
-
-// Ghost state of boundary row N+b from the interior state pc
-// (boundary.c:202-319).
-__device__ __forceinline__ void ghost_state(const DevMesh &m, const PhysDev &P,
-                                            int b, const Frame &fr,
-                                            const double *pc, double *g)
-{
-    const int region = m.bnd_region[b];
-    const int type = m.region_type[region];
-#pragma unroll
-    for (int j = 0; j < 10; ++j)
-        g[j] = pc[j];
-    if (type == BC_INFLOW || type == BC_STATE)
-    {
-#pragma unroll
-        for (int j = 0; j < 10; ++j)
-            g[j] = m.region_phi[region * 10 + j];
-    }
-    else if (type == BC_WALL_ADIABATIC || type == BC_WALL_ISOTHERMAL)
-    {
-        g[5] = dot3(pc[5], pc[6], pc[7], fr.n[0], fr.n[1], fr.n[2]);
-        g[6] = dot3(pc[5], pc[6], pc[7], fr.t1[0], fr.t1[1], fr.t1[2]);
-        g[7] = dot3(pc[5], pc[6], pc[7], fr.t2[0], fr.t2[1], fr.t2[2]);
-        g[8] = riemann_p(P, g);
-        g[5] = 0.0;
-        g[6] = 0.0;
-        g[7] = 0.0;
-        if (type == BC_WALL_ISOTHERMAL)
-            g[9] = m.region_phi[region * 10 + 9];
-        g[0] = g[8] / (P.R * g[9]);
-        rotate_back(fr, g);
-    }
-    else if (type == BC_WALL_SLIP || type == BC_SYMMETRY)
-    {
-        g[5] = dot3(pc[5], pc[6], pc[7], fr.n[0], fr.n[1], fr.n[2]);
-        g[6] = dot3(pc[5], pc[6], pc[7], fr.t1[0], fr.t1[1], fr.t1[2]);
-        g[7] = dot3(pc[5], pc[6], pc[7], fr.t2[0], fr.t2[1], fr.t2[2]);
-        g[8] = riemann_p(P, g);
-        g[5] = 0.0;
-        g[9] = g[8] / (g[0] * P.R);
-        rotate_back(fr, g);
-    }
-    prim_to_con(P, g);
-}
-
-// Ghost gradient (boundary.c:361-406): a copy of the interior gradient, with
-// the normal component removed for slip walls and symmetry.
-__device__ __forceinline__ void ghost_grad(int type, const Frame &fr, double gx,
-                                           double gy, double gz, double &ox,
-                                           double &oy, double &oz)
-{
-    if (type == BC_WALL_SLIP || type == BC_SYMMETRY)
-    {
-        double ty = gx * fr.t1[0] + gy * fr.t1[1] + gz * fr.t1[2];
-        double tz = gx * fr.t2[0] + gy * fr.t2[1] + gz * fr.t2[2];
-        double tx = 0.0;
-        ox = tx * fr.n[0] + ty * fr.t1[0] + tz * fr.t2[0];
-        oy = tx * fr.n[1] + ty * fr.t1[1] + tz * fr.t2[1];
-        oz = tx * fr.n[2] + ty * fr.t1[2] + tz * fr.t2[2];
-    }
-    else
-    {
-        ox = gx;
-        oy = gy;
-        oz = gz;
-    }
-}
-
-__device__ __forceinline__ void load_cons(const double *__restrict__ U, int c,
-                                          double *phi)
-{
-    const double *p = U + (size_t)c * 5;
-#pragma unroll
-    for (int j = 0; j < 5; ++j)
-        phi[j] = p[j];
-}
-
-// Workgroup -> cell tile mapping.  The hardware hands consecutive
-// workgroups to the 8 XCDs in turn; this gives each XCD one contiguous range
-// of cell tiles so that the neighbour data a tile gathers is mostly in that
-// XCD's own L2.  A bijection on [0, nblk) for any nblk.
-__device__ __forceinline__ int xcd_tile(int nblk)
-{
-    const int b = blockIdx.x;
-    const int x = b & 7, i = b >> 3;
-    const int q = nblk >> 3, r = nblk & 7;
-    return x * q + (x < r ? x : r) + i;
-}
-
-// Viscous gradient subset: momentum (vars 1,2,3) and T (9) per direction,
-// v[d*4 + q] = d/dx_d of var {1,2,3,9}[q].
-struct ViscGrad
-{
-    double v[12];
-};
-
-__device__ __forceinline__ int visc_var(int q) { return q < 3 ? q + 1 : 9; }
-
-__device__ __forceinline__ void load_vg(const double *__restrict__ vg, int c,
-                                        ViscGrad &g)
-{
-    const double2 *p = reinterpret_cast<const double2 *>(vg + (size_t)c * 12);
-#pragma unroll
-    for (int q = 0; q < 6; ++q)
-    {
-        const double2 v = p[q];
-        g.v[2 * q] = v.x;
-        g.v[2 * q + 1] = v.y;
-    }
-}
-
-// Limited linear extrapolation to a face (reconstruction.c:193-200 with
-// limiter.c:86-105: min(1, y) of the per-face-side Barth-Jespersen ratio).
-__device__ __forceinline__ double limited(double phi, double gx, double gy,
-                                          double gz, const double *r, double mn,
-                                          double mx, int bj)
-{
-    const double slope = r[0] * gx + r[1] * gy + r[2] * gz;
-    double lim = 1.0;
-    if (bj)
-    {
-        double y = 1.0;
-        if (slope > 0)
-            y = (mx - phi) / slope;
-        else if (slope < 0)
-            y = (mn - phi) / slope;
-        lim = rmin(1.0, y);
-    }
-    return phi + lim * slope;
-}
 
 // ---------------------------------------------------------------------------
 // Ghost rows N+b (boundary.c:188-320 and the update hook ns_update,
@@ -483,206 +261,6 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
         d[2] = make_double2(st[4], st[5]);
         d[3] = make_double2(st[6], st[7]);
     }
-}
-
-// ---------------------------------------------------------------------------
-// Fluxes (flux.c)
-// ---------------------------------------------------------------------------
-
-// flux.c:203-210
-__device__ __forceinline__ void euler_flux_1d(const double *phi, double *f)
-{
-    f[0] = phi[1];
-    f[1] = phi[1] * phi[5] + phi[8];
-    f[2] = phi[1] * phi[6];
-    f[3] = phi[1] * phi[7];
-    f[4] = (phi[4] + phi[8]) * phi[5];
-}
-
-// flux.c:320-336
-__device__ __forceinline__ void riemann_rusanov(const PhysDev &P, const double *l,
-                                                const double *r, double *f)
-{
-    const double c_l = sqrt(P.kappa * l[8] / l[0]);
-    const double c_r = sqrt(P.kappa * r[8] / r[0]);
-    const double eig = rmax(rabs(l[5]) + c_l, rabs(r[5]) + c_r);
-    double fl[5], fr[5];
-    euler_flux_1d(l, fl);
-    euler_flux_1d(r, fr);
-#pragma unroll
-    for (int j = 0; j < 5; ++j)
-        f[j] = 0.5 * (fl[j] + fr[j]) - 0.5 * eig * (r[j] - l[j]);
-}
-
-// flux.c:344-406
-__device__ __forceinline__ void riemann_ausm(const PhysDev &P, const double *l,
-                                             const double *r, double *f)
-{
-    const double c_l = sqrt(P.kappa * l[8] / l[0]);
-    const double c_r = sqrt(P.kappa * r[8] / r[0]);
-    const double M_l = l[5] / c_l;
-    const double M_r = r[5] / c_r;
-    const double H_l = (l[4] + l[8]) / l[0];
-    const double H_r = (r[4] + r[8]) / r[0];
-    double M_p, P_p, M_m, P_m;
-    if (M_l <= -1.0)
-    {
-        M_p = 0.0;
-        P_p = 0.0;
-    }
-    else if (M_l < 1.0)
-    {
-        M_p = 0.25 * (M_l + 1.0) * (M_l + 1.0);
-        P_p = 0.25 * l[8] * (1.0 + M_l) * (1.0 + M_l) * (2.0 - M_l);
-    }
-    else
-    {
-        M_p = M_l;
-        P_p = l[8];
-    }
-    if (M_r <= -1.0)
-    {
-        M_m = M_r;
-        P_m = r[8];
-    }
-    else if (M_r < 1.0)
-    {
-        M_m = -0.25 * (M_r - 1.0) * (M_r - 1.0);
-        P_m = 0.25 * r[8] * (1.0 - M_r) * (1.0 - M_r) * (2.0 + M_r);
-    }
-    else
-    {
-        M_m = 0.0;
-        P_m = 0.0;
-    }
-    const double mp = rmax(0.0, M_p + M_m);
-    const double mm = rmin(0.0, M_p + M_m);
-    f[0] = mp * c_l * l[0];
-    f[1] = mp * c_l * l[0] * l[5] + P_p;
-    f[2] = mp * c_l * l[0] * l[6];
-    f[3] = mp * c_l * l[0] * l[7];
-    f[4] = mp * c_l * l[0] * H_l;
-    f[0] += mm * c_r * r[0];
-    f[1] += mm * c_r * r[0] * r[5] + P_m;
-    f[2] += mm * c_r * r[0] * r[6];
-    f[3] += mm * c_r * r[0] * r[7];
-    f[4] += mm * c_r * r[0] * H_r;
-}
-
-// flux.c:215-263 with the reference's use of momentum gradients as velocity
-// gradients; phi holds the face state (reconstructed primitives at 5..7).
-__device__ __forceinline__ void viscous_1d(const PhysDev &P, const double *phi,
-                                           const ViscGrad &g, double *f,
-                                           double *gg, double *h)
-{
-    const double s_23 = 2.0 / 3.0;
-    const double s_43 = 4.0 / 3.0;
-    const double mu = P.mu;
-    const double gx1 = g.v[0], gx2 = g.v[1], gx3 = g.v[2], gxT = g.v[3];
-    const double gy1 = g.v[4], gy2 = g.v[5], gy3 = g.v[6], gyT = g.v[7];
-    const double gz1 = g.v[8], gz2 = g.v[9], gz3 = g.v[10], gzT = g.v[11];
-    const double tau_xx = mu * (s_43 * gx1 - s_23 * gy2 - s_23 * gz3);
-    const double tau_yy = mu * (-s_23 * gx1 + s_43 * gy2 - s_23 * gz3);
-    const double tau_zz = mu * (-s_23 * gx1 - s_23 * gy2 + s_43 * gz3);
-    const double tau_xy = mu * (gy1 + gx2);
-    const double tau_xz = mu * (gz1 + gx3);
-    const double tau_yz = mu * (gz2 + gy3);
-    f[0] = 0.0;
-    f[1] = -tau_xx;
-    f[2] = -tau_xy;
-    f[3] = -tau_xz;
-    f[4] = -tau_xx * phi[5] - tau_xy * phi[6] - tau_xz * phi[7] - P.lam * gxT;
-    gg[0] = 0.0;
-    gg[1] = -tau_xy;
-    gg[2] = -tau_yy;
-    gg[3] = -tau_yz;
-    gg[4] = -tau_xy * phi[5] - tau_yy * phi[6] - tau_yz * phi[7] - P.lam * gyT;
-    h[0] = 0.0;
-    h[1] = -tau_xz;
-    h[2] = -tau_yz;
-    h[3] = -tau_zz;
-    h[4] = -tau_xz * phi[5] - tau_yz * phi[6] - tau_zz * phi[7] - P.lam * gzT;
-}
-
-// One face flux in the face's own orientation (flux.c:42-111/113-197).
-template <int FLUX, bool VISC>
-__device__ __forceinline__ void face_flux(const PhysDev &P, const Frame &fr,
-                                          const double *L, const double *R,
-                                          const ViscGrad &gL, const ViscGrad &gR,
-                                          double *out)
-{
-    double lr[10], rr[10];
-    lr[0] = L[0];
-    lr[1] = dot3(L[1], L[2], L[3], fr.n[0], fr.n[1], fr.n[2]);
-    lr[2] = dot3(L[1], L[2], L[3], fr.t1[0], fr.t1[1], fr.t1[2]);
-    lr[3] = dot3(L[1], L[2], L[3], fr.t2[0], fr.t2[1], fr.t2[2]);
-    lr[4] = L[4];
-    con_to_prim(P, lr);
-    rr[0] = R[0];
-    rr[1] = dot3(R[1], R[2], R[3], fr.n[0], fr.n[1], fr.n[2]);
-    rr[2] = dot3(R[1], R[2], R[3], fr.t1[0], fr.t1[1], fr.t1[2]);
-    rr[3] = dot3(R[1], R[2], R[3], fr.t2[0], fr.t2[1], fr.t2[2]);
-    rr[4] = R[4];
-    con_to_prim(P, rr);
-
-    double fc[5];
-    if (FLUX == FluxRusanov)
-        riemann_rusanov(P, lr, rr, fc);
-    else
-        riemann_ausm(P, lr, rr, fc);
-
-    out[0] = fc[0];
-    out[1] = fc[1] * fr.n[0] + fc[2] * fr.t1[0] + fc[3] * fr.t2[0];
-    out[2] = fc[1] * fr.n[1] + fc[2] * fr.t1[1] + fc[3] * fr.t2[1];
-    out[3] = fc[1] * fr.n[2] + fc[2] * fr.t1[2] + fc[3] * fr.t2[2];
-    out[4] = fc[4];
-
-    if (VISC)
-    {
-        double fl[5], gl[5], hl[5], fr5[5], gr[5], hr[5];
-        viscous_1d(P, L, gL, fl, gl, hl);
-        viscous_1d(P, R, gR, fr5, gr, hr);
-#pragma unroll
-        for (int j = 0; j < 5; ++j)
-        {
-            const double fx = 0.5 * (fl[j] + fr5[j]);
-            const double fy = 0.5 * (gl[j] + gr[j]);
-            const double fz = 0.5 * (hl[j] + hr[j]);
-            out[j] += fx * fr.n[0] + fy * fr.n[1] + fz * fr.n[2];
-        }
-    }
-}
-
-__device__ __forceinline__ void block_sum(double *sh, double &v)
-{
-    const int t = threadIdx.x;
-    sh[t] = v;
-    __syncthreads();
-#pragma unroll
-    for (int s = kBlock / 2; s > 0; s >>= 1)
-    {
-        if (t < s)
-            sh[t] = sh[t] + sh[t + s];
-        __syncthreads();
-    }
-    v = sh[0];
-    __syncthreads();
-}
-
-__device__ __forceinline__ void block_min(double *sh, double &v)
-{
-    const int t = threadIdx.x;
-    sh[t] = v;
-    __syncthreads();
-#pragma unroll
-    for (int s = kBlock / 2; s > 0; s >>= 1)
-    {
-        if (t < s)
-            sh[t] = rmin(sh[t], sh[t + s]);
-        __syncthreads();
-    }
-    v = sh[0];
-    __syncthreads();
 }
 
 // ---------------------------------------------------------------------------

@@ -1,0 +1,395 @@
+// Steady Newton-Krylov (JFNK) driver: pseudo-transient Newton steps on the
+// device state, each solved by right-preconditioned restarted GMRES with a
+// matrix-free finite-difference operator.  Vectors live on the device; the
+// small Hessenberg least-squares problem (Givens rotations) runs on the host
+// from the dot products copied back once per Arnoldi step.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <stdexcept>
+#include <vector>
+
+#include "fvh_internal.h"
+#include "krylov.h"
+#include "runtime.h"
+
+namespace fvh {
+
+namespace {
+
+// Device work buffers, kept between solves of the same size.
+struct Workspace
+{
+    int N = 0, m = 0, prec = -1;
+    size_t ld = 0;
+    std::vector<void *> allocs;
+    double *R0 = nullptr, *Rt = nullptr, *Ut = nullptr, *dtau = nullptr,
+           *Minv = nullptr, *xacc = nullptr, *dU = nullptr, *partials = nullptr,
+           *dots = nullptr, *ydev = nullptr;
+    void *V = nullptr, *z = nullptr, *t = nullptr; // T storage
+    double *host = nullptr;                        // pinned, kHost doubles
+};
+
+constexpr int kHost = 4 * kry::kMaxDots + 64;
+// offsets in dots / host
+constexpr int kH1 = 0, kH2 = kry::kMaxDots, kWW = 2 * kry::kMaxDots,
+              kZZ = kWW + 1, kSums = kWW + 2, kRatio = kWW + 8, kEps = kWW + 9,
+              kUU = kWW + 10;
+
+Workspace g_ws;
+std::vector<double> g_hist;
+
+template <class X> X *walloc(size_t n)
+{
+    void *p = nullptr;
+    FVH_HIP_CHECK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(X)));
+    g_ws.allocs.push_back(p);
+    return static_cast<X *>(p);
+}
+
+void release()
+{
+    for (void *p : g_ws.allocs)
+        (void)hipFree(p);
+    if (g_ws.host)
+        (void)hipHostFree(g_ws.host);
+    g_ws = Workspace();
+}
+
+void prepare(int N, int m, int prec)
+{
+    if (g_ws.N == N && g_ws.m == m && g_ws.prec == prec)
+        return;
+    release();
+    g_ws.N = N;
+    g_ws.m = m;
+    g_ws.prec = prec;
+    const size_t n = (size_t)5 * N;
+    g_ws.ld = (n + 63) / 64 * 64;
+    const size_t es = prec == FVH_JFNK_FP32 ? sizeof(float) : sizeof(double);
+    g_ws.R0 = walloc<double>(n);
+    g_ws.Rt = walloc<double>(n);
+    g_ws.Ut = walloc<double>(n);
+    g_ws.dtau = walloc<double>(N);
+    g_ws.Minv = walloc<double>((size_t)25 * N);
+    g_ws.xacc = walloc<double>(n);
+    g_ws.dU = walloc<double>(n);
+    g_ws.partials = walloc<double>((size_t)kry::kDotBlocks * kry::kMaxDots);
+    g_ws.dots = walloc<double>(kHost);
+    g_ws.ydev = walloc<double>(kry::kMaxDots);
+    g_ws.V = walloc<char>((size_t)(m + 1) * g_ws.ld * es);
+    g_ws.z = walloc<char>(g_ws.ld * es);
+    g_ws.t = walloc<char>(g_ws.ld * es);
+    FVH_HIP_CHECK(hipHostMalloc((void **)&g_ws.host, kHost * sizeof(double),
+                                hipHostMallocDefault));
+}
+
+void fetch(hipStream_t s, int off, int count)
+{
+    FVH_HIP_CHECK(hipMemcpyAsync(g_ws.host + off, g_ws.dots + off, count * sizeof(double),
+                                 hipMemcpyDeviceToHost, s));
+    FVH_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+// stop norm max_j (1/V_glob) sum_i |R_ij| V_i and |R|_2 of g_ws.R0
+void residual_norms(const RuntimeView &v, double &norm, double &l2)
+{
+    const int n = 5 * v.m.N;
+    kry::residual_sums(v.stream, v.m.N, g_ws.R0, v.m.vol, g_ws.partials, g_ws.dots + kSums);
+    kry::multidot<double>(v.stream, n, 1, g_ws.R0, g_ws.ld, g_ws.R0, g_ws.partials,
+                          g_ws.dots + kWW);
+    FVH_HIP_CHECK(hipMemcpyAsync(g_ws.host + kWW, g_ws.dots + kWW, 7 * sizeof(double),
+                                 hipMemcpyDeviceToHost, v.stream));
+    FVH_HIP_CHECK(hipStreamSynchronize(v.stream));
+    norm = 0.0;
+    for (int j = 0; j < 5; ++j)
+        norm = std::max(norm, g_ws.host[kSums + j] / v.vglob);
+    l2 = std::sqrt(g_ws.host[kWW]);
+    if (!std::isfinite(norm) || !std::isfinite(l2))
+        norm = l2 = NAN;
+}
+
+struct GmresResult
+{
+    int its = 0, evals = 0;
+    double rel = 1.0;
+};
+
+// Solve (I/dtau - J) M^-1 y = R0 approximately; xacc = y (dU = M^-1 xacc).
+template <class T>
+GmresResult gmres(const RuntimeView &v, const fvh_jfnk_options_t &o, double fd_num)
+{
+    hipStream_t s = v.stream;
+    const int N = v.m.N, n = 5 * N, m = o.krylov_dims;
+    const size_t ld = g_ws.ld;
+    T *V = static_cast<T *>(g_ws.V), *z = static_cast<T *>(g_ws.z),
+      *t = static_cast<T *>(g_ws.t);
+    double *dots = g_ws.dots, *host = g_ws.host;
+    GmresResult res;
+    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
+    FVH_HIP_CHECK(hipMemsetAsync(g_ws.xacc, 0, (size_t)n * sizeof(double), s));
+    double beta0 = 0.0;
+
+    // z = M^-1 (vector in T), w = A z into out
+    auto apply_op = [&](const T *zin, T *out) {
+        kry::multidot<T>(s, n, 1, zin, ld, zin, g_ws.partials, dots + kZZ);
+        kry::fd_eps(s, dots + kZZ, fd_num, dots + kEps);
+        kry::perturb<T>(s, n, v.U, dots + kEps, zin, g_ws.Ut);
+        eval_residual(g_ws.Ut, g_ws.Rt);
+        kry::op_finish<T>(s, N, zin, g_ws.dtau, g_ws.Rt, g_ws.R0, dots + kEps, out);
+        ++res.evals;
+    };
+
+    for (int cycle = 0; cycle <= o.max_restarts; ++cycle)
+    {
+        // r = b - A M^-1 x into V_0
+        if (cycle == 0)
+            kry::convert<T>(s, n, g_ws.R0, V);
+        else
+        {
+            kry::convert<T>(s, n, g_ws.xacc, t);
+            kry::precond_apply<T>(s, N, g_ws.Minv, t, z);
+            apply_op(z, V);
+            kry::rhs_minus<T>(s, n, g_ws.R0, V);
+        }
+        kry::multidot<T>(s, n, 1, V, ld, V, g_ws.partials, dots + kWW);
+        fetch(s, kWW, 1);
+        const double beta = std::sqrt(host[kWW]);
+        if (cycle == 0)
+            beta0 = beta;
+        if (!(beta > 0.0) || !std::isfinite(beta))
+        {
+            res.rel = beta0 > 0.0 ? beta / beta0 : 0.0;
+            break;
+        }
+        kry::scale<T>(s, n, V, 1.0 / beta, V);
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        int k = 0;
+        bool done = false;
+        for (int j = 0; j < m; ++j)
+        {
+            T *w = V + (size_t)(j + 1) * ld;
+            kry::precond_apply<T>(s, N, g_ws.Minv, V + (size_t)j * ld, z);
+            apply_op(z, w);
+            // classical Gram-Schmidt, twice
+            kry::multidot<T>(s, n, j + 1, V, ld, w, g_ws.partials, dots + kH1);
+            kry::multiaxpy<T>(s, n, j + 1, V, ld, dots + kH1, w);
+            kry::multidot<T>(s, n, j + 1, V, ld, w, g_ws.partials, dots + kH2);
+            kry::multiaxpy<T>(s, n, j + 1, V, ld, dots + kH2, w);
+            kry::multidot<T>(s, n, 1, w, ld, w, g_ws.partials, dots + kWW);
+            fetch(s, 0, kWW + 1);
+            ++res.its;
+            double *Hj = &H[(size_t)j * (m + 1)]; // column j
+            for (int i = 0; i <= j; ++i)
+                Hj[i] = host[kH1 + i] + host[kH2 + i];
+            const double hn = std::sqrt(host[kWW]);
+            Hj[j + 1] = hn;
+            for (int i = 0; i < j; ++i)
+            {
+                const double a = Hj[i], b = Hj[i + 1];
+                Hj[i] = cs[i] * a + sn[i] * b;
+                Hj[i + 1] = -sn[i] * a + cs[i] * b;
+            }
+            const double a = Hj[j], b = Hj[j + 1];
+            const double r = std::hypot(a, b);
+            cs[j] = r > 0.0 ? a / r : 1.0;
+            sn[j] = r > 0.0 ? b / r : 0.0;
+            Hj[j] = r;
+            Hj[j + 1] = 0.0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            k = j + 1;
+            res.rel = std::fabs(g[j + 1]) / beta0;
+            if (!std::isfinite(res.rel))
+                throw std::runtime_error("GMRES: non-finite Arnoldi coefficients");
+            if (res.rel <= o.rtol || !(hn > 1e-300 * beta0))
+            {
+                done = true;
+                break;
+            }
+            kry::scale<T>(s, n, w, 1.0 / hn, w);
+        }
+        // y = H^-1 g (upper triangular, k x k)
+        for (int i = k - 1; i >= 0; --i)
+        {
+            double a = g[i];
+            for (int q = i + 1; q < k; ++q)
+                a -= H[(size_t)q * (m + 1) + i] * y[q];
+            y[i] = a / H[(size_t)i * (m + 1) + i];
+        }
+        for (int i = 0; i < k; ++i)
+            host[kH1 + i] = y[i];
+        FVH_HIP_CHECK(hipMemcpyAsync(g_ws.ydev, host + kH1, k * sizeof(double),
+                                     hipMemcpyHostToDevice, s));
+        kry::combine<T>(s, n, k, V, ld, g_ws.ydev, t);
+        kry::accumulate<T>(s, n, t, g_ws.xacc);
+        FVH_HIP_CHECK(hipStreamSynchronize(s)); // host y reused next cycle
+        if (done)
+            break;
+    }
+    return res;
+}
+
+int solve(const fvh_jfnk_options_t &o, fvh_jfnk_stats_t &st)
+{
+    if (o.krylov_dims < 1 || o.krylov_dims + 1 > kry::kMaxDots)
+        throw std::runtime_error("krylov_dims must be in [1, 63]");
+    if (o.precision != FVH_JFNK_FP64 && o.precision != FVH_JFNK_FP32)
+        throw std::runtime_error("unknown precision");
+    if (o.precond < FVH_PRECOND_SCALAR || o.precond > FVH_PRECOND_BLOCK)
+        throw std::runtime_error("unknown or unavailable preconditioner");
+    RuntimeView v = runtime_view();
+    const int N = v.m.N, n = 5 * N;
+    prepare(N, o.krylov_dims, o.precision);
+    hipStream_t s = v.stream;
+    FVH_HIP_CHECK(hipStreamSynchronize(s));
+    g_hist.clear();
+    st = fvh_jfnk_stats_t{};
+
+    const auto t0 = std::chrono::steady_clock::now();
+    auto elapsed = [&]() {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    };
+    eval_residual(v.U, g_ws.R0);
+    st.residual_evals = 1;
+    double norm, l2;
+    residual_norms(v, norm, l2);
+    st.norm0 = norm;
+    const double l20 = l2;
+    double cfl = o.cfl0;
+    if (o.verbose)
+        std::printf(" newton  stop-norm      cfl        gmres  rel        alpha\n");
+    int it = 0;
+    for (; it < o.max_newton; ++it)
+    {
+        if (!std::isfinite(norm))
+            throw std::runtime_error("non-finite residual in the Newton loop");
+        if (norm <= o.tol)
+            break;
+        kry::local_dt(s, v.m, v.P, v.U, cfl, g_ws.dtau);
+        kry::precond_blocks(s, v.m, v.P, v.flux, v.U, g_ws.dtau,
+                            o.precond == FVH_PRECOND_BLOCK ? 1 : 0, g_ws.Minv);
+        kry::multidot<double>(s, n, 1, v.U, g_ws.ld, v.U, g_ws.partials, g_ws.dots + kUU);
+        fetch(s, kUU, 1);
+        const double fd_num = o.fd_eps * (1.0 + std::sqrt(g_ws.host[kUU]));
+        GmresResult gr = o.precision == FVH_JFNK_FP32 ? gmres<float>(v, o, fd_num)
+                                                      : gmres<double>(v, o, fd_num);
+        st.gmres_iterations += gr.its;
+        st.residual_evals += gr.evals;
+        kry::precond_apply<double>(s, N, g_ws.Minv, g_ws.xacc, g_ws.dU);
+        kry::update_ratio(s, N, v.U, g_ws.dU, g_ws.partials, g_ws.dots + kRatio);
+        fetch(s, kRatio, 1);
+        const double ratio = g_ws.host[kRatio];
+        const double alpha = ratio > o.max_change ? o.max_change / ratio : 1.0;
+        if (!std::isfinite(alpha))
+            throw std::runtime_error("non-finite Newton update");
+        kry::update(s, n, v.U, g_ws.dU, alpha);
+        const double prev = norm;
+        eval_residual(v.U, g_ws.R0);
+        ++st.residual_evals;
+        residual_norms(v, norm, l2);
+        g_hist.insert(g_hist.end(), {prev, cfl, (double)gr.its, gr.rel, alpha, elapsed()});
+        if (o.verbose)
+            std::printf("%7d  %.6e  %.3e  %5d  %.3e  %.3f\n", it + 1, norm, cfl, gr.its,
+                        gr.rel, alpha);
+        cfl = norm < 3.0 * prev ? std::min(cfl * o.cfl_growth, o.cfl_max)
+                                : std::max(cfl * o.cfl_cut, 1.0);
+    }
+    FVH_HIP_CHECK(hipStreamSynchronize(s));
+    st.seconds = elapsed();
+    st.newton_steps = it;
+    st.norm = norm;
+    st.l2_drop = l20 > 0.0 ? l2 / l20 : 0.0;
+    st.status = norm <= o.tol ? 0 : 1;
+    state_changed();
+    if (o.verbose)
+        std::fflush(stdout);
+    return FVH_OK;
+}
+
+} // namespace
+
+void jfnk_release() { release(); }
+
+} // namespace fvh
+
+using namespace fvh;
+
+int fvh_jfnk_default_options(fvh_jfnk_options_t *o)
+{
+    if (!o)
+        return FVH_ERR_ARG;
+    o->max_newton = 300;
+    o->tol = 1e-8;
+    o->cfl0 = 10.0;
+    o->cfl_growth = 1.5;
+    o->cfl_cut = 0.3;
+    o->cfl_max = 1e6;
+    o->krylov_dims = 30;
+    o->max_restarts = 3;
+    o->rtol = 1e-2;
+    o->fd_eps = 1e-7;
+    o->max_change = 0.2;
+    o->precision = FVH_JFNK_FP64;
+    o->precond = FVH_PRECOND_BLOCK;
+    o->verbose = 0;
+    return FVH_OK;
+}
+
+int fvh_jfnk_solve(const fvh_jfnk_options_t *opt, fvh_jfnk_stats_t *stats)
+{
+    if (!opt || !stats)
+    {
+        set_error("fvh_jfnk_solve: null argument");
+        return FVH_ERR_ARG;
+    }
+    try
+    {
+        return solve(*opt, *stats);
+    }
+    catch (const std::exception &e)
+    {
+        set_error(e.what());
+        return FVH_ERR_STATE;
+    }
+}
+
+int fvh_jfnk_blocks(double cfl, int precond, double *dtau, double *minv)
+{
+    try
+    {
+        if (precond != FVH_PRECOND_SCALAR && precond != FVH_PRECOND_BLOCK)
+            throw std::runtime_error("fvh_jfnk_blocks: unknown preconditioner");
+        RuntimeView v = runtime_view();
+        const int N = v.m.N;
+        prepare(N, g_ws.m > 0 ? g_ws.m : 30, g_ws.prec >= 0 ? g_ws.prec : FVH_JFNK_FP64);
+        kry::local_dt(v.stream, v.m, v.P, v.U, cfl, g_ws.dtau);
+        kry::precond_blocks(v.stream, v.m, v.P, v.flux, v.U, g_ws.dtau,
+                            precond == FVH_PRECOND_BLOCK ? 1 : 0, g_ws.Minv);
+        FVH_HIP_CHECK(hipStreamSynchronize(v.stream));
+        if (dtau)
+            FVH_HIP_CHECK(hipMemcpy(dtau, g_ws.dtau, (size_t)N * sizeof(double),
+                                    hipMemcpyDeviceToHost));
+        if (minv)
+            FVH_HIP_CHECK(hipMemcpy(minv, g_ws.Minv, (size_t)25 * N * sizeof(double),
+                                    hipMemcpyDeviceToHost));
+    }
+    catch (const std::exception &e)
+    {
+        set_error(e.what());
+        return FVH_ERR_STATE;
+    }
+    return FVH_OK;
+}
+
+int fvh_jfnk_history(double *rows, int max_rows)
+{
+    const int n = (int)(g_hist.size() / 6);
+    if (rows)
+        std::copy(g_hist.begin(), g_hist.begin() + (size_t)std::min(n, max_rows) * 6, rows);
+    return n;
+}

@@ -1,0 +1,613 @@
+// Kernels of the Newton-Krylov (JFNK) path: pseudo time steps, block-Jacobi
+// preconditioner blocks, the finite-difference operator and the
+// deterministic vector reductions of GMRES.
+#include <cfloat>
+#include <cmath>
+#include <stdexcept>
+
+#include "fv_physics.h"
+#include "krylov.h"
+
+namespace fvh {
+namespace kry {
+
+using dev::DevMesh;
+using dev::PhysDev;
+using dev::kBlock;
+
+namespace {
+
+inline int blocks_for(long n) { return (int)((n + kBlock - 1) / kBlock); }
+
+__global__ __launch_bounds__(kBlock) void k_local_dt(DevMesh m, PhysDev P,
+                                                     const double *__restrict__ U,
+                                                     double cfl,
+                                                     double *__restrict__ dtau)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.N)
+        return;
+    double pc[10];
+    dev::load_cons(U, c, pc);
+    dev::con_to_prim(P, pc);
+    const double *dx = m.cdx + (size_t)c * 3;
+    const double s_rho = 1.0 / pc[0];
+    const double cs = sqrt(P.kappa * pc[8] * s_rho);
+    const double ds1 = (dev::rabs(pc[5]) + cs) * dx[0] + (dev::rabs(pc[6]) + cs) * dx[1] +
+                       (dev::rabs(pc[7]) + cs) * dx[2];
+    dtau[c] = cfl * (m.vol[c] / ds1);
+}
+
+// In-place inverse of a 5x5 row-major matrix (Gauss-Jordan, partial pivoting).
+__device__ void invert5(double *A, double *Ainv)
+{
+    double M[5][10];
+#pragma unroll
+    for (int r = 0; r < 5; ++r)
+    {
+#pragma unroll
+        for (int q = 0; q < 5; ++q)
+        {
+            M[r][q] = A[r * 5 + q];
+            M[r][5 + q] = r == q ? 1.0 : 0.0;
+        }
+    }
+#pragma unroll
+    for (int col = 0; col < 5; ++col)
+    {
+        int p = col;
+        double best = fabs(M[col][col]);
+#pragma unroll
+        for (int r = col + 1; r < 5; ++r)
+            if (fabs(M[r][col]) > best)
+            {
+                best = fabs(M[r][col]);
+                p = r;
+            }
+        // row swap with compile-time indices only (no private-memory arrays)
+#pragma unroll
+        for (int r = col + 1; r < 5; ++r)
+            if (r == p)
+#pragma unroll
+                for (int q = 0; q < 10; ++q)
+                {
+                    const double t = M[col][q];
+                    M[col][q] = M[r][q];
+                    M[r][q] = t;
+                }
+        const double inv = 1.0 / M[col][col];
+#pragma unroll
+        for (int q = 0; q < 10; ++q)
+            M[col][q] *= inv;
+#pragma unroll
+        for (int r = 0; r < 5; ++r)
+        {
+            if (r == col)
+                continue;
+            const double fct = M[r][col];
+#pragma unroll
+            for (int q = 0; q < 10; ++q)
+                M[r][q] -= fct * M[col][q];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 5; ++r)
+#pragma unroll
+        for (int q = 0; q < 5; ++q)
+            Ainv[r * 5 + q] = M[r][5 + q];
+}
+
+// First-order inviscid residual of cell c for the own state u (neighbours
+// from U, ghosts from u): r = -(sum_f +-F_f A_f) / V.
+template <int FLUX, int MAXF>
+__device__ void first_order_residual(const DevMesh &m, const PhysDev &P,
+                                     const double *__restrict__ U, int c,
+                                     const double *u10, double *r)
+{
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    const int nf = m.ncf[c];
+    const dev::ViscGrad none{};
+#pragma unroll 1
+    for (int s = 0; s < MAXF; ++s)
+    {
+        if (s >= nf)
+            break;
+        const int f = m.cf[(size_t)c * m.maxf + s];
+        const int c0 = m.fcell[2 * f], c1 = m.fcell[2 * f + 1];
+        const bool own0 = c0 == c;
+        const int nb = own0 ? c1 : c0;
+        dev::Frame fr;
+        dev::load_frame(m, f, fr);
+        double pn[10];
+        if (nb < m.N)
+        {
+            dev::load_cons(U, nb, pn);
+            dev::con_to_prim(P, pn);
+        }
+        else
+        {
+            dev::ghost_state(m, P, nb - m.N, fr, u10, pn);
+        }
+        double L[10], R[10];
+#pragma unroll
+        for (int j = 0; j < 10; ++j)
+        {
+            L[j] = own0 ? u10[j] : pn[j];
+            R[j] = own0 ? pn[j] : u10[j];
+        }
+        double fl[5];
+        dev::face_flux<FLUX, false>(P, fr, L, R, none, none, fl);
+#pragma unroll
+        for (int j = 0; j < 5; ++j)
+        {
+            const double v = fl[j] * fr.area;
+            acc[j] = acc[j] + (own0 ? v : -v);
+        }
+    }
+    const double vol = m.vol[c];
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+        r[j] = -acc[j] / vol;
+}
+
+template <int FLUX, int MAXF>
+__global__ __launch_bounds__(kBlock) void k_precond_blocks(
+    DevMesh m, PhysDev P, const double *__restrict__ U,
+    const double *__restrict__ dtau, int blocks, double *__restrict__ Minv)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.N)
+        return;
+    double *out = Minv + (size_t)c * 25;
+    const double idt = 1.0 / dtau[c];
+    if (!blocks)
+    {
+#pragma unroll
+        for (int q = 0; q < 25; ++q)
+            out[q] = (q % 6 == 0) ? dtau[c] : 0.0;
+        return;
+    }
+    double uc[10];
+    dev::load_cons(U, c, uc);
+    double base[5];
+    {
+        double u[10];
+#pragma unroll
+        for (int j = 0; j < 5; ++j)
+            u[j] = uc[j];
+        dev::con_to_prim(P, u);
+        first_order_residual<FLUX, MAXF>(m, P, U, c, u, base);
+    }
+    double A[25];
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+    {
+        double u[10];
+#pragma unroll
+        for (int j = 0; j < 5; ++j)
+            u[j] = uc[j];
+        const double h = 1e-7 * dev::rmax(fabs(uc[k]), 1.0);
+        u[k] = uc[k] + h;
+        const double hk = u[k] - uc[k];
+        dev::con_to_prim(P, u);
+        double r[5];
+        first_order_residual<FLUX, MAXF>(m, P, U, c, u, r);
+#pragma unroll
+        for (int j = 0; j < 5; ++j)
+            A[j * 5 + k] = (j == k ? idt : 0.0) - (r[j] - base[j]) / hk;
+    }
+    invert5(A, out);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_precond_apply(int N,
+                                                          const double *__restrict__ Minv,
+                                                          const T *__restrict__ v,
+                                                          T *__restrict__ z)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N)
+        return;
+    double x[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+        x[j] = (double)v[(size_t)c * 5 + j];
+    const double *Mi = Minv + (size_t)c * 25;
+#pragma unroll
+    for (int r = 0; r < 5; ++r)
+    {
+        double a = 0.0;
+#pragma unroll
+        for (int q = 0; q < 5; ++q)
+            a += Mi[r * 5 + q] * x[q];
+        z[(size_t)c * 5 + r] = (T)a;
+    }
+}
+
+__global__ void k_fd_eps(const double *__restrict__ zz, double num, double *__restrict__ eps)
+{
+    if (threadIdx.x == 0)
+    {
+        const double nz = sqrt(zz[0]);
+        eps[0] = num / (nz > 1e-300 ? nz : 1e-300);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_perturb(int n, const double *__restrict__ U,
+                                                    const double *__restrict__ eps,
+                                                    const T *__restrict__ z,
+                                                    double *__restrict__ Ut)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        Ut[i] = U[i] + eps[0] * (double)z[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_op_finish(int N, const T *__restrict__ z,
+                                                      const double *__restrict__ dtau,
+                                                      const double *__restrict__ Rt,
+                                                      const double *__restrict__ R0,
+                                                      const double *__restrict__ eps,
+                                                      T *__restrict__ w)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 5 * N)
+        return;
+    const int c = i / 5;
+    w[i] = (T)((double)z[i] / dtau[c] - (Rt[i] - R0[i]) / eps[0]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_rhs_minus(int n, const double *__restrict__ b,
+                                                      T *__restrict__ w)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        w[i] = (T)(b[i] - (double)w[i]);
+}
+
+// Deterministic multi-dot: a fixed grid of kDotBlocks blocks, grid-stride
+// loop, per-thread FP64 sums, then a per-block column reduction in thread
+// order; the fold adds the block rows in block order.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_multidot(int n, int k, const T *__restrict__ V,
+                                                     size_t ld, const T *__restrict__ w,
+                                                     double *__restrict__ partials)
+{
+    __shared__ double sh[kBlock * 17];
+    double acc[kMaxDots];
+#pragma unroll
+    for (int j = 0; j < kMaxDots; ++j)
+        acc[j] = 0.0;
+    const int stride = gridDim.x * blockDim.x;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    {
+        const double wi = (double)w[i];
+#pragma unroll
+        for (int j = 0; j < kMaxDots; ++j)
+            if (j < k)
+                acc[j] += (double)V[(size_t)j * ld + i] * wi;
+    }
+    // column sums over the block's threads, 16 columns at a time: 16 threads
+    // per column each add 16 rows in order, then one thread adds the 16
+    // sub-sums in order
+    __shared__ double sub[kBlock];
+#pragma unroll
+    for (int j0 = 0; j0 < kMaxDots; j0 += 16)
+    {
+        if (j0 >= k)
+            break;
+#pragma unroll
+        for (int q = 0; q < 16; ++q)
+            sh[threadIdx.x * 17 + q] = acc[j0 + q];
+        __syncthreads();
+        {
+            const int col = threadIdx.x & 15, part = threadIdx.x >> 4;
+            double s = 0.0;
+#pragma unroll
+            for (int t = 0; t < 16; ++t)
+                s += sh[(part * 16 + t) * 17 + col];
+            sub[part * 16 + col] = s;
+        }
+        __syncthreads();
+        if (threadIdx.x < 16 && j0 + (int)threadIdx.x < k)
+        {
+            double s = 0.0;
+#pragma unroll
+            for (int t = 0; t < 16; ++t)
+                s += sub[t * 16 + threadIdx.x];
+            partials[(size_t)blockIdx.x * kMaxDots + j0 + threadIdx.x] = s;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_fold_dots(int nblk, int k, const double *__restrict__ partials,
+                            double *__restrict__ dots)
+{
+    const int j = threadIdx.x;
+    if (j >= k)
+        return;
+    double s = 0.0;
+    for (int b = 0; b < nblk; ++b)
+        s += partials[(size_t)b * kMaxDots + j];
+    dots[j] = s;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_multiaxpy(int n, int k, const T *__restrict__ V,
+                                                      size_t ld, const double *__restrict__ h,
+                                                      T *__restrict__ w)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    double a = (double)w[i];
+    for (int j = 0; j < k; ++j)
+        a -= h[j] * (double)V[(size_t)j * ld + i];
+    w[i] = (T)a;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_scale(int n, const T *__restrict__ w, double sc,
+                                                  T *__restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        out[i] = (T)((double)w[i] * sc);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_combine(int n, int k, const T *__restrict__ V,
+                                                    size_t ld, const double *__restrict__ y,
+                                                    T *__restrict__ x)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    double a = 0.0;
+    for (int j = 0; j < k; ++j)
+        a += y[j] * (double)V[(size_t)j * ld + i];
+    x[i] = (T)a;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_accumulate(int n, const T *__restrict__ z,
+                                                       double *__restrict__ x)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        x[i] += (double)z[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_convert(int n, const double *__restrict__ src,
+                                                    T *__restrict__ dst)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        dst[i] = (T)src[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_residual_sums(int N, const double *__restrict__ R,
+                                                          const double *__restrict__ vol,
+                                                          double *__restrict__ partials)
+{
+    __shared__ double sh[kBlock];
+    double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    const int stride = gridDim.x * blockDim.x;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < N; c += stride)
+    {
+        const double v = vol[c];
+#pragma unroll
+        for (int j = 0; j < 5; ++j)
+            a[j] += dev::rabs(R[(size_t)c * 5 + j]) * v;
+    }
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+        dev::block_sum(sh, a[j]);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int j = 0; j < 5; ++j)
+            partials[(size_t)blockIdx.x * kMaxDots + j] = a[j];
+}
+
+__global__ __launch_bounds__(kBlock) void k_update_ratio(int N, const double *__restrict__ U,
+                                                         const double *__restrict__ dU,
+                                                         double *__restrict__ partials)
+{
+    __shared__ double sh[kBlock];
+    double a = 0.0;
+    const int stride = gridDim.x * blockDim.x;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < N; c += stride)
+    {
+        const double r0 = fabs(dU[(size_t)c * 5] / U[(size_t)c * 5]);
+        const double r4 = fabs(dU[(size_t)c * 5 + 4] / U[(size_t)c * 5 + 4]);
+        a = fmax(a, fmax(r0, r4));
+    }
+    // max through the min reduction of the negated value
+    double na = -a;
+    dev::block_min(sh, na);
+    if (threadIdx.x == 0)
+        partials[(size_t)blockIdx.x * kMaxDots] = -na;
+}
+
+__global__ void k_fold_max(int nblk, const double *__restrict__ partials,
+                           double *__restrict__ out)
+{
+    if (threadIdx.x != 0)
+        return;
+    double a = 0.0;
+    for (int b = 0; b < nblk; ++b)
+        a = fmax(a, partials[(size_t)b * kMaxDots]);
+    out[0] = a;
+}
+
+__global__ __launch_bounds__(kBlock) void k_update(int n, double *__restrict__ U,
+                                                   const double *__restrict__ dU,
+                                                   double alpha)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        U[i] += alpha * dU[i];
+}
+
+} // namespace
+
+void local_dt(hipStream_t s, const DevMesh &m, const PhysDev &P, const double *U,
+              double cfl, double *dtau)
+{
+    hipLaunchKernelGGL(k_local_dt, dim3(blocks_for(m.N)), dim3(kBlock), 0, s, m, P, U,
+                       cfl, dtau);
+}
+
+void precond_blocks(hipStream_t s, const DevMesh &m, const PhysDev &P, int flux,
+                    const double *U, const double *dtau, int blocks, double *Minv)
+{
+    if (m.maxf > 6)
+        throw std::runtime_error("cells with more than 6 faces are not supported");
+    const dim3 g(blocks_for(m.N)), b(kBlock);
+    if (flux == dev::FluxRusanov)
+    {
+        if (m.maxf <= 4)
+            hipLaunchKernelGGL((k_precond_blocks<dev::FluxRusanov, 4>), g, b, 0, s, m, P, U,
+                               dtau, blocks, Minv);
+        else
+            hipLaunchKernelGGL((k_precond_blocks<dev::FluxRusanov, 6>), g, b, 0, s, m, P, U,
+                               dtau, blocks, Minv);
+    }
+    else
+    {
+        if (m.maxf <= 4)
+            hipLaunchKernelGGL((k_precond_blocks<dev::FluxAUSM, 4>), g, b, 0, s, m, P, U,
+                               dtau, blocks, Minv);
+        else
+            hipLaunchKernelGGL((k_precond_blocks<dev::FluxAUSM, 6>), g, b, 0, s, m, P, U,
+                               dtau, blocks, Minv);
+    }
+}
+
+template <typename T>
+void precond_apply(hipStream_t s, int N, const double *Minv, const T *v, T *z)
+{
+    hipLaunchKernelGGL(k_precond_apply<T>, dim3(blocks_for(N)), dim3(kBlock), 0, s, N,
+                       Minv, v, z);
+}
+
+void fd_eps(hipStream_t s, const double *zz, double num, double *eps)
+{
+    hipLaunchKernelGGL(k_fd_eps, dim3(1), dim3(64), 0, s, zz, num, eps);
+}
+
+template <typename T>
+void perturb(hipStream_t s, int n, const double *U, const double *eps, const T *z,
+             double *Ut)
+{
+    hipLaunchKernelGGL(k_perturb<T>, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, U, eps,
+                       z, Ut);
+}
+
+template <typename T>
+void op_finish(hipStream_t s, int N, const T *z, const double *dtau, const double *Rt,
+               const double *R0, const double *eps, T *w)
+{
+    hipLaunchKernelGGL(k_op_finish<T>, dim3(blocks_for(5L * N)), dim3(kBlock), 0, s, N, z,
+                       dtau, Rt, R0, eps, w);
+}
+
+template <typename T>
+void rhs_minus(hipStream_t s, int n, const double *b, T *w)
+{
+    hipLaunchKernelGGL(k_rhs_minus<T>, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, b, w);
+}
+
+template <typename T>
+void multidot(hipStream_t s, int n, int k, const T *V, size_t ld, const T *w,
+              double *partials, double *dots)
+{
+    if (k < 1 || k > kMaxDots)
+        throw std::runtime_error("multidot: bad vector count");
+    hipLaunchKernelGGL(k_multidot<T>, dim3(kDotBlocks), dim3(kBlock), 0, s, n, k, V, ld, w,
+                       partials);
+    hipLaunchKernelGGL(k_fold_dots, dim3(1), dim3(kMaxDots), 0, s, kDotBlocks, k, partials,
+                       dots);
+}
+
+template <typename T>
+void multiaxpy(hipStream_t s, int n, int k, const T *V, size_t ld, const double *h, T *w)
+{
+    hipLaunchKernelGGL(k_multiaxpy<T>, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, k, V, ld,
+                       h, w);
+}
+
+template <typename T>
+void scale(hipStream_t s, int n, const T *w, double sc, T *out)
+{
+    hipLaunchKernelGGL(k_scale<T>, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, w, sc, out);
+}
+
+template <typename T>
+void combine(hipStream_t s, int n, int k, const T *V, size_t ld, const double *y, T *x)
+{
+    hipLaunchKernelGGL(k_combine<T>, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, k, V, ld,
+                       y, x);
+}
+
+template <typename T>
+void accumulate(hipStream_t s, int n, const T *z, double *x)
+{
+    hipLaunchKernelGGL(k_accumulate<T>, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, z, x);
+}
+
+template <typename T>
+void convert(hipStream_t s, int n, const double *src, T *dst)
+{
+    hipLaunchKernelGGL(k_convert<T>, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, src, dst);
+}
+
+void residual_sums(hipStream_t s, int N, const double *R, const double *vol,
+                   double *partials, double *sums)
+{
+    hipLaunchKernelGGL(k_residual_sums, dim3(kDotBlocks), dim3(kBlock), 0, s, N, R, vol,
+                       partials);
+    hipLaunchKernelGGL(k_fold_dots, dim3(1), dim3(kMaxDots), 0, s, kDotBlocks, 5, partials,
+                       sums);
+}
+
+void update_ratio(hipStream_t s, int N, const double *U, const double *dU, double *partials,
+                  double *out)
+{
+    hipLaunchKernelGGL(k_update_ratio, dim3(kDotBlocks), dim3(kBlock), 0, s, N, U, dU,
+                       partials);
+    hipLaunchKernelGGL(k_fold_max, dim3(1), dim3(64), 0, s, kDotBlocks, partials, out);
+}
+
+void update(hipStream_t s, int n, double *U, const double *dU, double alpha)
+{
+    hipLaunchKernelGGL(k_update, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, U, dU, alpha);
+}
+
+#define FVH_KRY_INST(T)                                                                    \
+    template void precond_apply<T>(hipStream_t, int, const double *, const T *, T *);      \
+    template void perturb<T>(hipStream_t, int, const double *, const double *, const T *,  \
+                             double *);                                                    \
+    template void op_finish<T>(hipStream_t, int, const T *, const double *,                \
+                               const double *, const double *, const double *, T *);       \
+    template void rhs_minus<T>(hipStream_t, int, const double *, T *);                     \
+    template void multidot<T>(hipStream_t, int, int, const T *, size_t, const T *,         \
+                              double *, double *);                                         \
+    template void multiaxpy<T>(hipStream_t, int, int, const T *, size_t, const double *,   \
+                               T *);                                                       \
+    template void scale<T>(hipStream_t, int, const T *, double, T *);                      \
+    template void combine<T>(hipStream_t, int, int, const T *, size_t, const double *,     \
+                             T *);                                                         \
+    template void accumulate<T>(hipStream_t, int, const T *, double *);                    \
+    template void convert<T>(hipStream_t, int, const double *, T *);
+
+FVH_KRY_INST(double)
+FVH_KRY_INST(float)
+
+} // namespace kry
+} // namespace fvh

@@ -1,0 +1,54 @@
+// Device runtime shared by the host translation units that drive kernels
+// (solver.cpp owns the state; the Newton-Krylov driver borrows it).
+// Internal; not part of the public C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+
+#include "fv_device.h"
+
+namespace fvh {
+
+#define FVH_HIP_CHECK(expr)                                                    \
+    do                                                                         \
+    {                                                                          \
+        hipError_t e_ = (expr);                                                \
+        if (e_ != hipSuccess)                                                  \
+            throw std::runtime_error(std::string("HIP error: ") +             \
+                                     hipGetErrorString(e_) + " at " +          \
+                                     __FILE__ + ":" + std::to_string(__LINE__)); \
+    } while (0)
+
+// The device state set up by init_finite_volume with the built-in hooks.
+struct RuntimeView
+{
+    hipStream_t stream;
+    dev::DevMesh m;
+    dev::PhysDev P;
+    int flux;      // dev::FluxKind
+    int linear;    // linear reconstruction
+    int viscous;   // mu != 0
+    double *U;     // current conservative state [N][5]
+    double vglob;  // total volume
+};
+
+// Throws unless init_finite_volume ran with the built-in hooks and the
+// reconstruction and limiter are set.
+RuntimeView runtime_view();
+
+// Enqueues R = dU/dt(U) (finite_volume_time_derivative semantics, the
+// reference's ghost rows recomputed from U) on the runtime stream.
+void eval_residual(const double *U, double *R);
+
+// The current state was changed outside the time loop: the cached residual
+// and host mirror are stale.
+void state_changed();
+
+// Releases device work buffers of the Newton-Krylov driver (called when the
+// device state is freed).
+void jfnk_release();
+
+} // namespace fvh

@@ -23,6 +23,9 @@
 #include "fv_device.h"
 #include "fvh_internal.h"
 #include "raw_container.h"
+#include "runtime.h"
+#include "implicit.h"
+#include "krylov.h"
 
 // ---------------------------------------------------------------------------
 // reference globals (solver_module.h:16-18, navier_stokes.c:16-55)
@@ -164,6 +167,11 @@ struct Device
     double *mirror_dt = nullptr;
     double vglob = 0.0;
     bool r_valid = false;
+    // reference-semantics implicit step (allocated on first use)
+    double *imp_old[2] = {nullptr, nullptr}; // BDF history phi_old[0..1]
+    double *imp_f = nullptr, *imp_dY = nullptr, *imp_jac = nullptr;
+    double *imp_Rp = nullptr, *imp_Rm = nullptr, *imp_Ut = nullptr;
+    double *imp_red = nullptr; // reduction partials [1024*64] + results
 };
 Device g_dev;
 
@@ -189,6 +197,7 @@ void free_device()
 {
     if (g_dev.stream)
         (void)hipStreamSynchronize(g_dev.stream);
+    jfnk_release();
     for (void *p : g_dev.allocs)
         (void)hipFree(p);
     if (g_dev.sc_host)
@@ -606,12 +615,16 @@ void print_header()
     std::printf("\n");
 }
 
-void print_row(int iter, const double *row, bool out)
+void print_row(int iter, const double *row, bool out, int inner = 0, int lsoe = 0)
 {
     if (!g_verbose)
         return;
-    std::printf("%09d %12.5e %12.5e %c %c:", iter, row[0], row[1],
-                row[2] != 0.0 ? 'T' : 'F', out ? '*' : ' ');
+    if (g_is_explicit)
+        std::printf("%09d %12.5e %12.5e %c %c:", iter, row[0], row[1],
+                    row[2] != 0.0 ? 'T' : 'F', out ? '*' : ' ');
+    else
+        std::printf("%09d %12.5e %12.5e %c %c %6d %6d:", iter, row[0], row[1],
+                    row[2] != 0.0 ? 'T' : 'F', out ? '*' : ' ', inner, lsoe);
     for (int j = 0; j < 5; ++j)
         std::printf(" %12.5e", row[3 + j]);
     std::printf("\n");
@@ -746,7 +759,193 @@ int run_explicit(int max_iter)
     return status;
 }
 
+// ---- reference-semantics implicit step (implicit.c:224-352) --------------
+void ensure_implicit_buffers()
+{
+    const size_t n = (size_t)5 * g_dev.m.N;
+    if (g_dev.imp_f)
+        return;
+    g_dev.imp_old[0] = dmalloc<double>(n);
+    g_dev.imp_old[1] = dmalloc<double>(n);
+    g_dev.imp_f = dmalloc<double>(n);
+    g_dev.imp_dY = dmalloc<double>(n);
+    g_dev.imp_jac = dmalloc<double>(5 * n);
+    g_dev.imp_Rp = dmalloc<double>(n);
+    g_dev.imp_Rm = dmalloc<double>(n);
+    g_dev.imp_Ut = dmalloc<double>(n);
+    g_dev.imp_red = dmalloc<double>((size_t)1024 * 64 + 64);
+}
+
+double device_norm2(const double *v, int n)
+{
+    // |v|_2 through the column sums of squares of v viewed as [n/5][5]
+    double sums[5];
+    double *out = g_dev.imp_red + (size_t)1024 * 64;
+    imp::column_sumsq(g_dev.stream, n / 5, v, g_dev.imp_red, out);
+    HIP_CHECK(hipMemcpyAsync(sums, out, sizeof(sums), hipMemcpyDeviceToHost,
+                             g_dev.stream));
+    HIP_CHECK(hipStreamSynchronize(g_dev.stream));
+    return std::sqrt((((sums[0] + sums[1]) + sums[2]) + sums[3]) + sums[4]);
+}
+
+// One implicit step from the current state U (R = g_dev.R holds R(U)).
+// Returns the inner Newton iterations.
+int implicit_step(int iter, double t, double dt, bool transient)
+{
+    ensure_implicit_buffers();
+    hipStream_t s = g_dev.stream;
+    const int N = g_dev.m.N, n = 5 * N;
+    const int n_bdf = g_bdf_scheme == BDF2 ? 2 : 1;
+    const int n_loc = iter < n_bdf ? 1 : n_bdf;
+    const bool euler = iter < n_bdf || g_bdf_scheme != BDF2;
+    const double a0 = euler ? 0.0 : -1. / 3., a1 = euler ? 0.0 : 1 / 3.;
+    const double b = euler ? 1.0 : 2. / 3.;
+    const double tpdt = t + dt;
+    (void)tpdt; // the built-in hooks are time independent
+    double *U = g_dev.U[g_dev.cur];
+    double **old = g_dev.imp_old;
+    for (int i = n_loc - 1; i > 0; --i)
+        HIP_CHECK(hipMemcpyAsync(old[i], old[i - 1], (size_t)n * 8,
+                                 hipMemcpyDeviceToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(old[0], U, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
+    imp::bdf_rhs(s, n, U, old[0], old[1], n_loc, a0, a1, b, dt, g_dev.R, g_dev.imp_f);
+    const double e0 = device_norm2(g_dev.imp_f, n);
+    double e_old = e0;
+    int inner;
+    for (inner = 1; inner <= g_max_iter_inner; ++inner)
+    {
+        // lumped numerical Jacobian (implicit.c:61-120)
+        double sq[5];
+        {
+            double *out = g_dev.imp_red + (size_t)1024 * 64;
+            imp::column_sumsq(s, N, U, g_dev.imp_red, out);
+            HIP_CHECK(hipMemcpyAsync(sq, out, sizeof(sq), hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipStreamSynchronize(s));
+        }
+        for (int k = 0; k < 5; ++k)
+        {
+            const double eps = std::sqrt(sq[k]) * 1e-4;
+            imp::shift_var(s, N, U, k, 0.5 * eps, g_dev.imp_Ut);
+            eval_residual(g_dev.imp_Ut, g_dev.imp_Rp);
+            imp::shift_var(s, N, U, k, -(0.5 * eps), g_dev.imp_Ut);
+            eval_residual(g_dev.imp_Ut, g_dev.imp_Rm);
+            imp::jac_column(s, N, g_dev.imp_Rp, g_dev.imp_Rm, k, b / (eps + 1e-16),
+                            1.0 / dt, g_dev.imp_jac);
+        }
+        if (e_old >= e0)
+        {
+            imp::block_solve(s, N, g_dev.imp_jac, g_dev.imp_f, g_dev.imp_dY);
+            kry::update(s, n, U, g_dev.imp_dY, 1.0);
+            eval_residual(U, g_dev.R);
+            imp::bdf_rhs(s, n, U, old[0], old[1], n_loc, a0, a1, b, dt, g_dev.R,
+                         g_dev.imp_f);
+            e_old = device_norm2(g_dev.imp_f, n);
+        }
+        else
+        {
+            // the reference evaluates R at the state the Jacobian loop left
+            // in phi_total (variable 4 shifted by -eps/2)
+            eval_residual(g_dev.imp_Ut, g_dev.R);
+        }
+        if (e_old < e0)
+            break;
+        if (!transient)
+            break;
+    }
+    if (inner >= g_max_iter_inner)
+        throw std::runtime_error("implicit step: inner Newton iterations exhausted "
+                                 "(max_iter_inner)");
+    return inner;
+}
+
+// timedisc.c:134-222 with timestep_newton; host-driven (one synchronisation
+// per step), bookkeeping on the device scalars like the explicit loop.
+int run_implicit(int max_iter)
+{
+    require_ready();
+    int iter0 = g_use_restart ? g_iter_restart : 0;
+    double t0 = g_use_restart ? g_t_restart : g_t_start;
+    print_header();
+    if (iter0 == 0)
+    {
+        eval_residual(g_dev.U[g_dev.cur], g_dev.R);
+        g_dev.r_valid = true;
+    }
+    if (g_do_output && !g_use_restart)
+        write_output(iter0, t0);
+    if (g_dev.log_cap < max_iter)
+    {
+        g_dev.log_cap = max_iter;
+        g_dev.log = dmalloc<double>((size_t)g_dev.log_cap * 8);
+    }
+    g_log.assign((size_t)max_iter * 8, 0.0);
+    init_scalars(iter0, t0, max_iter, g_transient, g_t_end);
+    read_scalars();
+    int status = 0, printed = 0;
+    while (true)
+    {
+        const dev::DevScalars h0 = *g_dev.sc_host;
+        const int inner = implicit_step(h0.iter, h0.t, h0.dt, g_transient);
+        g_dev.r_valid = true;
+        imp::step_partials(g_dev.stream, g_dev.m, phys(), g_dev.U[g_dev.cur], g_dev.R,
+                           g_dev.partials);
+        dev::launch_finalize(g_dev.stream, g_dev.sc, g_dev.partials,
+                             dev::cell_blocks(g_dev.m.N), g_dev.fold, g_dev.fold_cap,
+                             phys(), 1, g_dev.log, g_dev.log_cap);
+        read_scalars();
+        const dev::DevScalars &h = *g_dev.sc_host;
+        if (h.done < 0)
+        {
+            status = FVH_ERR_NAN;
+            break;
+        }
+        HIP_CHECK(hipMemcpy(&g_log[(size_t)printed * 8], &g_dev.log[(size_t)printed * 8],
+                            8 * sizeof(double), hipMemcpyDeviceToHost));
+        const int it = iter0 + printed + 1;
+        const bool last = h.done > 0;
+        const bool out = last || (g_i_output_data > 0 && it % g_i_output_data == 0);
+        print_row(it, &g_log[(size_t)printed * 8], out, inner, 1);
+        for (int j = 0; j < 5; ++j)
+            g_residual[j] = g_log[(size_t)printed * 8 + 3 + j];
+        g_viscous_dt = g_log[(size_t)printed * 8 + 2] != 0.0;
+        ++printed;
+        if (g_do_output && out)
+            write_output(it, h.t);
+        if (last)
+            break;
+    }
+    g_last_iterations = printed;
+    g_last_status = status;
+    if (g_verbose)
+        std::fflush(stdout);
+    return status;
+}
+
 } // namespace
+
+RuntimeView runtime_view()
+{
+    require_ready();
+    ensure_gradient_buffers();
+    RuntimeView v{};
+    v.stream = g_dev.stream;
+    v.m = g_dev.m;
+    v.P = phys();
+    v.flux = g_flux;
+    v.linear = is_linear();
+    v.viscous = is_viscous();
+    v.U = g_dev.U[g_dev.cur];
+    v.vglob = g_dev.vglob;
+    return v;
+}
+
+void eval_residual(const double *U, double *R)
+{
+    enqueue_residual(U, nullptr, R, 0, false, false, nullptr);
+}
+
+void state_changed() { g_dev.r_valid = false; }
+
 } // namespace fvh
 
 using namespace fvh;
@@ -1186,10 +1385,7 @@ void timedisc(void)
     int rc;
     try
     {
-        if (!g_is_explicit)
-            fatal("timedisc: the implicit scheme is driven by the fvh_jfnk_* "
-                  "extension API");
-        rc = run_explicit(g_max_iter);
+        rc = g_is_explicit ? run_explicit(g_max_iter) : run_implicit(g_max_iter);
     }
     catch (const std::exception &e)
     {
@@ -1674,7 +1870,7 @@ int fvh_last_status(int *iterations)
 
 int fvh_timedisc(void)
 {
-    FVH_TRY({ return run_explicit(g_max_iter); });
+    FVH_TRY({ return g_is_explicit ? run_explicit(g_max_iter) : run_implicit(g_max_iter); });
 }
 
 int fvh_bench_explicit(int n_steps, int n_warmup, double *seconds,

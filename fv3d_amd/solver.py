@@ -11,7 +11,7 @@ implicit -> restart) against libfv3d_hip.so.
 import ctypes
 import numpy as np
 
-from ._lib import lib, P_DOUBLE, P_INT
+from ._lib import lib, P_DOUBLE, P_INT, JfnkOptions, JfnkStats
 
 c_int, c_double = ctypes.c_int, ctypes.c_double
 PD, PI = ctypes.POINTER(c_double), ctypes.POINTER(c_int)
@@ -230,7 +230,18 @@ class Case:
                         numerics.get("max_iter", 100), 1 if numerics.get("transient", True) else 0,
                         numerics.get("abort_residual", 1e-10), numerics.get("t_start", 0.0),
                         numerics.get("t_end", 1e300))
-        L.init_explicit(SCHEMES[numerics.get("scheme", "RK3-3")])
+        if numerics.get("timestep", "Explicit") == "Explicit":
+            L.init_explicit(SCHEMES[numerics.get("scheme", "RK3-3")])
+        else:
+            # TimeDisc/Implicit keys (src/FV3D/src/timedisc/implicit.c:36-56): scheme,
+            # solver, max_iter_inner, tolerance_lsoe, max_iter_lsoe, krylov dims/restarts
+            L.init_implicit({"Euler": EULER_IMPLICIT, "BDF2": BDF2}[numerics.get("scheme", "Euler")],
+                            {"BiCGStab": 0, "GMRes": 1}[numerics.get("solver", "GMRes")],
+                            numerics.get("max_iter_inner", 100),
+                            numerics.get("tolerance_lsoe", 1e-12),
+                            numerics.get("max_iter_lsoe", 100),
+                            numerics.get("max_krylov_dims", 15),
+                            numerics.get("max_krylov_restarts", 2))
         L.init_restart(0, b"")
         self.n_cells = m.cells.contents.n_domain_cells
         self.n_faces = m.faces.contents.n_faces
@@ -279,6 +290,42 @@ class Case:
         kus = (ctypes.c_double * 3)()
         check(self.L.fvh_bench_explicit(steps, warmup, ctypes.byref(sec), kus))
         return sec.value, list(kus)
+
+    def jfnk(self, **options):
+        """Steady Newton-Krylov solve of the current state (fvh_jfnk_solve).
+        options override fvh_jfnk_default_options fields; precision and
+        precond also take the names "fp64"/"fp32" and
+        "scalar"/"block"/"learned".  Returns (stats dict, history rows)."""
+        o = JfnkOptions()
+        check(self.L.fvh_jfnk_default_options(ctypes.byref(o)))
+        names = {"precision": {"fp64": 0, "fp32": 1},
+                 "precond": {"scalar": 0, "block": 1, "learned": 2}}
+        for k, v in options.items():
+            if not hasattr(o, k):
+                raise KeyError("unknown JFNK option %r" % k)
+            if k in names and isinstance(v, str):
+                v = names[k][v]
+            setattr(o, k, v)
+        st = JfnkStats()
+        check(self.L.fvh_jfnk_solve(ctypes.byref(o), ctypes.byref(st)))
+        n = self.L.fvh_jfnk_history(None, 0)
+        rows = np.zeros((n, 6))
+        if n:
+            self.L.fvh_jfnk_history(rows.ctypes.data_as(P_DOUBLE), n)
+        stats = {f: getattr(st, f) for f, _ in JfnkStats._fields_}
+        return stats, rows
+
+    def jfnk_blocks(self, cfl, precond, dtau, minv):
+        """Pseudo time steps and inverse preconditioner blocks of one Newton
+        step at the current state (fvh_jfnk_blocks) into dtau [N] and
+        minv [N,5,5] (float64, C-contiguous)."""
+        kinds = {"scalar": 0, "block": 1}
+        assert dtau.dtype == np.float64 and dtau.flags.c_contiguous
+        assert minv.dtype == np.float64 and minv.flags.c_contiguous
+        assert dtau.size == self.n_cells and minv.size == 25 * self.n_cells
+        check(self.L.fvh_jfnk_blocks(cfl, kinds.get(precond, precond),
+                                     dtau.ctypes.data_as(P_DOUBLE),
+                                     minv.ctypes.data_as(P_DOUBLE)))
 
     def host_phi_total(self):
         check(self.L.fvh_sync_host())

@@ -106,6 +106,67 @@ int fvh_bench_explicit(int n_steps, int n_warmup, double *seconds,
 int fvh_device_info(char *name, int name_len, int *cu_count,
                     double *hbm_gbps_peak);
 
+/* ---- steady Newton-Krylov (JFNK) solve --------------------------------- */
+/* Drives the current device state to the steady state R(U) = 0 of the
+ * configured residual (finite_volume_time_derivative semantics) with
+ * pseudo-transient Newton steps (I/dtau - dR/dU) dU = R(U), dtau the local
+ * convective step at the pseudo CFL, each solved by right-preconditioned
+ * GMRES(krylov_dims) with a matrix-free finite-difference operator
+ * J v = [R(U + eps z) - R(U)] / eps, eps = fd_eps (1 + |U|_2) / |z|_2.
+ * No counterpart in the reference (its implicit mode is the lumped-Jacobian
+ * block solve of init_implicit / timedisc). */
+#define FVH_JFNK_FP64 0      /* Krylov vectors and J v in FP64 */
+#define FVH_JFNK_FP32 1      /* Krylov vectors and J v results stored in FP32;
+                                residuals, dots, update in FP64 */
+#define FVH_PRECOND_SCALAR 0 /* M = I / dtau */
+#define FVH_PRECOND_BLOCK 1  /* block Jacobi: exact 5x5 blocks of the
+                                first-order inviscid residual */
+#define FVH_PRECOND_LEARNED 2 /* 5x5 blocks predicted by the trained cell
+                                 network; block Jacobi where rejected */
+
+typedef struct
+{
+    int max_newton;      /* Newton steps (default 300) */
+    double tol;          /* stop when max_j (1/V) sum_i |R_ij| V_i <= tol
+                            (default 1e-8) */
+    double cfl0;         /* initial pseudo CFL (default 10) */
+    double cfl_growth;   /* CFL factor after a step that did not raise the
+                            norm above 3x the previous one (default 1.5) */
+    double cfl_cut;      /* CFL factor otherwise (default 0.3) */
+    double cfl_max;      /* (default 1e6) */
+    int krylov_dims;     /* GMRES restart length m (default 30, <= 60) */
+    int max_restarts;    /* GMRES cycles per Newton step minus one (default 3) */
+    double rtol;         /* GMRES relative tolerance (default 1e-2) */
+    double fd_eps;       /* finite-difference scale (default 1e-7) */
+    double max_change;   /* damping: largest relative change of rho and
+                            rho*E per step (default 0.2) */
+    int precision;       /* FVH_JFNK_FP64 / FVH_JFNK_FP32 */
+    int precond;         /* FVH_PRECOND_* (default FVH_PRECOND_BLOCK) */
+    int verbose;         /* print one line per Newton step */
+} fvh_jfnk_options_t;
+
+typedef struct
+{
+    int status;           /* 0 converged, 1 Newton step limit reached */
+    int newton_steps;
+    int gmres_iterations; /* total Arnoldi steps */
+    int residual_evals;   /* full residual evaluations */
+    double seconds;       /* wall time of the Newton loop */
+    double norm0, norm;   /* stop norm at the start and at the end */
+    double l2_drop;       /* |R|_2 end / start */
+} fvh_jfnk_stats_t;
+
+int fvh_jfnk_default_options(fvh_jfnk_options_t *opt);
+int fvh_jfnk_solve(const fvh_jfnk_options_t *opt, fvh_jfnk_stats_t *stats);
+/* Per Newton step rows {norm, cfl, gmres iterations, gmres relative
+ * residual, damping factor, elapsed seconds}; returns the row count. */
+int fvh_jfnk_history(double *rows, int max_rows);
+/* The preconditioner of one Newton step at the current state: the local
+ * pseudo time steps dtau[N] at the given CFL and the inverse blocks
+ * minv[N][5][5] (row-major) of M_c = I/dtau_c - dR1_c/dU_c (precond
+ * FVH_PRECOND_BLOCK) or of I/dtau_c (FVH_PRECOND_SCALAR). */
+int fvh_jfnk_blocks(double cfl, int precond, double *dtau, double *minv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1040,50 +1040,58 @@ static void solve_block5(const double *jac, const double *rhs, double *x)
     }
 }
 
-/* implicit.c:61-120 (calc_jacobian_numerical) + implicit.c:224-352
- * (timestep_newton); BDF tables implicit.c:34-38 */
-int fvref_timestep_newton(fvref_t *s, int scheme, int iter, double t,
-                          double dt, int transient, int max_iter_inner,
-                          double *jac_out)
+/* f = -(Y - old_0)/dt + b R - sum_s a_s/dt old_s  (implicit.c:254-270) */
+static void bdf_rhs(int nv, const double *Y, double *const *old, int n_loc,
+                    const double *a, double b, double dt, const double *R,
+                    double *f)
+{
+    for (int idx = 0; idx < nv; ++idx)
+        f[idx] = -(Y[idx] - old[0][idx]) / dt + b * R[idx];
+    for (int st = 0; st < n_loc; ++st)
+        for (int idx = 0; idx < nv; ++idx)
+            f[idx] -= a[st] / dt * old[st][idx];
+}
+
+/* One implicit step, implicit.c:224-352 (timestep_newton) with
+ * calc_jacobian_numerical (implicit.c:61-120).  old[0..n_bdf-1] is the BDF
+ * history kept by the caller across steps (phi_old); on entry phi_dt holds
+ * R of the current state.  Returns the inner iterations, or -2 when the
+ * inner loop ran to max_iter_inner (the reference aborts there). */
+static int timestep_newton(fvref_t *s, int scheme, int iter, double t,
+                           double dt, int transient, int max_iter_inner,
+                           double **old, double *jac_out)
 {
     static const double bdf_a_euler[1] = {0.0};
     static const double bdf_a_bdf2[2] = {-1. / 3., 1 / 3.};
-    int n = s->n_cells;
-    int n_bdf = scheme == 1 ? 2 : 1;
-    int n_loc = (iter < n_bdf) ? 1 : n_bdf;
-    const double *bdf_a = (iter < n_bdf) ? bdf_a_euler
-                                         : (scheme == 1 ? bdf_a_bdf2
-                                                        : bdf_a_euler);
-    double bdf_b = (iter < n_bdf) ? 1.0 : (scheme == 1 ? 2. / 3. : 1.0);
-    double tpdt = t + dt;
-    size_t nv = (size_t)n * NSOL;
+    const int n = s->n_cells;
+    const int n_bdf = scheme == 1 ? 2 : 1;
+    const int n_loc = (iter < n_bdf) ? 1 : n_bdf;
+    const double *a = (iter < n_bdf) ? bdf_a_euler
+                                     : (scheme == 1 ? bdf_a_bdf2 : bdf_a_euler);
+    const double b = (iter < n_bdf) ? 1.0 : (scheme == 1 ? 2. / 3. : 1.0);
+    const double tpdt = t + dt;
+    const int nv = n * NSOL;
 
-    /* only one old level is kept here: the caller drives single steps */
-    double *phi_old = xcalloc(nv, sizeof(double));
-    double *Y = xcalloc(nv, sizeof(double));
-    double *f = xcalloc(nv, sizeof(double));
-    double *dY = xcalloc(nv, sizeof(double));
-    double *dYdt = xcalloc(nv, sizeof(double));
-    double *jac = xcalloc(nv * NSOL, sizeof(double));
-    (void)n_loc;
+    double *Y = xcalloc((size_t)nv, sizeof(double));
+    double *f = xcalloc((size_t)nv, sizeof(double));
+    double *dY = xcalloc((size_t)nv, sizeof(double));
+    double *dYdt = xcalloc((size_t)nv, sizeof(double));
+    double *jac = xcalloc((size_t)nv * NSOL, sizeof(double));
 
+    for (int i = n_loc - 1; i > 0; --i)
+        copy_n(old[i - 1], nv, old[i]);
     for (int i = 0; i < n; ++i)
         for (int j = 0; j < NSOL; ++j)
-            phi_old[i * NSOL + j] = s->phi_total[i * NTOT + j];
-    copy_n(phi_old, (int)nv, Y);
-    copy_n(s->phi_dt, (int)nv, dYdt);
+            old[0][i * NSOL + j] = s->phi_total[i * NTOT + j];
+    copy_n(old[0], nv, Y);
+    copy_n(s->phi_dt, nv, dYdt);
+    bdf_rhs(nv, Y, old, n_loc, a, b, dt, dYdt, f);
 
-    for (size_t idx = 0; idx < nv; ++idx)
-        f[idx] = -(Y[idx] - phi_old[idx]) / dt + bdf_b * dYdt[idx];
-    for (size_t idx = 0; idx < nv; ++idx)
-        f[idx] -= bdf_a[0] / dt * phi_old[idx];
-
-    double e0 = sqrt(dot_n(f, f, (int)nv));
+    const double e0 = sqrt(dot_n(f, f, nv));
     double e_old = e0;
     int inner;
     for (inner = 1; inner <= max_iter_inner; ++inner)
     {
-        /* lumped numerical Jacobian, implicit.c:61-120 */
         for (int iv = 0; iv < NSOL; ++iv)
         {
             double eps = 0.0;
@@ -1115,13 +1123,13 @@ int fvref_timestep_newton(fvref_t *s, int scheme, int iter, double t,
                 for (int j = 0; j < NSOL; ++j)
                 {
                     jac[idx + j] += s->phi_dt[i * NSOL + j];
-                    jac[idx + j] *= bdf_b / (eps + SMALL);
+                    jac[idx + j] *= b / (eps + SMALL);
                 }
                 jac[idx + iv] += 1.0 / dt;
             }
         }
         if (jac_out)
-            copy_n(jac, (int)(nv * NSOL), jac_out);
+            copy_n(jac, nv * NSOL, jac_out);
 
         if (e_old >= e0)
         {
@@ -1134,12 +1142,9 @@ int fvref_timestep_newton(fvref_t *s, int scheme, int iter, double t,
                     s->phi_total[i * NTOT + j] = Y[i * NSOL + j];
                 }
             time_derivative(s, tpdt);
-            copy_n(s->phi_dt, (int)nv, dYdt);
-            for (size_t idx = 0; idx < nv; ++idx)
-                f[idx] = -(Y[idx] - phi_old[idx]) / dt + bdf_b * dYdt[idx];
-            for (size_t idx = 0; idx < nv; ++idx)
-                f[idx] -= bdf_a[0] / dt * phi_old[idx];
-            e_old = sqrt(dot_n(f, f, (int)nv));
+            copy_n(s->phi_dt, nv, dYdt);
+            bdf_rhs(nv, Y, old, n_loc, a, b, dt, dYdt, f);
+            e_old = sqrt(dot_n(f, f, nv));
         }
         else
         {
@@ -1150,13 +1155,92 @@ int fvref_timestep_newton(fvref_t *s, int scheme, int iter, double t,
         if (!transient)
             break;
     }
-    free(phi_old);
     free(Y);
     free(f);
     free(dY);
     free(dYdt);
     free(jac);
+    if (inner >= max_iter_inner)
+        return -2;
     return inner;
+}
+
+int fvref_timestep_newton(fvref_t *s, int scheme, int iter, double t,
+                          double dt, int transient, int max_iter_inner,
+                          double *old_levels, double *jac_out)
+{
+    const int nv = s->n_cells * NSOL;
+    double *old[2] = {old_levels, old_levels + nv};
+    return timestep_newton(s, scheme, iter, t, dt, transient, max_iter_inner,
+                           old, jac_out);
+}
+
+/* timedisc.c:134-222 with the implicit step: the same loop as the explicit
+ * driver (dt from calc_ns_timestep on the refreshed state, t_end clamp,
+ * residual, NaN stop, steady and iteration stops); log rows as there. */
+int fvref_timedisc_implicit(fvref_t *s, int scheme, int max_iter,
+                            int transient, double abort_residual,
+                            double t_start, double t_end, int max_iter_inner,
+                            double *log)
+{
+    const int nv = s->n_cells * NSOL;
+    double *levels = xcalloc((size_t)nv * 2, sizeof(double));
+    double *old[2] = {levels, levels + nv};
+    double t_end_loc = transient ? t_end : DBL_MAX;
+    int iter = 0;
+    double t = t_start;
+    double res[NSOL];
+    int finalize = 0;
+
+    time_derivative(s, t);
+
+    while (1)
+    {
+        double dt = calc_ns_timestep(s);
+        if (t + dt > t_end_loc)
+        {
+            dt = t_end_loc - t;
+            finalize = 1;
+        }
+        if (timestep_newton(s, scheme, iter, t, dt, transient, max_iter_inner,
+                            old, NULL) < 0)
+        {
+            free(levels);
+            return -2;
+        }
+        fvref_global_residual(s, dt, res);
+        for (int j = 0; j < NSOL; ++j)
+            if (isnan(res[j]) || isinf(res[j]))
+            {
+                free(levels);
+                return -1;
+            }
+        t = t + dt;
+        iter = iter + 1;
+        if (log)
+        {
+            double *row = &log[(iter - 1) * 8];
+            row[0] = t;
+            row[1] = dt;
+            row[2] = s->is_viscous_dt;
+            for (int j = 0; j < NSOL; ++j)
+                row[3 + j] = res[j];
+        }
+        if (!transient)
+        {
+            double mx = res[0];
+            for (int j = 1; j < NSOL; ++j)
+                mx = REF_MAX(mx, res[j]);
+            if (mx < abort_residual)
+                finalize = 1;
+        }
+        if (iter >= max_iter)
+            finalize = 1;
+        if (finalize)
+            break;
+    }
+    free(levels);
+    return iter;
 }
 
 /* ------------------------------------------------------------------------ */

@@ -94,10 +94,21 @@ double fvref_lserk_amplification(int scheme, double z);
 /* Reference-semantics implicit step (timestep_newton): BDF1/BDF2 with the
  * lumped finite-difference block Jacobian; the block-diagonal linear system
  * is solved exactly per cell (the reference hands it to an external Krylov
- * solver, see fvref.c).  Returns Newton iterations. */
+ * solver, see fvref.c).  old_levels [2*N*5] is the BDF history (phi_old[0],
+ * phi_old[1]) the caller keeps across steps; phi_dt must hold R of the
+ * current state.  Returns the inner iterations (-2: max_iter_inner
+ * reached, where the reference aborts). */
 int fvref_timestep_newton(fvref_t *s, int scheme, int iter, double t,
                           double dt, int transient, int max_iter_inner,
-                          double *jac_out);
+                          double *old_levels, double *jac_out);
+
+/* timedisc() with the implicit step (scheme 0 Euler, 1 BDF2); log rows as
+ * fvref_timedisc_explicit.  Returns the steps done, -1 on NaN/Inf, -2 when
+ * the inner Newton loop ran out. */
+int fvref_timedisc_implicit(fvref_t *s, int scheme, int max_iter,
+                            int transient, double abort_residual,
+                            double t_start, double t_end, int max_iter_inner,
+                            double *log);
 
 /* Field access.  what: 0 phi_total[(N+B)*10], 1 phi_dt[(N+B)*5],
  * 2 grad_x, 3 grad_y, 4 grad_z [(N+B)*10], 5 flux[F*5], 6 left[F*10],

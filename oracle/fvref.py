@@ -51,7 +51,10 @@ def lib():
                                           c_double, c_double, PD]
     L.fvref_timestep_newton.restype = c_int
     L.fvref_timestep_newton.argtypes = [c_void_p, c_int, c_int, c_double, c_double,
-                                        c_int, c_int, PD]
+                                        c_int, c_int, PD, PD]
+    L.fvref_timedisc_implicit.restype = c_int
+    L.fvref_timedisc_implicit.argtypes = [c_void_p, c_int, c_int, c_int, c_double,
+                                          c_double, c_double, c_int, PD]
     L.fvref_get.argtypes = [c_void_p, c_int, PD]
     L.fvref_n_elements.restype = c_int
     L.fvref_n_elements.argtypes = [c_void_p]
@@ -175,6 +178,28 @@ class Oracle:
                                            abort_residual, t_start, t_end,
                                            log.ctypes.data_as(PD))
         return n, log[: max(n, 0)]
+
+    def timedisc_implicit(self, scheme="Euler", max_iter=10, transient=True,
+                          abort_residual=1e-10, t_start=0.0, t_end=1e300, max_iter_inner=100):
+        """timedisc() with the implicit step (scheme "Euler" or "BDF2")."""
+        sch = {"Euler": 0, "BDF2": 1}[scheme]
+        log = np.zeros((max_iter, 8))
+        n = self.L.fvref_timedisc_implicit(self.h, sch, max_iter, 1 if transient else 0,
+                                           abort_residual, t_start, t_end, max_iter_inner,
+                                           log.ctypes.data_as(PD))
+        return n, log[: max(n, 0)]
+
+    def timestep_newton(self, scheme, it, t, dt, old_levels, transient=True,
+                        max_iter_inner=100, want_jac=False):
+        """One implicit step on the current state; old_levels [2, N*5] is the
+        BDF history (updated in place).  phi_dt must hold R of the state."""
+        assert old_levels.dtype == np.float64 and old_levels.flags.c_contiguous
+        jac = np.zeros(self.N * 25) if want_jac else None
+        k = self.L.fvref_timestep_newton(self.h, {"Euler": 0, "BDF2": 1}[scheme], it, t, dt,
+                                         1 if transient else 0, max_iter_inner,
+                                         old_levels.ctypes.data_as(PD),
+                                         jac.ctypes.data_as(PD) if want_jac else None)
+        return k, jac
 
     # physics helpers
     def con_to_prim(self, phi10):

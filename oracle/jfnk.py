@@ -1,0 +1,218 @@
+"""CPU restatement of the steady Newton-Krylov (JFNK) solve on the oracle's
+residual.
+
+TEST INFRASTRUCTURE ONLY: used by tests/ and the CPU-baseline leg of
+bench.py as the checker, never by the product.
+
+The reference has no JFNK; its residual is `finite_volume_time_derivative`
+(src/Solver/src/finite_volume/finite_volume.c:81-124), which the oracle
+restates.  The algorithm here is the one the product's fvh_jfnk_solve runs
+(same options, same pseudo-time step, same operator and the same GMRES),
+written plainly in FP64 numpy:
+  - dtau_c = cfl * V_c / sum_d (|u_d| + c) dx_d  (the convective limit of
+    src/FV3D/src/navier_stokes/navier_stokes.c:171-180, per cell);
+  - block-Jacobi blocks M_c = I/dtau_c - dR1_c/dU_c of the first-order
+    inviscid residual R1, by one-sided differences with h = 1e-7 max(|U|, 1)
+    over a distance-1 colouring of the cells (all cells of one colour
+    perturbed at once: a first-order residual sees no two of them), a
+    different route to the same derivatives as the device's per-cell
+    differences;
+  - right-preconditioned GMRES(m) with restarts on
+    A z = z/dtau - [R(U + eps z) - R(U)]/eps, eps = fd_eps (1+|U|)/|z|,
+    classical Gram-Schmidt twice, Givens rotations, relative tolerance rtol;
+  - update U += alpha M^-1 x with alpha = min(1, max_change / max relative
+    change of rho and rho*E);
+  - CFL times cfl_growth when the new stop norm is below 3x the old one,
+    else times cfl_cut (not below 1), capped at cfl_max.
+Stop norm: max_j (1/V_glob) sum_i |R_ij| V_i (src/Solver/src/analyze/
+analyze.c:37-61 without the dt factor).
+"""
+import time
+
+import numpy as np
+
+from . import fvref
+
+DEFAULTS = dict(max_newton=300, tol=1e-8, cfl0=10.0, cfl_growth=1.5, cfl_cut=0.3,
+                cfl_max=1e6, krylov_dims=30, max_restarts=3, rtol=1e-2, fd_eps=1e-7,
+                max_change=0.2, precond="block")
+
+
+def colouring(mesh):
+    """Greedy distance-1 colouring of the cell graph (cells sharing a face)."""
+    N, F = mesh["n_cells"], mesh["n_faces"]
+    fc = np.asarray(mesh["face_cells_file"]).reshape(F, 2)
+    adj = [[] for _ in range(N)]
+    for a, b in fc:
+        if 0 <= b < N:
+            adj[a].append(b)
+            adj[b].append(a)
+    colour = -np.ones(N, dtype=np.int64)
+    for c in range(N):
+        used = {colour[x] for x in adj[c]}
+        k = 0
+        while k in used:
+            k += 1
+        colour[c] = k
+    return colour
+
+
+def gmres(op, b, m, max_restarts, rtol):
+    """Restarted GMRES(m) for op(x) = b from x = 0: classical Gram-Schmidt
+    twice, Givens rotations; stops when the estimated residual is at most
+    rtol |b|.  Returns (x, Arnoldi steps, estimated relative residual)."""
+    x = np.zeros(b.size)
+    beta0 = None
+    its = 0
+    rel = 1.0
+    for cycle in range(max_restarts + 1):
+        r = b.copy() if cycle == 0 else b - op(x)
+        beta = np.linalg.norm(r)
+        if beta0 is None:
+            beta0 = beta
+        if not beta > 0:
+            rel = beta / beta0 if beta0 > 0 else 0.0
+            break
+        Vk = np.zeros((m + 1, b.size))
+        H = np.zeros((m + 1, m))
+        cs, sn = np.zeros(m), np.zeros(m)
+        g = np.zeros(m + 1)
+        g[0] = beta
+        Vk[0] = r / beta
+        k = 0
+        done = False
+        for j in range(m):
+            w = op(Vk[j])
+            h1 = Vk[: j + 1] @ w
+            w = w - h1 @ Vk[: j + 1]
+            h2 = Vk[: j + 1] @ w
+            w = w - h2 @ Vk[: j + 1]
+            its += 1
+            H[: j + 1, j] = h1 + h2
+            hn = np.linalg.norm(w)
+            H[j + 1, j] = hn
+            for i in range(j):
+                a_, b_ = H[i, j], H[i + 1, j]
+                H[i, j] = cs[i] * a_ + sn[i] * b_
+                H[i + 1, j] = -sn[i] * a_ + cs[i] * b_
+            rr = np.hypot(H[j, j], H[j + 1, j])
+            cs[j] = H[j, j] / rr if rr > 0 else 1.0
+            sn[j] = H[j + 1, j] / rr if rr > 0 else 0.0
+            H[j, j] = rr
+            H[j + 1, j] = 0.0
+            g[j + 1] = -sn[j] * g[j]
+            g[j] = cs[j] * g[j]
+            k = j + 1
+            rel = abs(g[j + 1]) / beta0
+            if rel <= rtol or not hn > 1e-300 * beta0:
+                done = True
+                break
+            Vk[j + 1] = w / hn
+        y = np.zeros(k)
+        for i in range(k - 1, -1, -1):
+            y[i] = (g[i] - H[i, i + 1:k] @ y[i + 1:k]) / H[i, i]
+        x = x + y @ Vk[:k]
+        if done:
+            break
+    return x, its, rel
+
+
+class OracleJfnk:
+    def __init__(self, mesh, physics, regions, numerics):
+        self.mesh = mesh
+        self.N = mesh["n_cells"]
+        self.V = np.asarray(mesh["cell_volume"], dtype=np.float64)
+        self.Vg = self.V.sum()
+        self.dx = np.asarray(mesh["cell_dx"], dtype=np.float64).reshape(self.N, 3)
+        self.kappa = physics.get("kappa", 1.4)
+        self.full = fvref.from_case(mesh, physics, regions, numerics)
+        self.first = fvref.from_case(mesh, physics, regions,
+                                     dict(numerics, reconstruction="First-Order",
+                                          limiter="None"))
+        self.colour = colouring(mesh)
+        self.evals = 0
+
+    def residual(self, U, first=False):
+        o = self.first if first else self.full
+        o.set_state(U.reshape(-1))
+        if not first:
+            self.evals += 1
+        return o.time_derivative()
+
+    def stop_norm(self, R):
+        return (np.abs(R) * self.V[:, None]).sum(0).max() / self.Vg
+
+    def local_dt(self, U, cfl):
+        rho = U[:, 0]
+        u = U[:, 1:4] / rho[:, None]
+        p = (self.kappa - 1.0) * (U[:, 4] - 0.5 * (U[:, 1:4] * u).sum(1))
+        p = np.maximum(p, 1e-10)
+        c = np.sqrt(self.kappa * p / rho)
+        return cfl * (self.V / ((np.abs(u) + c[:, None]) * self.dx).sum(1))
+
+    def blocks(self, U, dtau):
+        """M_c = I/dtau_c - dR1_c/dU_c, [N,5,5] (row j, column k)."""
+        base = self.residual(U, first=True)
+        J = np.zeros((self.N, 5, 5))
+        for col in range(self.colour.max() + 1):
+            mask = self.colour == col
+            for k in range(5):
+                h = 1e-7 * np.maximum(np.abs(U[mask, k]), 1.0)
+                Up = U.copy()
+                Up[mask, k] += h
+                hk = Up[mask, k] - U[mask, k]
+                d = self.residual(Up, first=True) - base
+                J[mask, :, k] = d[mask] / hk[:, None]
+        return np.eye(5)[None] / dtau[:, None, None] - J
+
+    def solve(self, U, verbose=False, **opts):
+        o = dict(DEFAULTS, **opts)
+        U = np.array(U, dtype=np.float64).reshape(self.N, 5)
+        t0 = time.perf_counter()
+        self.evals = 0
+        R0 = self.residual(U)
+        norm = self.stop_norm(R0)
+        l20 = np.linalg.norm(R0)
+        cfl = o["cfl0"]
+        hist = []
+        gm_total = 0
+        it = 0
+        while it < o["max_newton"] and norm > o["tol"]:
+            if not np.isfinite(norm):
+                raise FloatingPointError("non-finite residual")
+            dtau = self.local_dt(U, cfl)
+            if o["precond"] == "block":
+                Minv = np.linalg.inv(self.blocks(U, dtau))
+            else:
+                Minv = np.eye(5)[None] * dtau[:, None, None]
+            M = lambda v: np.einsum("nij,nj->ni", Minv, v.reshape(self.N, 5)).reshape(-1)
+            fd_num = o["fd_eps"] * (1.0 + np.linalg.norm(U))
+            b = R0.reshape(-1)
+
+            def A(z):
+                eps = fd_num / max(np.linalg.norm(z), 1e-300)
+                Rt = self.residual(U + eps * z.reshape(self.N, 5))
+                return (z.reshape(self.N, 5) / dtau[:, None] - (Rt - R0) / eps).reshape(-1)
+
+            x, its, rel = gmres(lambda v: A(M(v)), b, o["krylov_dims"],
+                                o["max_restarts"], o["rtol"])
+            gm_total += its
+            dU = M(x).reshape(self.N, 5)
+            ratio = np.max(np.abs(dU[:, [0, 4]] / U[:, [0, 4]]))
+            alpha = o["max_change"] / ratio if ratio > o["max_change"] else 1.0
+            U = U + alpha * dU
+            prev = norm
+            R0 = self.residual(U)
+            norm = self.stop_norm(R0)
+            it += 1
+            hist.append((prev, cfl, its, rel, alpha, time.perf_counter() - t0))
+            if verbose:
+                print("%4d  %.6e  cfl %.3e  gmres %3d  rel %.2e  alpha %.3f"
+                      % (it, norm, cfl, its, rel, alpha), flush=True)
+            cfl = (min(cfl * o["cfl_growth"], o["cfl_max"]) if norm < 3.0 * prev
+                   else max(cfl * o["cfl_cut"], 1.0))
+        stats = dict(status=0 if norm <= o["tol"] else 1, newton_steps=it,
+                     gmres_iterations=gm_total, residual_evals=self.evals,
+                     seconds=time.perf_counter() - t0, norm=norm,
+                     l2_drop=np.linalg.norm(R0) / l20 if l20 > 0 else 0.0)
+        return U, stats, np.array(hist)
