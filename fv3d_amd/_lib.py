@@ -8,23 +8,27 @@ _LIB = None
 c_int, c_double, c_char_p, c_void_p = ctypes.c_int, ctypes.c_double, ctypes.c_char_p, ctypes.c_void_p
 P_DOUBLE = ctypes.POINTER(ctypes.c_double)
 P_INT = ctypes.POINTER(ctypes.c_int)
+P_FLOAT = ctypes.POINTER(ctypes.c_float)
 
 
 
 class JfnkOptions(ctypes.Structure):
     """fvh_jfnk_options_t (include/fv3d_hip.h)."""
     _fields_ = [("max_newton", c_int), ("tol", c_double), ("cfl0", c_double),
-                ("cfl_growth", c_double), ("cfl_cut", c_double), ("cfl_max", c_double),
+                ("cfl_growth", c_double), ("cfl_cut", c_double), ("cfl_stall", c_double),
+                ("cfl_stall_cut", c_double), ("cfl_max", c_double),
                 ("krylov_dims", c_int), ("max_restarts", c_int), ("rtol", c_double),
                 ("fd_eps", c_double), ("max_change", c_double), ("precision", c_int),
-                ("precond", c_int), ("verbose", c_int)]
+                ("precond", c_int), ("verbose", c_int), ("learned_check", c_int),
+                ("learned_tol", c_double)]
 
 
 class JfnkStats(ctypes.Structure):
     """fvh_jfnk_stats_t (include/fv3d_hip.h)."""
     _fields_ = [("status", c_int), ("newton_steps", c_int), ("gmres_iterations", c_int),
                 ("residual_evals", c_int), ("seconds", c_double), ("norm0", c_double),
-                ("norm", c_double), ("l2_drop", c_double)]
+                ("norm", c_double), ("l2_drop", c_double), ("precond_fallbacks", c_int),
+                ("learned_error", c_double), ("precond_seconds", c_double)]
 
 
 EXACT_FT = ctypes.CFUNCTYPE(None, c_int, c_double, P_DOUBLE, P_DOUBLE)
@@ -110,6 +114,17 @@ SIGNATURES = [
     ("fvh_jfnk_solve", c_int, [ctypes.POINTER(JfnkOptions), ctypes.POINTER(JfnkStats)]),
     ("fvh_jfnk_history", c_int, [P_DOUBLE, c_int]),
     ("fvh_jfnk_blocks", c_int, [c_double, c_int, P_DOUBLE, P_DOUBLE]),
+    ("fvh_nn_param_count", c_int, []),
+    ("fvh_nn_init", c_int, [ctypes.c_ulonglong]),
+    ("fvh_nn_get_params", c_int, [P_FLOAT]),
+    ("fvh_nn_set_params", c_int, [P_FLOAT]),
+    ("fvh_nn_features", c_int, [c_double, P_FLOAT]),
+    ("fvh_nn_collect", c_int, [c_double, ctypes.POINTER(ctypes.c_longlong)]),
+    ("fvh_nn_clear", c_int, []),
+    ("fvh_nn_train", c_int, [c_int, c_int, c_double, ctypes.c_ulonglong, P_DOUBLE]),
+    ("fvh_nn_forward", c_int, [P_FLOAT, c_int, P_FLOAT]),
+    ("fvh_nn_loss_grad", c_int, [P_FLOAT, P_FLOAT, c_int, P_FLOAT, P_DOUBLE]),
+    ("fvh_nn_validate", c_int, [c_double, P_DOUBLE]),
 ]
 
 # exported data symbols (globals) of fv3d_solver.h

@@ -156,15 +156,17 @@ __global__ __launch_bounds__(kBlock) void k_column_sumsq(int N, const double *__
             partials[(size_t)blockIdx.x * kSumStride + k] = a[k];
 }
 
-__global__ void k_fold5(int nblk, const double *__restrict__ partials, double *__restrict__ out)
+__global__ __launch_bounds__(kBlock) void k_fold5(int nblk, const double *__restrict__ partials,
+                                                  double *__restrict__ out)
 {
-    const int k = threadIdx.x;
-    if (k >= 5)
-        return;
+    __shared__ double sh[kBlock];
+    const int k = blockIdx.x;
     double s = 0.0;
-    for (int b = 0; b < nblk; ++b)
+    for (int b = threadIdx.x; b < nblk; b += kBlock)
         s += partials[(size_t)b * kSumStride + k];
-    out[k] = s;
+    dev::block_sum(sh, s);
+    if (threadIdx.x == 0)
+        out[k] = s;
 }
 
 __global__ __launch_bounds__(kBlock) void k_step_partials(DevMesh m, PhysDev P,
@@ -244,7 +246,7 @@ void column_sumsq(hipStream_t s, int N, const double *U, double *partials, doubl
 {
     hipLaunchKernelGGL(k_column_sumsq, dim3(kSumBlocks), dim3(kBlock), 0, s, N, U,
                        partials);
-    hipLaunchKernelGGL(k_fold5, dim3(1), dim3(64), 0, s, kSumBlocks, partials, sums);
+    hipLaunchKernelGGL(k_fold5, dim3(5), dim3(kBlock), 0, s, kSumBlocks, partials, sums);
 }
 
 void step_partials(hipStream_t s, const DevMesh &m, const PhysDev &P, const double *U,

@@ -28,7 +28,7 @@ struct Workspace
     std::vector<void *> allocs;
     double *R0 = nullptr, *Rt = nullptr, *Ut = nullptr, *dtau = nullptr,
            *Minv = nullptr, *xacc = nullptr, *dU = nullptr, *partials = nullptr,
-           *dots = nullptr, *ydev = nullptr;
+           *dots = nullptr, *ydev = nullptr, *Mex = nullptr;
     void *V = nullptr, *z = nullptr, *t = nullptr; // T storage
     double *host = nullptr;                        // pinned, kHost doubles
 };
@@ -240,8 +240,11 @@ int solve(const fvh_jfnk_options_t &o, fvh_jfnk_stats_t &st)
         throw std::runtime_error("krylov_dims must be in [1, 63]");
     if (o.precision != FVH_JFNK_FP64 && o.precision != FVH_JFNK_FP32)
         throw std::runtime_error("unknown precision");
-    if (o.precond < FVH_PRECOND_SCALAR || o.precond > FVH_PRECOND_BLOCK)
-        throw std::runtime_error("unknown or unavailable preconditioner");
+    if (o.precond < FVH_PRECOND_SCALAR || o.precond > FVH_PRECOND_LEARNED)
+        throw std::runtime_error("unknown preconditioner");
+    if (o.precond == FVH_PRECOND_LEARNED && !learned_ready())
+        throw std::runtime_error("learned preconditioner: no network (fvh_nn_init / "
+                                 "fvh_nn_set_params)");
     RuntimeView v = runtime_view();
     const int N = v.m.N, n = 5 * N;
     prepare(N, o.krylov_dims, o.precision);
@@ -261,6 +264,7 @@ int solve(const fvh_jfnk_options_t &o, fvh_jfnk_stats_t &st)
     st.norm0 = norm;
     const double l20 = l2;
     double cfl = o.cfl0;
+    bool learned = o.precond == FVH_PRECOND_LEARNED;
     if (o.verbose)
         std::printf(" newton  stop-norm      cfl        gmres  rel        alpha\n");
     int it = 0;
@@ -270,9 +274,36 @@ int solve(const fvh_jfnk_options_t &o, fvh_jfnk_stats_t &st)
             throw std::runtime_error("non-finite residual in the Newton loop");
         if (norm <= o.tol)
             break;
+        const auto tp = std::chrono::steady_clock::now();
         kry::local_dt(s, v.m, v.P, v.U, cfl, g_ws.dtau);
-        kry::precond_blocks(s, v.m, v.P, v.flux, v.U, g_ws.dtau,
-                            o.precond == FVH_PRECOND_BLOCK ? 1 : 0, g_ws.Minv);
+        if (learned)
+        {
+            learned_blocks(v, cfl, g_ws.dtau, g_ws.Minv);
+            if (it == 0 || (o.learned_check > 0 && it % o.learned_check == 0))
+            {
+                if (!g_ws.Mex)
+                    g_ws.Mex = walloc<double>((size_t)25 * N);
+                kry::precond_blocks(s, v.m, v.P, v.flux, v.U, g_ws.dtau, 1, g_ws.Mex);
+                double err[2];
+                learned_block_error(v, g_ws.Minv, g_ws.Mex, err);
+                st.learned_error = err[1];
+                if (!(err[1] <= o.learned_tol))
+                {
+                    // fall back to block Jacobi for the rest of the solve
+                    learned = false;
+                    ++st.precond_fallbacks;
+                    FVH_HIP_CHECK(hipMemcpyAsync(g_ws.Minv, g_ws.Mex,
+                                                 (size_t)25 * N * sizeof(double),
+                                                 hipMemcpyDeviceToDevice, s));
+                }
+            }
+        }
+        else
+            kry::precond_blocks(s, v.m, v.P, v.flux, v.U, g_ws.dtau,
+                                o.precond == FVH_PRECOND_SCALAR ? 0 : 1, g_ws.Minv);
+        FVH_HIP_CHECK(hipStreamSynchronize(s));
+        st.precond_seconds +=
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - tp).count();
         kry::multidot<double>(s, n, 1, v.U, g_ws.ld, v.U, g_ws.partials, g_ws.dots + kUU);
         fetch(s, kUU, 1);
         const double fd_num = o.fd_eps * (1.0 + std::sqrt(g_ws.host[kUU]));
@@ -296,8 +327,12 @@ int solve(const fvh_jfnk_options_t &o, fvh_jfnk_stats_t &st)
         if (o.verbose)
             std::printf("%7d  %.6e  %.3e  %5d  %.3e  %.3f\n", it + 1, norm, cfl, gr.its,
                         gr.rel, alpha);
-        cfl = norm < 3.0 * prev ? std::min(cfl * o.cfl_growth, o.cfl_max)
-                                : std::max(cfl * o.cfl_cut, 1.0);
+        if (!(norm < 3.0 * prev))
+            cfl = std::max(cfl * o.cfl_cut, 1.0);
+        else if (gr.rel > o.cfl_stall)
+            cfl = std::max(cfl * o.cfl_stall_cut, 1.0);
+        else
+            cfl = std::min(cfl * o.cfl_growth, o.cfl_max);
     }
     FVH_HIP_CHECK(hipStreamSynchronize(s));
     st.seconds = elapsed();
@@ -328,6 +363,8 @@ int fvh_jfnk_default_options(fvh_jfnk_options_t *o)
     o->cfl0 = 10.0;
     o->cfl_growth = 1.5;
     o->cfl_cut = 0.3;
+    o->cfl_stall = 0.5;
+    o->cfl_stall_cut = 0.7;
     o->cfl_max = 1e6;
     o->krylov_dims = 30;
     o->max_restarts = 3;
@@ -337,6 +374,8 @@ int fvh_jfnk_default_options(fvh_jfnk_options_t *o)
     o->precision = FVH_JFNK_FP64;
     o->precond = FVH_PRECOND_BLOCK;
     o->verbose = 0;
+    o->learned_check = 10;
+    o->learned_tol = 0.5;
     return FVH_OK;
 }
 

@@ -324,16 +324,20 @@ __global__ __launch_bounds__(kBlock) void k_multidot(int n, int k, const T *__re
     }
 }
 
-__global__ void k_fold_dots(int nblk, int k, const double *__restrict__ partials,
-                            double *__restrict__ dots)
+// One block per dot: thread t adds the rows t, t + 256, ... in order, then
+// a fixed tree over the threads (deterministic, and short: the rows are
+// read in parallel).
+__global__ __launch_bounds__(kBlock) void k_fold_dots(int nblk, const double *__restrict__ partials,
+                                                      double *__restrict__ dots)
 {
-    const int j = threadIdx.x;
-    if (j >= k)
-        return;
+    __shared__ double sh[kBlock];
+    const int j = blockIdx.x;
     double s = 0.0;
-    for (int b = 0; b < nblk; ++b)
+    for (int b = threadIdx.x; b < nblk; b += kBlock)
         s += partials[(size_t)b * kMaxDots + j];
-    dots[j] = s;
+    dev::block_sum(sh, s);
+    if (threadIdx.x == 0)
+        dots[j] = s;
 }
 
 template <typename T>
@@ -434,15 +438,17 @@ __global__ __launch_bounds__(kBlock) void k_update_ratio(int N, const double *__
         partials[(size_t)blockIdx.x * kMaxDots] = -na;
 }
 
-__global__ void k_fold_max(int nblk, const double *__restrict__ partials,
-                           double *__restrict__ out)
+__global__ __launch_bounds__(kBlock) void k_fold_max(int nblk, const double *__restrict__ partials,
+                                                     double *__restrict__ out)
 {
-    if (threadIdx.x != 0)
-        return;
+    __shared__ double sh[kBlock];
     double a = 0.0;
-    for (int b = 0; b < nblk; ++b)
+    for (int b = threadIdx.x; b < nblk; b += kBlock)
         a = fmax(a, partials[(size_t)b * kMaxDots]);
-    out[0] = a;
+    double na = -a;
+    dev::block_min(sh, na);
+    if (threadIdx.x == 0)
+        out[0] = -na;
 }
 
 __global__ __launch_bounds__(kBlock) void k_update(int n, double *__restrict__ U,
@@ -531,8 +537,7 @@ void multidot(hipStream_t s, int n, int k, const T *V, size_t ld, const T *w,
         throw std::runtime_error("multidot: bad vector count");
     hipLaunchKernelGGL(k_multidot<T>, dim3(kDotBlocks), dim3(kBlock), 0, s, n, k, V, ld, w,
                        partials);
-    hipLaunchKernelGGL(k_fold_dots, dim3(1), dim3(kMaxDots), 0, s, kDotBlocks, k, partials,
-                       dots);
+    hipLaunchKernelGGL(k_fold_dots, dim3(k), dim3(kBlock), 0, s, kDotBlocks, partials, dots);
 }
 
 template <typename T>
@@ -572,8 +577,7 @@ void residual_sums(hipStream_t s, int N, const double *R, const double *vol,
 {
     hipLaunchKernelGGL(k_residual_sums, dim3(kDotBlocks), dim3(kBlock), 0, s, N, R, vol,
                        partials);
-    hipLaunchKernelGGL(k_fold_dots, dim3(1), dim3(kMaxDots), 0, s, kDotBlocks, 5, partials,
-                       sums);
+    hipLaunchKernelGGL(k_fold_dots, dim3(5), dim3(kBlock), 0, s, kDotBlocks, partials, sums);
 }
 
 void update_ratio(hipStream_t s, int N, const double *U, const double *dU, double *partials,
@@ -581,7 +585,7 @@ void update_ratio(hipStream_t s, int N, const double *U, const double *dU, doubl
 {
     hipLaunchKernelGGL(k_update_ratio, dim3(kDotBlocks), dim3(kBlock), 0, s, N, U, dU,
                        partials);
-    hipLaunchKernelGGL(k_fold_max, dim3(1), dim3(64), 0, s, kDotBlocks, partials, out);
+    hipLaunchKernelGGL(k_fold_max, dim3(1), dim3(kBlock), 0, s, kDotBlocks, partials, out);
 }
 
 void update(hipStream_t s, int n, double *U, const double *dU, double alpha)

@@ -33,6 +33,7 @@ struct RuntimeView
     int viscous;   // mu != 0
     double *U;     // current conservative state [N][5]
     double vglob;  // total volume
+    double rho_ref, p_ref; // density and pressure of the flow region state
 };
 
 // Throws unless init_finite_volume ran with the built-in hooks and the
@@ -47,8 +48,18 @@ void eval_residual(const double *U, double *R);
 // and host mirror are stale.
 void state_changed();
 
-// Releases device work buffers of the Newton-Krylov driver (called when the
-// device state is freed).
+// Releases device work buffers of the Newton-Krylov driver and the
+// per-mesh buffers of the learned preconditioner (called when the device
+// state is freed).
 void jfnk_release();
+void learned_release_mesh();
+
+// Learned preconditioner (learned.cpp): whether a network is loaded, its
+// inverse blocks for the current state at the pseudo steps dtau, and the
+// max / mean per-cell relative Frobenius error of blocks A against B.
+bool learned_ready();
+void learned_blocks(const RuntimeView &v, double cfl, const double *dtau, double *Minv);
+void learned_block_error(const RuntimeView &v, const double *A, const double *B,
+                         double *err);
 
 } // namespace fvh

@@ -198,6 +198,7 @@ void free_device()
     if (g_dev.stream)
         (void)hipStreamSynchronize(g_dev.stream);
     jfnk_release();
+    learned_release_mesh();
     for (void *p : g_dev.allocs)
         (void)hipFree(p);
     if (g_dev.sc_host)
@@ -936,6 +937,14 @@ RuntimeView runtime_view()
     v.viscous = is_viscous();
     v.U = g_dev.U[g_dev.cur];
     v.vglob = g_dev.vglob;
+    const Regions_t *r = solver_mesh->regions;
+    v.rho_ref = r->phi_total[(size_t)r->flow_region * 10 + 0];
+    v.p_ref = r->phi_total[(size_t)r->flow_region * 10 + 8];
+    if (!(v.rho_ref > 0.0) || !(v.p_ref > 0.0))
+    {
+        v.rho_ref = 1.0;
+        v.p_ref = 1.0;
+    }
     return v;
 }
 

@@ -247,6 +247,7 @@ class Case:
         self.n_faces = m.faces.contents.n_faces
         self.n_boundaries = m.boundaries.contents.n_boundaries
         Case._active = self
+        self.initial_state = self.state()
 
     # -- state ---------------------------------------------------------------
     def state(self):

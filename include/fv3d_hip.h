@@ -130,9 +130,12 @@ typedef struct
     double tol;          /* stop when max_j (1/V) sum_i |R_ij| V_i <= tol
                             (default 1e-8) */
     double cfl0;         /* initial pseudo CFL (default 10) */
-    double cfl_growth;   /* CFL factor after a step that did not raise the
-                            norm above 3x the previous one (default 1.5) */
-    double cfl_cut;      /* CFL factor otherwise (default 0.3) */
+    double cfl_growth;   /* CFL factor after a successful step (default 1.5) */
+    double cfl_cut;      /* CFL factor after a step that raised the stop norm
+                            above 3x the previous one (default 0.3) */
+    double cfl_stall;    /* GMRES relative residual above which a step counts
+                            as stalled (default 0.5; > 1 disables) */
+    double cfl_stall_cut; /* CFL factor after a stalled step (default 0.7) */
     double cfl_max;      /* (default 1e6) */
     int krylov_dims;     /* GMRES restart length m (default 30, <= 60) */
     int max_restarts;    /* GMRES cycles per Newton step minus one (default 3) */
@@ -143,6 +146,11 @@ typedef struct
     int precision;       /* FVH_JFNK_FP64 / FVH_JFNK_FP32 */
     int precond;         /* FVH_PRECOND_* (default FVH_PRECOND_BLOCK) */
     int verbose;         /* print one line per Newton step */
+    int learned_check;   /* FVH_PRECOND_LEARNED: compare the predicted blocks
+                            with the block-Jacobi blocks every learned_check
+                            Newton steps (and at the first); 0 = first only */
+    double learned_tol;  /* mean per-cell relative Frobenius error above which
+                            the solve falls back to block Jacobi (default 0.5) */
 } fvh_jfnk_options_t;
 
 typedef struct
@@ -154,6 +162,9 @@ typedef struct
     double seconds;       /* wall time of the Newton loop */
     double norm0, norm;   /* stop norm at the start and at the end */
     double l2_drop;       /* |R|_2 end / start */
+    int precond_fallbacks; /* learned -> block-Jacobi fallbacks (0 or 1) */
+    double learned_error;  /* last measured mean block error of the network */
+    double precond_seconds; /* wall time spent building preconditioners */
 } fvh_jfnk_stats_t;
 
 int fvh_jfnk_default_options(fvh_jfnk_options_t *opt);
@@ -166,6 +177,40 @@ int fvh_jfnk_history(double *rows, int max_rows);
  * minv[N][5][5] (row-major) of M_c = I/dtau_c - dR1_c/dU_c (precond
  * FVH_PRECOND_BLOCK) or of I/dtau_c (FVH_PRECOND_SCALAR). */
 int fvh_jfnk_blocks(double cfl, int precond, double *dtau, double *minv);
+
+/* ---- learned block preconditioner (network of FVH_PRECOND_LEARNED) ---- */
+/* A per-cell MLP 28 -> 64 -> 64 -> 25 (tanh) maps local features of the
+ * state (normalised primitives, their scaled Green-Gauss gradients, face
+ * geometry moments, boundary face count, log CFL) to the 5x5 inverse
+ * block-Jacobi block, normalised by the flow-region state and the pseudo
+ * step.  BF16 MFMA kernels with FP32 accumulation run inference and
+ * training; master weights and Adam are FP32.  The parameters are one FP32
+ * vector of fvh_nn_param_count() values (W1[32][64], b1[64], W2[64][64],
+ * b2[64], W3[64][32], b3[32], row-major [in][out]; input rows 28..31 and
+ * output columns 25..31 are padding). */
+int fvh_nn_param_count(void);
+int fvh_nn_init(unsigned long long seed);
+int fvh_nn_get_params(float *theta);
+int fvh_nn_set_params(const float *theta);
+/* Features [N][32] of the current state at the given pseudo CFL. */
+int fvh_nn_features(double cfl, float *feat);
+/* Appends the features and block-Jacobi targets of the current state at
+ * the given CFL to the device training set; n_total receives its size. */
+int fvh_nn_collect(double cfl, long long *n_total);
+int fvh_nn_clear(void);
+/* Adam training over the training set, mini-batches in shuffled order;
+ * loss[epochs] receives the mean batch loss per epoch. */
+int fvh_nn_train(int epochs, int batch, double lr, unsigned long long seed,
+                 double *loss);
+/* Network output [n][32] for host features [n][32] (inference kernel). */
+int fvh_nn_forward(const float *feat, int n, float *out);
+/* Loss and its gradient [param_count] for host features and targets
+ * [n][32] (the training kernel's forward and backward pass). */
+int fvh_nn_loss_grad(const float *feat, const float *target, int n,
+                     float *grad, double *loss);
+/* Max and mean per-cell relative Frobenius error of the network's inverse
+ * blocks against the block-Jacobi blocks at the current state. */
+int fvh_nn_validate(double cfl, double *err);
 
 #ifdef __cplusplus
 }

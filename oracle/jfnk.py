@@ -22,8 +22,9 @@ written plainly in FP64 numpy:
     classical Gram-Schmidt twice, Givens rotations, relative tolerance rtol;
   - update U += alpha M^-1 x with alpha = min(1, max_change / max relative
     change of rho and rho*E);
-  - CFL times cfl_growth when the new stop norm is below 3x the old one,
-    else times cfl_cut (not below 1), capped at cfl_max.
+  - CFL times cfl_cut (not below 1) when the new stop norm exceeds 3x the
+    old one, else times cfl_stall_cut when GMRES ended above cfl_stall
+    relative residual, else times cfl_growth (capped at cfl_max).
 Stop norm: max_j (1/V_glob) sum_i |R_ij| V_i (src/Solver/src/analyze/
 analyze.c:37-61 without the dt factor).
 """
@@ -34,7 +35,7 @@ import numpy as np
 from . import fvref
 
 DEFAULTS = dict(max_newton=300, tol=1e-8, cfl0=10.0, cfl_growth=1.5, cfl_cut=0.3,
-                cfl_max=1e6, krylov_dims=30, max_restarts=3, rtol=1e-2, fd_eps=1e-7,
+                cfl_stall=0.5, cfl_stall_cut=0.7, cfl_max=1e6, krylov_dims=30, max_restarts=3, rtol=1e-2, fd_eps=1e-7,
                 max_change=0.2, precond="block")
 
 
@@ -209,8 +210,12 @@ class OracleJfnk:
             if verbose:
                 print("%4d  %.6e  cfl %.3e  gmres %3d  rel %.2e  alpha %.3f"
                       % (it, norm, cfl, its, rel, alpha), flush=True)
-            cfl = (min(cfl * o["cfl_growth"], o["cfl_max"]) if norm < 3.0 * prev
-                   else max(cfl * o["cfl_cut"], 1.0))
+            if not norm < 3.0 * prev:
+                cfl = max(cfl * o["cfl_cut"], 1.0)
+            elif rel > o["cfl_stall"]:
+                cfl = max(cfl * o["cfl_stall_cut"], 1.0)
+            else:
+                cfl = min(cfl * o["cfl_growth"], o["cfl_max"])
         stats = dict(status=0 if norm <= o["tol"] else 1, newton_steps=it,
                      gmres_iterations=gm_total, residual_evals=self.evals,
                      seconds=time.perf_counter() - t0, norm=norm,
