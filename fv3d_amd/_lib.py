@@ -114,6 +114,13 @@ SIGNATURES = [
     ("fvh_jfnk_solve", c_int, [ctypes.POINTER(JfnkOptions), ctypes.POINTER(JfnkStats)]),
     ("fvh_jfnk_history", c_int, [P_DOUBLE, c_int]),
     ("fvh_jfnk_blocks", c_int, [c_double, c_int, P_DOUBLE, P_DOUBLE]),
+    ("fvh_hdf5_available", c_int, []),
+    ("fvh_set_output_format", c_int, [c_int]),
+    ("fvh_mesh_batch", c_int, [c_int, P_INT, c_double, c_int]),
+    ("fvh_batch_setup", c_int, [c_int, P_INT, P_INT, P_DOUBLE]),
+    ("fvh_batch_run", c_int, [c_int, c_int, P_DOUBLE]),
+    ("fvh_batch_info", c_int, [P_INT, P_INT, c_int]),
+    ("fvh_batch_case_scalars", c_int, [P_DOUBLE, P_DOUBLE, P_DOUBLE]),
     ("fvh_nn_param_count", c_int, []),
     ("fvh_nn_init", c_int, [ctypes.c_ulonglong]),
     ("fvh_nn_get_params", c_int, [P_FLOAT]),

@@ -1,0 +1,403 @@
+// Batched mode: many independent periodic cases of uneven size in one mesh
+// (disjoint cell ranges, one device state), advanced together by the
+// explicit RK scheme.  Cases are grouped by (flux, viscous) into contiguous
+// cell ranges; each group is one launch of the gradient (viscous groups)
+// and residual/stage kernels per stage, every case steps with its own dt,
+// and per-case residual norms and time steps come from chunked reductions
+// folded per case.  A step is captured once into a HIP graph (one per
+// ping-pong parity) and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <vector>
+
+#include "batch.h"
+#include "fvh_internal.h"
+#include "runtime.h"
+
+namespace fvh {
+
+namespace {
+
+struct Group
+{
+    int flux, viscous, c_begin, c_end;
+    double mu;
+};
+
+struct BatchState
+{
+    bool ready = false;
+    std::vector<int> offsets; // case cell offsets in the union mesh [K+1]
+    int K = 0;
+    std::vector<Group> groups;
+    std::vector<int> case_flux, case_visc;
+    std::vector<double> case_mu;
+    int n_chunks = 0;
+    std::vector<void *> allocs;
+    int *cell_case = nullptr, *chunk_begin = nullptr;
+    int3 *chunks = nullptr;
+    double *partials = nullptr, *d_mu = nullptr, *d_vol = nullptr, *d_dt = nullptr,
+           *d_t = nullptr, *d_res = nullptr, *vg = nullptr;
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    int steps = 0;
+};
+
+BatchState g_b;
+std::vector<int> g_mesh_offsets; // from the last fvh_mesh_batch
+
+template <class X> X *balloc(size_t n)
+{
+    void *p = nullptr;
+    FVH_HIP_CHECK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(X)));
+    g_b.allocs.push_back(p);
+    return static_cast<X *>(p);
+}
+
+void release_graphs()
+{
+    for (int i = 0; i < 2; ++i)
+    {
+        if (g_b.exec[i])
+            (void)hipGraphExecDestroy(g_b.exec[i]);
+        if (g_b.graph[i])
+            (void)hipGraphDestroy(g_b.graph[i]);
+        g_b.exec[i] = nullptr;
+        g_b.graph[i] = nullptr;
+    }
+}
+
+void release()
+{
+    release_graphs();
+    for (void *p : g_b.allocs)
+        (void)hipFree(p);
+    g_b = BatchState();
+}
+
+// Appends src to dst with index offsets (cells, faces, vertices,
+// boundaries); regions are those of the first mesh.
+void append_mesh(RawMesh &d, const RawMesh &s)
+{
+    const int vo = (int)(d.vertices.size() / 3), co = d.n_cells(), fo = d.n_faces(),
+              bo = d.n_boundaries();
+    auto shift = [](std::vector<int> &dst, const std::vector<int> &src, int off) {
+        for (int v : src)
+            dst.push_back(v >= 0 ? v + off : v);
+    };
+    auto cat = [](auto &dst, const auto &src) { dst.insert(dst.end(), src.begin(), src.end()); };
+    if (co == 0)
+    {
+        d.dimension = s.dimension;
+        d.region_names = s.region_names;
+        d.region_is_boundary = s.region_is_boundary;
+    }
+    if (co > 0 && (d.max_cell_vertices != s.max_cell_vertices ||
+                   d.max_cell_faces != s.max_cell_faces ||
+                   d.max_face_vertices != s.max_face_vertices ||
+                   d.max_boundary_vertices != s.max_boundary_vertices))
+        throw std::runtime_error("batch: cases must share the element type");
+    d.max_cell_vertices = s.max_cell_vertices;
+    d.max_cell_faces = s.max_cell_faces;
+    d.max_face_vertices = s.max_face_vertices;
+    d.max_boundary_vertices = s.max_boundary_vertices;
+    cat(d.vertices, s.vertices);
+    cat(d.cell_region, s.cell_region);
+    cat(d.cell_type, s.cell_type);
+    cat(d.cell_nv, s.cell_nv);
+    shift(d.cell_vertices, s.cell_vertices, vo);
+    cat(d.cell_nf, s.cell_nf);
+    shift(d.cell_faces, s.cell_faces, fo);
+    cat(d.cell_volume, s.cell_volume);
+    cat(d.cell_x, s.cell_x);
+    cat(d.cell_dx, s.cell_dx);
+    cat(d.face_type, s.face_type);
+    cat(d.face_nv, s.face_nv);
+    shift(d.face_vertices, s.face_vertices, vo);
+    shift(d.face_cells, s.face_cells, co);
+    shift(d.face_boundary, s.face_boundary, bo);
+    cat(d.face_x, s.face_x);
+    cat(d.face_n, s.face_n);
+    cat(d.face_t1, s.face_t1);
+    cat(d.face_t2, s.face_t2);
+    cat(d.face_area, s.face_area);
+    cat(d.face_lambda, s.face_lambda);
+    cat(d.face_shift, s.face_shift);
+    cat(d.bnd_region, s.bnd_region);
+    cat(d.bnd_type, s.bnd_type);
+    cat(d.bnd_nv, s.bnd_nv);
+    shift(d.bnd_vertices, s.bnd_vertices, vo);
+    shift(d.bnd_face, s.bnd_face, fo);
+    cat(d.bnd_n, s.bnd_n);
+    cat(d.bnd_t1, s.bnd_t1);
+    cat(d.bnd_t2, s.bnd_t2);
+    cat(d.bnd_distance, s.bnd_distance);
+}
+
+// Enqueues one RK step of every case (the stages of every group, then the
+// per-case residuals and next time steps).
+void enqueue_step(const RuntimeView &v, const StageBuffers &sb)
+{
+    hipStream_t s = v.stream;
+    const dev::DevMesh &m = v.m;
+    for (int st = 0; st < sb.stages; ++st)
+    {
+        const bool last = st == sb.stages - 1;
+        double *Uin = sb.U[*sb.cur], *Uout = sb.U[1 - *sb.cur];
+        for (const Group &g : g_b.groups)
+        {
+            dev::PhysDev P = v.P;
+            P.mu = g.mu;
+            P.lam = g.mu * cp / Pr;
+            if (g.viscous)
+            {
+                dev::GradOut o{};
+                o.vg = g_b.vg;
+                o.c_begin = g.c_begin;
+                o.c_end = g.c_end;
+                dev::launch_gradient(s, m, P, Uin, nullptr, o, 0, nullptr);
+            }
+            dev::StageParams sp{};
+            sp.U_in = Uin;
+            sp.U_out = Uout;
+            sp.k = sb.k;
+            sp.R_out = last ? sb.R : nullptr;
+            sp.vg = g_b.vg;
+            sp.rk_a = sb.rk_a[st];
+            sp.rk_g = sb.rk_g[st];
+            sp.stage0 = st == 0;
+            sp.do_update = 1;
+            sp.c_begin = g.c_begin;
+            sp.c_end = g.c_end;
+            sp.case_dt = g_b.d_dt;
+            sp.cell_case = g_b.cell_case;
+            dev::launch_residual(s, m, P, sp, g.flux, false, g.viscous != 0);
+        }
+        *sb.cur = 1 - *sb.cur;
+    }
+    // residual of the last stage, dt from the new density and the last
+    // stage's input primitives
+    batch::chunk_partials(s, m, v.P, g_b.n_chunks, g_b.chunks, sb.U[*sb.cur],
+                          sb.U[1 - *sb.cur], sb.R, g_b.partials);
+    batch::case_fold(s, 1, v.P, g_b.K, g_b.chunk_begin, g_b.partials, g_b.d_mu, g_b.d_vol,
+                     g_b.d_dt, g_b.d_t, g_b.d_res);
+}
+
+} // namespace
+
+void batch_release() { release(); }
+
+} // namespace fvh
+
+using namespace fvh;
+
+#define B_TRY(...)                                                             \
+    try                                                                        \
+    {                                                                          \
+        __VA_ARGS__;                                                           \
+    }                                                                          \
+    catch (const std::exception &e)                                            \
+    {                                                                          \
+        set_error(e.what());                                                   \
+        return FVH_ERR_STATE;                                                  \
+    }                                                                          \
+    return FVH_OK;
+
+int fvh_mesh_batch(int n_cases, const int *n_hex, double length, int tile)
+{
+    B_TRY({
+        if (n_cases < 1 || !n_hex)
+            throw std::runtime_error("fvh_mesh_batch: no cases");
+        RawMesh all;
+        std::vector<int> off(1, 0);
+        for (int k = 0; k < n_cases; ++k)
+        {
+            if (n_hex[k] < 3)
+                throw std::runtime_error("fvh_mesh_batch: a periodic case needs >= 3 "
+                                         "hexahedra per side");
+            RawMesh m;
+            generate_box(m, n_hex[k], n_hex[k], n_hex[k], 0.0, 0.0, 0.0, length, length,
+                         length, true, 7, tile);
+            append_mesh(all, m);
+            off.push_back(all.n_cells());
+        }
+        release();
+        install_mesh(std::move(all));
+        g_mesh_offsets = off;
+    });
+}
+
+int fvh_batch_setup(int n_cases, const int *flux, const int *viscous, const double *mu)
+{
+    B_TRY({
+        if (n_cases + 1 != (int)g_mesh_offsets.size())
+            throw std::runtime_error("fvh_batch_setup: case count differs from fvh_mesh_batch");
+        RuntimeView v = runtime_view();
+        if (v.m.B != 0)
+            throw std::runtime_error("fvh_batch_setup: batched cases are periodic");
+        if (v.linear)
+            throw std::runtime_error("fvh_batch_setup: batched cases use first-order "
+                                     "reconstruction");
+        release();
+        g_b.offsets = g_mesh_offsets;
+        g_b.K = n_cases;
+        const int N = v.m.N;
+        if (g_b.offsets.back() != N)
+            throw std::runtime_error("fvh_batch_setup: mesh is not the last fvh_mesh_batch");
+        // groups: maximal runs of equal (flux, viscous, mu)
+        for (int k = 0; k < n_cases; ++k)
+        {
+            const int vis = viscous[k] != 0 && mu[k] != 0.0;
+            g_b.case_flux.push_back(flux[k]);
+            g_b.case_visc.push_back(vis);
+            g_b.case_mu.push_back(vis ? mu[k] : 0.0);
+            if (!g_b.groups.empty() && g_b.groups.back().flux == flux[k] &&
+                g_b.groups.back().viscous == vis && g_b.groups.back().mu == g_b.case_mu[k])
+                g_b.groups.back().c_end = g_b.offsets[k + 1];
+            else
+                g_b.groups.push_back({flux[k], vis, g_b.offsets[k], g_b.offsets[k + 1],
+                                      g_b.case_mu[k]});
+        }
+        // host tables
+        std::vector<int> cell_case(N), chunk_begin(1, 0);
+        std::vector<int3> chunks;
+        std::vector<double> vol(n_cases, 0.0);
+        const double *cv = solver_mesh->cells->volume;
+        for (int k = 0; k < n_cases; ++k)
+        {
+            for (int c = g_b.offsets[k]; c < g_b.offsets[k + 1]; ++c)
+            {
+                cell_case[c] = k;
+                vol[k] += cv[c];
+            }
+            for (int c = g_b.offsets[k]; c < g_b.offsets[k + 1]; c += batch::kChunkCells)
+                chunks.push_back(make_int3(k, c, std::min(c + batch::kChunkCells,
+                                                          g_b.offsets[k + 1])));
+            chunk_begin.push_back((int)chunks.size());
+        }
+        g_b.n_chunks = (int)chunks.size();
+        g_b.cell_case = balloc<int>(N);
+        g_b.chunk_begin = balloc<int>(n_cases + 1);
+        g_b.chunks = balloc<int3>(chunks.size());
+        g_b.partials = balloc<double>((size_t)8 * chunks.size());
+        g_b.d_mu = balloc<double>(n_cases);
+        g_b.d_vol = balloc<double>(n_cases);
+        g_b.d_dt = balloc<double>(n_cases);
+        g_b.d_t = balloc<double>(n_cases);
+        g_b.d_res = balloc<double>((size_t)5 * n_cases);
+        bool any_visc = false;
+        for (const Group &g : g_b.groups)
+            any_visc = any_visc || g.viscous;
+        if (any_visc)
+            g_b.vg = balloc<double>((size_t)12 * N);
+        FVH_HIP_CHECK(hipMemcpy(g_b.cell_case, cell_case.data(), N * sizeof(int),
+                                hipMemcpyHostToDevice));
+        FVH_HIP_CHECK(hipMemcpy(g_b.chunk_begin, chunk_begin.data(),
+                                chunk_begin.size() * sizeof(int), hipMemcpyHostToDevice));
+        FVH_HIP_CHECK(hipMemcpy(g_b.chunks, chunks.data(), chunks.size() * sizeof(int3),
+                                hipMemcpyHostToDevice));
+        FVH_HIP_CHECK(hipMemcpy(g_b.d_mu, g_b.case_mu.data(), n_cases * sizeof(double),
+                                hipMemcpyHostToDevice));
+        FVH_HIP_CHECK(hipMemcpy(g_b.d_vol, vol.data(), n_cases * sizeof(double),
+                                hipMemcpyHostToDevice));
+        std::vector<double> zeros((size_t)5 * n_cases, 0.0);
+        FVH_HIP_CHECK(hipMemcpy(g_b.d_t, zeros.data(), n_cases * sizeof(double),
+                                hipMemcpyHostToDevice));
+        FVH_HIP_CHECK(hipMemcpy(g_b.d_res, zeros.data(), zeros.size() * sizeof(double),
+                                hipMemcpyHostToDevice));
+        // initial time steps from the current state (primitives refreshed)
+        batch::chunk_partials(v.stream, v.m, v.P, g_b.n_chunks, g_b.chunks, v.U, v.U, nullptr,
+                              g_b.partials);
+        batch::case_fold(v.stream, 0, v.P, n_cases, g_b.chunk_begin, g_b.partials, g_b.d_mu,
+                         g_b.d_vol, g_b.d_dt, g_b.d_t, g_b.d_res);
+        FVH_HIP_CHECK(hipStreamSynchronize(v.stream));
+        g_b.ready = true;
+    });
+}
+
+int fvh_batch_run(int steps, int use_graph, double *seconds)
+{
+    B_TRY({
+        if (!g_b.ready)
+            throw std::runtime_error("fvh_batch_run: no batch (fvh_batch_setup)");
+        RuntimeView v = runtime_view();
+        StageBuffers sb = stage_buffers();
+        if (sb.stages % 2 == 0 && use_graph)
+            throw std::runtime_error("fvh_batch_run: graphs assume an odd stage count");
+        hipStream_t s = v.stream;
+        if (use_graph)
+            for (int par = 0; par < 2; ++par)
+            {
+                const int want = (*sb.cur + par) & 1;
+                if (g_b.exec[want])
+                    continue;
+                // capture the step that starts from ping-pong buffer `want`
+                const int saved = *sb.cur;
+                *sb.cur = want;
+                FVH_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeGlobal));
+                enqueue_step(v, sb);
+                FVH_HIP_CHECK(hipStreamEndCapture(s, &g_b.graph[want]));
+                FVH_HIP_CHECK(hipGraphInstantiate(&g_b.exec[want], g_b.graph[want], nullptr,
+                                                  nullptr, 0));
+                *sb.cur = saved;
+            }
+        hipEvent_t e0, e1;
+        FVH_HIP_CHECK(hipEventCreate(&e0));
+        FVH_HIP_CHECK(hipEventCreate(&e1));
+        FVH_HIP_CHECK(hipEventRecord(e0, s));
+        for (int i = 0; i < steps; ++i)
+        {
+            if (use_graph)
+            {
+                FVH_HIP_CHECK(hipGraphLaunch(g_b.exec[*sb.cur], s));
+                *sb.cur = 1 - *sb.cur; // odd stage count: the step flips the buffers
+            }
+            else
+                enqueue_step(v, sb);
+        }
+        FVH_HIP_CHECK(hipEventRecord(e1, s));
+        FVH_HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        FVH_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (seconds)
+            *seconds = ms * 1e-3;
+        g_b.steps += steps;
+        state_changed();
+    });
+}
+
+int fvh_batch_info(int *n_cases, int *offsets, int max_offsets)
+{
+    B_TRY({
+        if (!g_b.ready)
+            throw std::runtime_error("no batch");
+        if (n_cases)
+            *n_cases = g_b.K;
+        if (offsets)
+            for (int i = 0; i <= g_b.K && i < max_offsets; ++i)
+                offsets[i] = g_b.offsets[i];
+    });
+}
+
+int fvh_batch_case_scalars(double *dt, double *t, double *res)
+{
+    B_TRY({
+        if (!g_b.ready)
+            throw std::runtime_error("no batch");
+        FVH_HIP_CHECK(hipDeviceSynchronize());
+        if (dt)
+            FVH_HIP_CHECK(hipMemcpy(dt, g_b.d_dt, g_b.K * sizeof(double), hipMemcpyDeviceToHost));
+        if (t)
+            FVH_HIP_CHECK(hipMemcpy(t, g_b.d_t, g_b.K * sizeof(double), hipMemcpyDeviceToHost));
+        if (res)
+            FVH_HIP_CHECK(hipMemcpy(res, g_b.d_res, (size_t)5 * g_b.K * sizeof(double),
+                                    hipMemcpyDeviceToHost));
+    });
+}

@@ -74,6 +74,7 @@ struct GradOut
     double *fs;   // [F*16] face states
     double *grad; // [N*30] all gradients (tests)
     double *mnmx; // [N*20] neighbourhood min/max (tests)
+    int c_begin, c_end; // cells [c_begin, c_end); c_end = 0: all cells
 };
 
 struct StageParams
@@ -92,6 +93,9 @@ struct StageParams
     DevScalars *sc;        // dt, done flag
     double *partials;      // [partials_cap * 8]
     int partials_cap;      // rows available in partials
+    int c_begin, c_end;    // cells [c_begin, c_end); c_end = 0: all cells
+    const double *case_dt; // batched cases: dt of case cell_case[c] (sc unused)
+    const int *cell_case;
 };
 
 constexpr int kBlock = 256;

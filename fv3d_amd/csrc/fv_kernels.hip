@@ -95,8 +95,8 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
     if (sc && sc->done)
         return;
     const int N = m.N;
-    const int c = xcd_tile(gridDim.x) * kBlock + threadIdx.x;
-    if (c >= N)
+    const int c = out.c_begin + xcd_tile(gridDim.x) * kBlock + threadIdx.x;
+    if (c >= out.c_end)
         return;
 
     double pc[10];
@@ -282,9 +282,9 @@ __global__ __launch_bounds__(kBlock, FVH_K2_MINW) void fv_residual_group_kernel(
     __shared__ double sh[kBlock];
     const int N = m.N;
     const int tid = xcd_tile(gridDim.x) * kBlock + threadIdx.x;
-    const int c = tid / LANES;
+    const int c = sp.c_begin + tid / LANES;
     const int s = tid & (LANES - 1);
-    const bool cell_ok = c < N;
+    const bool cell_ok = c < sp.c_end;
 
     if (sp.sc && sp.sc->done)
     {
@@ -417,7 +417,7 @@ __global__ __launch_bounds__(kBlock, FVH_K2_MINW) void fv_residual_group_kernel(
         {
             double pc[10];
             load_cons(sp.U_in, c, pc);
-            const double dt = sp.sc->dt;
+            const double dt = sp.case_dt ? sp.case_dt[sp.cell_case[c]] : sp.sc->dt;
             double unew[5];
 #pragma unroll
             for (int j = 0; j < 5; ++j)
@@ -702,16 +702,25 @@ void launch_ghost(hipStream_t s, const DevMesh &m, const PhysDev &P,
 }
 
 void launch_gradient(hipStream_t s, const DevMesh &m, const PhysDev &P,
-                     const double *U, const double *gphi, const GradOut &out,
+                     const double *U, const double *gphi, const GradOut &out_in,
                      int linear, const DevScalars *sc)
 {
     check_faces(m);
+    GradOut out = out_in;
+    if (out.c_end <= 0)
+    {
+        out.c_begin = 0;
+        out.c_end = m.N;
+    }
+    if (out.c_begin < 0 || out.c_end > m.N || out.c_begin >= out.c_end)
+        throw std::runtime_error("launch_gradient: bad cell range");
+    const int grid = cell_blocks(out.c_end - out.c_begin);
     if (m.maxf <= 4)
-        hipLaunchKernelGGL(fv_gradient_kernel<4>, dim3(cell_blocks(m.N)),
+        hipLaunchKernelGGL(fv_gradient_kernel<4>, dim3(grid),
                            dim3(kBlock), FVH_K1_LDS, s, m, P, U, gphi, out, linear,
                            sc);
     else
-        hipLaunchKernelGGL(fv_gradient_kernel<6>, dim3(cell_blocks(m.N)),
+        hipLaunchKernelGGL(fv_gradient_kernel<6>, dim3(grid),
                            dim3(kBlock), FVH_K1_LDS, s, m, P, U, gphi, out, linear,
                            sc);
 }
@@ -720,20 +729,33 @@ template <int FLUX, bool FS, bool VISC>
 static void launch_res_t(hipStream_t s, const DevMesh &m, const PhysDev &P,
                          const StageParams &sp)
 {
+    const int lanes = m.maxf <= 4 ? 4 : 8;
+    const int grid = cell_blocks((sp.c_end - sp.c_begin) * lanes);
     if (m.maxf <= 4)
         hipLaunchKernelGGL((fv_residual_group_kernel<FLUX, FS, VISC, 4>),
-                           dim3(residual_grid(m)), dim3(kBlock), 0, s, m, P, sp);
+                           dim3(grid), dim3(kBlock), 0, s, m, P, sp);
     else
         hipLaunchKernelGGL((fv_residual_group_kernel<FLUX, FS, VISC, 8>),
-                           dim3(residual_grid(m)), dim3(kBlock), 0, s, m, P, sp);
+                           dim3(grid), dim3(kBlock), 0, s, m, P, sp);
 }
 
 void launch_residual(hipStream_t s, const DevMesh &m, const PhysDev &P,
-                     const StageParams &sp, int flux, bool face_states,
+                     const StageParams &sp_in, int flux, bool face_states,
                      bool viscous)
 {
     check_faces(m);
-    if (sp.do_reduce && residual_grid(m) > sp.partials_cap)
+    StageParams sp = sp_in;
+    if (sp.c_end <= 0)
+    {
+        sp.c_begin = 0;
+        sp.c_end = m.N;
+    }
+    if (sp.c_begin < 0 || sp.c_end > m.N || sp.c_begin >= sp.c_end)
+        throw std::runtime_error("launch_residual: bad cell range");
+    if (sp.do_update && !sp.case_dt && !sp.sc)
+        throw std::runtime_error("launch_residual: stage update without a time step");
+    if (sp.do_reduce && cell_blocks((sp.c_end - sp.c_begin) * (m.maxf <= 4 ? 4 : 8)) >
+                            sp.partials_cap)
         throw std::runtime_error("reduction partials buffer too small");
     if (flux == FluxRusanov)
     {

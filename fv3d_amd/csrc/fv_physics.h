@@ -402,6 +402,66 @@ __device__ __forceinline__ void face_flux(const PhysDev &P, const Frame &fr,
     }
 }
 
+// Inverse of a 5x5 row-major matrix (Gauss-Jordan, partial pivoting; row
+// swaps with compile-time indices so everything stays in registers).
+__device__ __forceinline__ void invert5(const double *A, double *Ainv)
+{
+    double M[5][10];
+#pragma unroll
+    for (int r = 0; r < 5; ++r)
+    {
+#pragma unroll
+        for (int q = 0; q < 5; ++q)
+        {
+            M[r][q] = A[r * 5 + q];
+            M[r][5 + q] = r == q ? 1.0 : 0.0;
+        }
+    }
+#pragma unroll
+    for (int col = 0; col < 5; ++col)
+    {
+        int p = col;
+        double best = fabs(M[col][col]);
+#pragma unroll
+        for (int r = col + 1; r < 5; ++r)
+            if (fabs(M[r][col]) > best)
+            {
+                best = fabs(M[r][col]);
+                p = r;
+            }
+        // row swap with compile-time indices only (no private-memory arrays)
+#pragma unroll
+        for (int r = col + 1; r < 5; ++r)
+            if (r == p)
+#pragma unroll
+                for (int q = 0; q < 10; ++q)
+                {
+                    const double t = M[col][q];
+                    M[col][q] = M[r][q];
+                    M[r][q] = t;
+                }
+        const double inv = 1.0 / M[col][col];
+#pragma unroll
+        for (int q = 0; q < 10; ++q)
+            M[col][q] *= inv;
+#pragma unroll
+        for (int r = 0; r < 5; ++r)
+        {
+            if (r == col)
+                continue;
+            const double fct = M[r][col];
+#pragma unroll
+            for (int q = 0; q < 10; ++q)
+                M[r][q] -= fct * M[col][q];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 5; ++r)
+#pragma unroll
+        for (int q = 0; q < 5; ++q)
+            Ainv[r * 5 + q] = M[r][5 + q];
+}
+
 __device__ __forceinline__ void block_sum(double *sh, double &v)
 {
     const int t = threadIdx.x;

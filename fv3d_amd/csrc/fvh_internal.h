@@ -47,10 +47,11 @@ int generate_box(RawMesh &m, int nx, int ny, int nz, double x0, double y0,
 int generate_channel(RawMesh &m, int nx, int ny, int nz, double jitter,
                      unsigned long long seed);
 
-// Raw container I/O (mesh_io.cpp)
-int write_raw_mesh(const RawMesh &m, const char *path);
-int read_raw_mesh(RawMesh &m, const char *path);
-int read_hdf5_mesh(RawMesh &m, const char *path);
+// Mesh file I/O (mesh_io.cpp): HDF5 (.h5, the reference layout) or the raw
+// container, chosen by extension when writing and by signature when
+// reading.  Throw on error.
+int write_mesh_file(const RawMesh &m, const char *path);
+int read_mesh_file(RawMesh &m, const char *path);
 
 // Install a raw mesh as solver_mesh (reader layout) and run the metrics
 // (solver.cpp).

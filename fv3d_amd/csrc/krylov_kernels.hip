@@ -38,65 +38,6 @@ __global__ __launch_bounds__(kBlock) void k_local_dt(DevMesh m, PhysDev P,
     dtau[c] = cfl * (m.vol[c] / ds1);
 }
 
-// In-place inverse of a 5x5 row-major matrix (Gauss-Jordan, partial pivoting).
-__device__ void invert5(double *A, double *Ainv)
-{
-    double M[5][10];
-#pragma unroll
-    for (int r = 0; r < 5; ++r)
-    {
-#pragma unroll
-        for (int q = 0; q < 5; ++q)
-        {
-            M[r][q] = A[r * 5 + q];
-            M[r][5 + q] = r == q ? 1.0 : 0.0;
-        }
-    }
-#pragma unroll
-    for (int col = 0; col < 5; ++col)
-    {
-        int p = col;
-        double best = fabs(M[col][col]);
-#pragma unroll
-        for (int r = col + 1; r < 5; ++r)
-            if (fabs(M[r][col]) > best)
-            {
-                best = fabs(M[r][col]);
-                p = r;
-            }
-        // row swap with compile-time indices only (no private-memory arrays)
-#pragma unroll
-        for (int r = col + 1; r < 5; ++r)
-            if (r == p)
-#pragma unroll
-                for (int q = 0; q < 10; ++q)
-                {
-                    const double t = M[col][q];
-                    M[col][q] = M[r][q];
-                    M[r][q] = t;
-                }
-        const double inv = 1.0 / M[col][col];
-#pragma unroll
-        for (int q = 0; q < 10; ++q)
-            M[col][q] *= inv;
-#pragma unroll
-        for (int r = 0; r < 5; ++r)
-        {
-            if (r == col)
-                continue;
-            const double fct = M[r][col];
-#pragma unroll
-            for (int q = 0; q < 10; ++q)
-                M[r][q] -= fct * M[col][q];
-        }
-    }
-#pragma unroll
-    for (int r = 0; r < 5; ++r)
-#pragma unroll
-        for (int q = 0; q < 5; ++q)
-            Ainv[r * 5 + q] = M[r][5 + q];
-}
-
 // First-order inviscid residual of cell c for the own state u (neighbours
 // from U, ghosts from u): r = -(sum_f +-F_f A_f) / V.
 template <int FLUX, int MAXF>
@@ -196,7 +137,7 @@ __global__ __launch_bounds__(kBlock) void k_precond_blocks(
         for (int j = 0; j < 5; ++j)
             A[j * 5 + k] = (j == k ? idt : 0.0) - (r[j] - base[j]) / hk;
     }
-    invert5(A, out);
+    dev::invert5(A, out);
 }
 
 template <typename T>

@@ -112,7 +112,7 @@ void learned_blocks(const RuntimeView &v, double cfl, const double *dtau, double
     ensure_mesh(v.m.N);
     nn::features(v.stream, v.m, v.P, v.U, cfl, learned_refs(v), g_nn.cfeat);
     nn::forward(v.stream, v.m.N, g_nn.theta, g_nn.cfeat, g_nn.cout);
-    nn::apply_blocks(v.stream, v.m.N, g_nn.cout, dtau, learned_refs(v), Minv);
+    nn::apply_blocks(v.stream, v.m.N, g_nn.cout, dtau, cfl, learned_refs(v), Minv);
 }
 
 // max and mean per-cell relative Frobenius error of A against B
@@ -226,7 +226,7 @@ int fvh_nn_collect(double cfl, long long *n_total)
         kry::precond_blocks(v.stream, v.m, v.P, v.flux, v.U, g_nn.dtau, 1, g_nn.minv);
         nn::features(v.stream, v.m, v.P, v.U, cfl, learned_refs(v),
                      g_nn.feat + g_nn.n * nn::kIn);
-        nn::targets(v.stream, N, g_nn.minv, g_nn.dtau, learned_refs(v),
+        nn::targets(v.stream, N, g_nn.minv, g_nn.dtau, cfl, learned_refs(v),
                     g_nn.target + g_nn.n * nn::kOut);
         FVH_HIP_CHECK(hipStreamSynchronize(v.stream));
         g_nn.n += N;

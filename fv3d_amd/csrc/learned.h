@@ -36,7 +36,7 @@ struct Refs
     double rho, p, a;
 };
 
-// Features [N][kIn] of the current state U at the pseudo steps dtau (CFL cfl):
+// Features [N][kIn] of the current state U (cfl is not used):
 //  0      rho/rho_r - 1
 //  1..3   u/c, v/c, w/c (c the cell's sound speed)
 //  4      p/p_r - 1
@@ -45,24 +45,26 @@ struct Refs
 //         (ghost state on boundary faces); order q*3 + d
 //  20..25 sum_f A_f n_f n_f^T / V^(2/3): xx, yy, zz, xy, xz, yz
 //  26     boundary faces / 4
-//  27     log10(cfl) / 3
+//  27     |u| / c
 //  28..31 zero
 void features(hipStream_t s, const dev::DevMesh &m, const dev::PhysDev &P,
               const double *U, double cfl, Refs r, float *feat);
 
-// Targets [N][kOut]: S (Minv_c / dtau_c) S^-1 - I (25 entries row-major,
-// then zeros).
+// Targets [N][kOut]: S K_c S^-1 (25 entries row-major, then zeros), where
+// the block-Jacobi block is M_c = Minv_c^-1 = (I + cfl K_c) / dtau_c; K_c is
+// the dimensionless first-order diagonal Jacobian block -(V/ds1) J_c,
+// independent of the CFL number.
 void targets(hipStream_t s, int N, const double *Minv, const double *dtau,
-             Refs r, float *T);
+             double cfl, Refs r, float *T);
 
 // out [N][kOut] = MLP(feat) with the parameters theta (FP32 master values,
 // rounded to BF16 for the MFMA operands).
 void forward(hipStream_t s, int N, const float *theta, const float *feat,
              float *out);
 
-// Minv_c = dtau_c S^-1 (I + out_c) S (FP64 blocks, row-major).
+// Minv_c = dtau_c (I + cfl S^-1 out_c S)^-1 (FP64 blocks, row-major).
 void apply_blocks(hipStream_t s, int N, const float *out, const double *dtau,
-                  Refs r, double *Minv);
+                  double cfl, Refs r, double *Minv);
 
 // Loss sum_c sum_{j<25} (y_cj - t_cj)^2 / (n * 25) over samples [0, n) and
 // its gradient grad[kParams]; partials holds kTrainBlocks * (kParams + 1)

@@ -225,7 +225,7 @@ __global__ __launch_bounds__(kB) void k_features(DevMesh m, PhysDev P,
     for (int i = 0; i < 6; ++i)
         o[20 + i] = (float)(mom[i] / h2);
     o[26] = (float)(nb_bnd * 0.25);
-    o[27] = (float)(log10(cfl) / 3.0);
+    o[27] = (float)(sqrt(pc[5] * pc[5] + pc[6] * pc[6] + pc[7] * pc[7]) / cs);
     for (int i = kFeat; i < kIn; ++i)
         o[i] = 0.f;
 }
@@ -237,29 +237,37 @@ __device__ __forceinline__ void scales(const Refs &r, double *s)
     s[4] = 1.0 / (r.rho * r.a * r.a);
 }
 
+// The block M_c = I/dtau_c - J_c (J_c the first-order diagonal Jacobian
+// block) written as M_c = (I + cfl K_c) / dtau_c: K_c = -(V_c/ds1_c) J_c is
+// dimensionless and independent of the CFL number.  The network predicts
+// S K_c S^-1.
 __global__ __launch_bounds__(kB) void k_targets(int N, const double *__restrict__ Minv,
-                                                const double *__restrict__ dtau, Refs r,
-                                                float *__restrict__ T)
+                                                const double *__restrict__ dtau, double cfl,
+                                                Refs r, float *__restrict__ T)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= N)
         return;
     double s[5];
     scales(r, s);
-    const double idt = 1.0 / dtau[c];
+    double M[25];
+    dev::invert5(Minv + (size_t)c * 25, M);
+    const double dt = dtau[c];
     float *o = T + (size_t)c * kOut;
     for (int i = 0; i < 5; ++i)
         for (int j = 0; j < 5; ++j)
         {
-            const double t = s[i] * Minv[(size_t)c * 25 + i * 5 + j] * idt / s[j];
-            o[i * 5 + j] = (float)(t - (i == j ? 1.0 : 0.0));
+            const double k = (dt * M[i * 5 + j] - (i == j ? 1.0 : 0.0)) / cfl;
+            o[i * 5 + j] = (float)(s[i] * k / s[j]);
         }
     for (int i = kBlk; i < kOut; ++i)
         o[i] = 0.f;
 }
 
+// Minv_c = dtau_c (I + cfl S^-1 out_c S)^-1
 __global__ __launch_bounds__(kB) void k_apply_blocks(int N, const float *__restrict__ out,
-                                                     const double *__restrict__ dtau, Refs r,
+                                                     const double *__restrict__ dtau,
+                                                     double cfl, Refs r,
                                                      double *__restrict__ Minv)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -267,14 +275,15 @@ __global__ __launch_bounds__(kB) void k_apply_blocks(int N, const float *__restr
         return;
     double s[5];
     scales(r, s);
-    const double dt = dtau[c];
     const float *y = out + (size_t)c * kOut;
+    double A[25], Ai[25];
     for (int i = 0; i < 5; ++i)
         for (int j = 0; j < 5; ++j)
-        {
-            const double t = (double)y[i * 5 + j] + (i == j ? 1.0 : 0.0);
-            Minv[(size_t)c * 25 + i * 5 + j] = dt * t * s[j] / s[i];
-        }
+            A[i * 5 + j] = (i == j ? 1.0 : 0.0) + cfl * ((double)y[i * 5 + j] * s[j] / s[i]);
+    dev::invert5(A, Ai);
+    const double dt = dtau[c];
+    for (int i = 0; i < 25; ++i)
+        Minv[(size_t)c * 25 + i] = dt * Ai[i];
 }
 
 // ---- inference -------------------------------------------------------------
@@ -531,9 +540,11 @@ void features(hipStream_t s, const DevMesh &m, const PhysDev &P, const double *U
                        feat);
 }
 
-void targets(hipStream_t s, int N, const double *Minv, const double *dtau, Refs r, float *T)
+void targets(hipStream_t s, int N, const double *Minv, const double *dtau, double cfl, Refs r,
+             float *T)
 {
-    hipLaunchKernelGGL(k_targets, dim3(blocks_for(N)), dim3(kB), 0, s, N, Minv, dtau, r, T);
+    hipLaunchKernelGGL(k_targets, dim3(blocks_for(N)), dim3(kB), 0, s, N, Minv, dtau, cfl, r,
+                       T);
 }
 
 void forward(hipStream_t s, int N, const float *theta, const float *feat, float *out)
@@ -547,11 +558,11 @@ void forward(hipStream_t s, int N, const float *theta, const float *feat, float 
                        out);
 }
 
-void apply_blocks(hipStream_t s, int N, const float *out, const double *dtau, Refs r,
-                  double *Minv)
+void apply_blocks(hipStream_t s, int N, const float *out, const double *dtau, double cfl,
+                  Refs r, double *Minv)
 {
-    hipLaunchKernelGGL(k_apply_blocks, dim3(blocks_for(N)), dim3(kB), 0, s, N, out, dtau, r,
-                       Minv);
+    hipLaunchKernelGGL(k_apply_blocks, dim3(blocks_for(N)), dim3(kB), 0, s, N, out, dtau, cfl,
+                       r, Minv);
 }
 
 void loss_grad(hipStream_t s, int n, const float *theta, const float *feat,

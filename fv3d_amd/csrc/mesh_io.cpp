@@ -194,9 +194,35 @@ std::vector<std::string> RawReader::strings(const std::string &name) const
     return out;
 }
 
-int write_raw_mesh(const RawMesh &m, const char *path)
+std::unique_ptr<DataWriter> open_writer(const char *path)
 {
-    RawWriter w(path);
+    const std::string p(path);
+    if (p.size() > 3 && p.compare(p.size() - 3, 3, ".h5") == 0)
+        return hdf5_writer(path);
+    return std::unique_ptr<DataWriter>(new RawWriter(path));
+}
+
+std::unique_ptr<DataReader> open_reader(const char *path)
+{
+    FILE *fp = std::fopen(path, "rb");
+    if (!fp)
+        throw std::runtime_error(std::string("cannot open ") + path);
+    unsigned char sig[8] = {0};
+    const size_t got = std::fread(sig, 1, 8, fp);
+    std::fclose(fp);
+    static const unsigned char h5sig[8] = {0x89, 'H', 'D', 'F', '\r', '\n', 0x1a, '\n'};
+    if (got == 8 && std::memcmp(sig, h5sig, 8) == 0)
+        return hdf5_reader(path);
+    std::unique_ptr<RawReader> r(new RawReader());
+    if (!r->open(path))
+        throw std::runtime_error(std::string("not an HDF5 file or raw container: ") + path);
+    return std::unique_ptr<DataReader>(r.release());
+}
+
+int write_mesh_file(const RawMesh &m, const char *path)
+{
+    std::unique_ptr<DataWriter> wp = open_writer(path);
+    DataWriter &w = *wp;
     w.attr_int("dimension", m.dimension);
     w.attr_int("is_partitioned", 0);
     w.attr_int("VERTICES/n_vertices", (int)(m.vertices.size() / 3));
@@ -244,11 +270,10 @@ int write_raw_mesh(const RawMesh &m, const char *path)
     return 0;
 }
 
-int read_raw_mesh(RawMesh &m, const char *path)
+int read_mesh_file(RawMesh &m, const char *path)
 {
-    RawReader r;
-    if (!r.open(path))
-        return -1;
+    std::unique_ptr<DataReader> rp = open_reader(path);
+    const DataReader &r = *rp;
     m = RawMesh();
     m.dimension = r.ints("dimension")[0];
     m.vertices = r.doubles("VERTICES/x");

@@ -40,6 +40,19 @@ struct RuntimeView
 // reconstruction and limiter are set.
 RuntimeView runtime_view();
 
+// The explicit stepping buffers: the ping-pong state pair (U[*cur] is
+// current), the RK register k, the residual R and the LSERK tables of the
+// scheme set by init_explicit.
+struct StageBuffers
+{
+    double *U[2];
+    int *cur;
+    double *k, *R;
+    int stages;
+    const double *rk_a, *rk_g;
+};
+StageBuffers stage_buffers();
+
 // Enqueues R = dU/dt(U) (finite_volume_time_derivative semantics, the
 // reference's ghost rows recomputed from U) on the runtime stream.
 void eval_residual(const double *U, double *R);
@@ -53,6 +66,7 @@ void state_changed();
 // state is freed).
 void jfnk_release();
 void learned_release_mesh();
+void batch_release();
 
 // Learned preconditioner (learned.cpp): whether a network is loaded, its
 // inverse blocks for the current state at the pseudo steps dtau, and the

@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -111,6 +112,7 @@ int g_ic_kind = 2;
 double g_ic_params[8] = {0};
 
 int g_verbose = 1;
+int g_output_format = 0; // 0: HDF5 when libhdf5 loads, else raw; 1: raw; 2: HDF5
 int g_sync_interval = 16;
 int g_last_status = 0;
 int g_last_iterations = 0;
@@ -199,6 +201,7 @@ void free_device()
         (void)hipStreamSynchronize(g_dev.stream);
     jfnk_release();
     learned_release_mesh();
+    batch_release();
     for (void *p : g_dev.allocs)
         (void)hipFree(p);
     if (g_dev.sc_host)
@@ -955,6 +958,23 @@ void eval_residual(const double *U, double *R)
 
 void state_changed() { g_dev.r_valid = false; }
 
+StageBuffers stage_buffers()
+{
+    require_ready();
+    if (g_rk_stages <= 0)
+        throw std::runtime_error("explicit scheme not initialised");
+    StageBuffers b{};
+    b.U[0] = g_dev.U[0];
+    b.U[1] = g_dev.U[1];
+    b.cur = &g_dev.cur;
+    b.k = g_dev.k;
+    b.R = g_dev.R;
+    b.stages = g_rk_stages;
+    b.rk_a = g_rk_a;
+    b.rk_g = g_rk_g;
+    return b;
+}
+
 } // namespace fvh
 
 using namespace fvh;
@@ -1053,13 +1073,7 @@ void init_mesh(cstring_t mesh_file)
     {
         RawMesh m;
         std::string p = mesh_file ? mesh_file : "";
-        int rc;
-        if (p.size() > 3 && p.substr(p.size() - 3) == ".h5")
-            rc = read_hdf5_mesh(m, p.c_str());
-        else
-            rc = read_raw_mesh(m, p.c_str());
-        if (rc != 0)
-            fatal("cannot read mesh file " + p + ": " + g_error);
+        read_mesh_file(m, p.c_str());
         install_mesh(std::move(m));
     }
     catch (const std::exception &e)
@@ -1412,20 +1426,40 @@ void init_output(int i_output_data)
 }
 void free_output(void) {}
 
+namespace
+{
+bool output_hdf5()
+{
+    if (g_output_format == 1)
+        return false;
+    std::string why;
+    const bool ok = hdf5_available(&why);
+    if (g_output_format == 2 && !ok)
+        throw std::runtime_error("HDF5 output requested: " + why);
+    return ok;
+}
+
+// output.c:20-67 (the header part of every output file)
+void write_header(DataWriter &w)
+{
+    w.attr_int("n_tot_variables", solver_variables->n_tot_variables);
+    w.attr_int("n_sol_variables", solver_variables->n_sol_variables);
+    std::vector<std::string> tot, sol;
+    for (int i = 0; i < solver_variables->n_tot_variables; ++i)
+        tot.push_back(solver_variables->tot_variables[i]->name);
+    for (int i = 0; i < solver_variables->n_sol_variables; ++i)
+        sol.push_back(solver_variables->sol_variables[i].name);
+    w.strings("tot_variables", tot);
+    w.strings("sol_variables", sol);
+}
+} // namespace
+
 void create_file_header(cstring_t file_name)
 {
     try
     {
-        RawWriter w(file_name);
-        w.attr_int("n_tot_variables", solver_variables->n_tot_variables);
-        w.attr_int("n_sol_variables", solver_variables->n_sol_variables);
-        std::vector<std::string> tot, sol;
-        for (int i = 0; i < solver_variables->n_tot_variables; ++i)
-            tot.push_back(solver_variables->tot_variables[i]->name);
-        for (int i = 0; i < solver_variables->n_sol_variables; ++i)
-            sol.push_back(solver_variables->sol_variables[i].name);
-        w.strings("tot_variables", tot);
-        w.strings("sol_variables", sol);
+        std::unique_ptr<DataWriter> w = open_writer(file_name);
+        write_header(*w);
     }
     catch (const std::exception &e)
     {
@@ -1433,31 +1467,39 @@ void create_file_header(cstring_t file_name)
     }
 }
 
-// output.c:92-191: <title>.%09d.fvs with attrs iter, t and datasets
-// phi_total [N x 10], phi_dt [N x 5]
+// output.c:92-191: <title>.%09d.h5 (raw container: .fvs) with the header,
+// attrs iter, t and datasets phi_total [N x 10], phi_dt [N x 5]; in implicit
+// mode also n_stages and the BDF history phi_old:<i> [N x 5]
 void write_output(int iter, double t)
 {
     try
     {
         sync_mirror(true);
         char suffix[64];
-        std::snprintf(suffix, sizeof(suffix), ".%09d.fvs", iter);
+        const bool h5 = output_hdf5();
+        std::snprintf(suffix, sizeof(suffix), h5 ? ".%09d.h5" : ".%09d.fvs", iter);
         std::string path = g_title + suffix;
-        RawWriter w(path.c_str());
-        w.attr_int("n_tot_variables", solver_variables->n_tot_variables);
-        w.attr_int("n_sol_variables", solver_variables->n_sol_variables);
-        std::vector<std::string> tot, sol;
-        for (int i = 0; i < solver_variables->n_tot_variables; ++i)
-            tot.push_back(solver_variables->tot_variables[i]->name);
-        for (int i = 0; i < solver_variables->n_sol_variables; ++i)
-            sol.push_back(solver_variables->sol_variables[i].name);
-        w.strings("tot_variables", tot);
-        w.strings("sol_variables", sol);
+        std::unique_ptr<DataWriter> wp = h5 ? hdf5_writer(path.c_str())
+                                            : std::unique_ptr<DataWriter>(new RawWriter(path.c_str()));
+        DataWriter &w = *wp;
+        write_header(w);
         w.attr_int("iter", iter);
         w.attr_double("t", t);
         const int N = solver_mesh->cells->n_domain_cells;
         w.doubles("phi_total", solver_data->phi_total, (size_t)N * 10, 10);
         w.doubles("phi_dt", solver_data->phi_dt, (size_t)N * 5, 5);
+        if (!g_is_explicit && g_dev.imp_old[0])
+        {
+            const int n_stages = g_bdf_scheme == BDF2 ? 2 : 1;
+            w.attr_int("n_stages", n_stages);
+            std::vector<double> h((size_t)N * 5);
+            for (int i = 0; i < n_stages; ++i)
+            {
+                HIP_CHECK(hipMemcpy(h.data(), g_dev.imp_old[i], h.size() * 8,
+                                    hipMemcpyDeviceToHost));
+                w.doubles("phi_old:" + std::to_string(i), h.data(), h.size(), 5);
+            }
+        }
     }
     catch (const std::exception &e)
     {
@@ -1479,9 +1521,8 @@ void read_restart_data(cstring_t restart_file)
 {
     try
     {
-        RawReader r;
-        if (!r.open(restart_file))
-            fatal(std::string("cannot read restart file ") + restart_file);
+        std::unique_ptr<DataReader> rp = open_reader(restart_file);
+        const DataReader &r = *rp;
         g_iter_restart = r.ints("iter")[0];
         g_t_restart = r.doubles("t")[0];
         std::vector<double> phi = r.doubles("phi_total");
@@ -1735,7 +1776,7 @@ int fvh_mesh_write(cstring_t path)
         for (int i = 0; i < mm->regions->n_regions; ++i)
             m.region_names.push_back(mm->regions->name[i]);
         m.region_is_boundary = vi(mm->regions->is_boundary, mm->regions->n_regions);
-        if (write_raw_mesh(m, path) != 0)
+        if (write_mesh_file(m, path) != 0)
             throw std::runtime_error("cannot write mesh");
     });
     return FVH_OK;
@@ -1847,6 +1888,19 @@ int fvh_sync_host(void)
     FVH_TRY({ sync_mirror(true); });
     return FVH_OK;
 }
+
+int fvh_set_output_format(int format)
+{
+    if (format < 0 || format > 2)
+    {
+        set_error("fvh_set_output_format: 0 auto, 1 raw, 2 HDF5");
+        return FVH_ERR_ARG;
+    }
+    g_output_format = format;
+    return FVH_OK;
+}
+
+int fvh_hdf5_available(void) { return hdf5_available() ? 1 : 0; }
 
 int fvh_set_verbose(int verbose)
 {

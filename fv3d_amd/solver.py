@@ -194,6 +194,10 @@ class Case:
             n = p["n"]
             check(L.fvh_mesh_channel(n[0], n[1], n[2], p.get("jitter", 0.05),
                                      p.get("seed", 20261203)))
+        elif kind == "batch":
+            p = mesh[1]
+            nh = (ctypes.c_int * len(p["n_hex"]))(*p["n_hex"])
+            check(L.fvh_mesh_batch(len(p["n_hex"]), nh, p.get("length", 1.0), p.get("tile", 0)))
         elif kind == "file":
             L.init_mesh(mesh[1].encode())
         else:

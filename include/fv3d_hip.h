@@ -47,10 +47,17 @@ int fvh_mesh_box(int nx, int ny, int nz, double x0, double y0, double z0,
 int fvh_mesh_channel(int nx, int ny, int nz, double jitter,
                      unsigned long long seed);
 
-/* Write / read the mesh in a raw little-endian container (.fvm) with the
- * same groups and datasets as the reference HDF5 mesh file.  init_mesh()
- * accepts .fvm files and, when libhdf5 can be loaded, .h5 files. */
+/* Write the mesh in the reference's HDF5 layout (path ending in .h5; libhdf5
+ * 1.10 is loaded at run time) or in a raw little-endian container with the
+ * same groups and datasets (any other name).  init_mesh() reads either,
+ * recognised by the file signature.  Both add FACES/periodic_shift [F*3]. */
 int fvh_mesh_write(cstring_t path);
+/* 1 when libhdf5 can be loaded (FV3D_HDF5_LIB, the loader path or the conda
+ * copy), else 0. */
+int fvh_hdf5_available(void);
+/* Format of write_output: 0 HDF5 when available, else raw (default);
+ * 1 raw container (<title>.%09d.fvs); 2 HDF5 (<title>.%09d.h5). */
+int fvh_set_output_format(int format);
 /* Periodic translation of each face's cells[1] image, [F*3] (zeros on
  * non-periodic faces).  Faces_t has no field for it. */
 int fvh_get_face_shift(double *shift);
@@ -178,12 +185,36 @@ int fvh_jfnk_history(double *rows, int max_rows);
  * FVH_PRECOND_BLOCK) or of I/dtau_c (FVH_PRECOND_SCALAR). */
 int fvh_jfnk_blocks(double cfl, int precond, double *dtau, double *minv);
 
+/* ---- batched cases ------------------------------------------------------ */
+/* Many independent cases in one device state and one launch per kernel
+ * and stage: n_cases periodic boxes [0,length]^3 of n_hex[k]^3 hexahedra,
+ * each Kuhn-split into 6 tetrahedra, stored as one mesh with the cases'
+ * cells in order (case k owns cells offsets[k] .. offsets[k+1]).  Builds
+ * solver_mesh like fvh_mesh_box; the usual init sequence follows
+ * (navier_stokes_initialize ... init_finite_volume, first-order
+ * reconstruction, init_explicit), then fvh_batch_setup. */
+int fvh_mesh_batch(int n_cases, const int *n_hex, double length, int tile);
+/* Per-case flux (0 AUSM, 1 Rusanov), viscous flag and viscosity; cases
+ * with equal (flux, viscous, mu) must be adjacent (they form one launch
+ * group).  Computes every case's first time step from the current state. */
+int fvh_batch_setup(int n_cases, const int *flux, const int *viscous,
+                    const double *mu);
+/* Advances every case by `steps` steps of the explicit scheme, each case
+ * with its own time step (cfl_scale, dfl_scale of the globals); use_graph
+ * replays a captured HIP graph of one step.  seconds: device time. */
+int fvh_batch_run(int steps, int use_graph, double *seconds);
+int fvh_batch_info(int *n_cases, int *offsets, int max_offsets);
+/* Per case: next dt [K], time [K], residual norms of the last step [K*5]
+ * (analyze.c norms with the case's volume and dt). */
+int fvh_batch_case_scalars(double *dt, double *t, double *res);
+
 /* ---- learned block preconditioner (network of FVH_PRECOND_LEARNED) ---- */
 /* A per-cell MLP 28 -> 64 -> 64 -> 25 (tanh) maps local features of the
  * state (normalised primitives, their scaled Green-Gauss gradients, face
- * geometry moments, boundary face count, log CFL) to the 5x5 inverse
- * block-Jacobi block, normalised by the flow-region state and the pseudo
- * step.  BF16 MFMA kernels with FP32 accumulation run inference and
+ * geometry moments, boundary face count, Mach number) to the dimensionless
+ * first-order diagonal Jacobian block K_c = -(V_c/ds1_c) dR1_c/dU_c
+ * (normalised by the flow-region state), from which the block-Jacobi block
+ * (I + cfl K_c)/dtau_c is inverted exactly for any pseudo CFL.  BF16 MFMA kernels with FP32 accumulation run inference and
  * training; master weights and Adam are FP32.  The parameters are one FP32
  * vector of fvh_nn_param_count() values (W1[32][64], b1[64], W2[64][64],
  * b2[64], W3[64][32], b3[32], row-major [in][out]; input rows 28..31 and

@@ -87,14 +87,22 @@ def scales(rho, p, kappa):
                      1.0 / (rho * a * a)])
 
 
-def targets(Minv, dtau, rho, p, kappa):
-    """S (Minv_c / dtau_c) S^-1 - I, [N, 32] (25 used)."""
+def targets(Minv, dtau, cfl, rho, p, kappa):
+    """S K S^-1 with K = (dtau Minv^-1 - I) / cfl, [N, 32] (25 used)."""
     s = scales(rho, p, kappa)
-    T = s[None, :, None] * np.asarray(Minv).reshape(-1, 5, 5) / np.asarray(dtau)[:, None, None] \
-        / s[None, None, :] - np.eye(5)[None]
+    M = np.linalg.inv(np.asarray(Minv).reshape(-1, 5, 5))
+    K = (np.asarray(dtau)[:, None, None] * M - np.eye(5)[None]) / cfl
+    T = s[None, :, None] * K / s[None, None, :]
     out = np.zeros((T.shape[0], K_OUT))
     out[:, :K_BLK] = T.reshape(-1, 25)
     return out
+
+
+def blocks(out, dtau, cfl, rho, p, kappa):
+    """Minv = dtau (I + cfl S^-1 out S)^-1, [N, 5, 5]."""
+    s = scales(rho, p, kappa)
+    K = np.asarray(out)[:, :K_BLK].reshape(-1, 5, 5) * s[None, None, :] / s[None, :, None]
+    return np.asarray(dtau)[:, None, None] * np.linalg.inv(np.eye(5)[None] + cfl * K)
 
 
 def features(mesh, phi_total, cfl, rho_r, p_r, kappa):
@@ -131,5 +139,5 @@ def features(mesh, phi_total, cfl, rho_r, p_r, kappa):
     X[:, 5:20] = (g * (h / V)[:, None, None]).reshape(N, 15)
     X[:, 20:26] = mom / (h * h)[:, None]
     X[:, 26] = nbnd * 0.25
-    X[:, 27] = math.log10(cfl) / 3.0
+    X[:, 27] = np.sqrt((phi[:N, 5:8] ** 2).sum(1)) / c
     return X

@@ -4,8 +4,9 @@ solve; src/Solver/src/timedisc/implicit.c) against the oracle, through the
 C-ABI.
 
 Bar: 1e-12 relative per component (as the explicit path) for the state
-after N steps, the time and dt columns of the log bitwise, the residual
-columns to 1e-12.  The finite-difference scale eps = 1e-4 |Y_k|_2 is a sum
+after N steps on the periodic density wave, dt and time to 1e-13, the
+residual columns to 1e-11; on the walled channel a bar set by the
+algorithm's own conditioning (see that test).  The finite-difference scale eps = 1e-4 |Y_k|_2 is a sum
 over the cells, which the device adds in a different order: eps, and with
 it the Jacobian, differ in the last bits.
 """
@@ -46,14 +47,27 @@ def test_density_wave_implicit(scheme):
 
 
 def test_channel_walls_implicit_steady():
-    cfg = configs.bump_channel(nx=12, ny=4, nz=2, flux="AUSM")
+    """Slip walls, inflow/outflow, linear reconstruction.  The reference's
+    lumped Jacobian differences R(Y + eps/2) - R(Y - eps/2) of whole-field
+    shifts, whose terms nearly cancel, so the step amplifies rounding: the
+    oracle itself moves by `sens` under a 1-ulp change of the initial state
+    (about 6e-11 here; 2.5e-5 with the Barth-Jespersen limiter, whose
+    switches the shifts cross).  The bar is 20x the oracle's own
+    sensitivity, measured in the test, and at least 1e-12."""
+    cfg = configs.bump_channel(nx=12, ny=4, nz=2, flux="AUSM", limiter="None")
     cfg = _implicit(cfg, "Euler", 3, transient=False)
     case, o, _ = make_pair(cfg)
+    U0 = case.state()
     rc, log_g = case.run()
     n, log_o = o.timedisc_implicit("Euler", 3, False, 1e-10, 0.0, 1e300)
     assert rc == 0 and n == 3
-    e = rel_err(case.state(), o.state())
-    _print("implicit channel", rel=e)
-    assert e <= TOL
-    assert np.allclose(log_g[:, 3:8], log_o[:, 3:8], rtol=1e-11, atol=0)
+    Uo = o.state()
+    rng = np.random.default_rng(0)
+    o.set_state((U0 * (1 + 2.0 ** -52 * rng.choice([-1, 0, 1], U0.shape))).reshape(-1))
+    o.timedisc_implicit("Euler", 3, False, 1e-10, 0.0, 1e300)
+    sens = rel_err(o.state(), Uo)
+    e = rel_err(case.state(), Uo)
+    _print("implicit channel", rel=e, oracle_ulp_sensitivity=sens)
+    assert e <= max(TOL, 20 * sens)
+    assert np.allclose(log_g[:, 3:8], log_o[:, 3:8], rtol=max(1e-11, 20 * sens), atol=0)
     case.close()

@@ -74,12 +74,23 @@ def test_uniform_box_features_closed_form():
     assert np.allclose(X[:, 20:23], 2.0, rtol=1e-12)
     assert np.abs(X[:, 23:26]).max() < 1e-12
     assert np.all(X[:, 26] == 0.0)
-    assert np.allclose(X[:, 27], math.log10(50.0) / 3.0)
+    assert np.allclose(X[:, 27], math.sqrt(u * u + v * v + w * w) / c)
     assert np.all(X[:, 28:] == 0.0)
 
 
 def test_scalar_blocks_give_zero_targets():
     dtau = np.array([0.1, 2.0, 3e-3])
     Minv = np.eye(5)[None] * dtau[:, None, None]
-    T = mlp.targets(Minv.reshape(-1), dtau, 1.2, 0.7, 1.4)
+    T = mlp.targets(Minv.reshape(-1), dtau, 30.0, 1.2, 0.7, 1.4)
     assert np.abs(T).max() < 1e-15
+
+
+def test_targets_and_blocks_are_inverse():
+    rng = np.random.default_rng(8)
+    n, cfl = 6, 75.0
+    dtau = rng.uniform(0.1, 2.0, n)
+    K = rng.standard_normal((n, 5, 5)) * 0.3 + np.eye(5)[None]
+    Minv = dtau[:, None, None] * np.linalg.inv(np.eye(5)[None] + cfl * K)
+    T = mlp.targets(Minv.reshape(-1), dtau, cfl, 1.3, 0.9, 1.4)
+    back = mlp.blocks(T, dtau, cfl, 1.3, 0.9, 1.4)
+    assert np.allclose(back, Minv, rtol=1e-10, atol=1e-14)
