@@ -1,17 +1,26 @@
-"""Benchmark of the explicit RK path on one MI355X.
+"""Benchmark of the residual and time-stepping path on one MI355X.
 
-Workload: Navier-Stokes Taylor-Green vortex, Re 1600, Ma 0.1, [0,2pi]^3
-periodic, 96^3 hexahedra Kuhn-split into 6 tetrahedra (5,308,416 cells),
-FP64, AUSM + Green-Gauss + Barth-Jespersen, RK3-3.  A "step" is one RK3-3
-time step (3 stages) over the whole mesh with the state resident in HBM.
+Headline workload: Navier-Stokes Taylor-Green vortex, Re 1600, Ma 0.1,
+[0,2pi]^3 periodic, 96^3 hexahedra Kuhn-split into 6 tetrahedra (5,308,416
+cells), FP64, AUSM + Green-Gauss + Barth-Jespersen, RK3-3.  A "step" is one
+RK3-3 time step (3 stages) over the whole mesh with the state resident in
+HBM.  `value` is cell-stage updates per second of that step loop.
 
-Prints one JSON line: metric "cell-stage updates/s", the roofline of the
-dominant kernel (algorithmic bytes per launch over its HIP-event-timed
-average duration) and the CPU baseline (the oracle, one core, on a bounded
-sample of the same workload).
+Beside it, in the same JSON line:
+- `jfnk`: the steady bump channel (160x40x13 hexahedra x 6 tetrahedra =
+  499,200 cells, Ma 0.5): wall time to the 1e-8 stop norm of plain FP64
+  block-Jacobi JFNK and of the learned-preconditioner, FP32-Krylov JFNK, the
+  speedup and the relative difference of the two converged solutions;
+- `batched`: 64 periodic cases of log-uniform size (seeded), one device
+  state, graph-replayed steps: aggregate cell updates per second and the
+  per-case step time (each case alone) p50/p99;
+- `roofline` of the dominant kernel (algorithmic bytes per launch over its
+  HIP-event-timed average duration) and `cpu_baseline` (the oracle, one
+  core, on a bounded sample of the same workload).
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -33,6 +42,8 @@ B_GRAD = 40 + 96 + 20 + 8 + 240 + 160
 B_STAGE = 80 + 80 + 224 + 24 + 8 + 24 + 20 + 240 + 160
 # Whole-stage compulsory traffic (each array element once per stage).
 B_CELL_STAGE = 1276
+# Tetrahedra, first order, inviscid (batched Euler cases)
+B_CELL_STAGE_FO = 388
 
 
 def main():
@@ -46,6 +57,11 @@ def main():
     ap.add_argument("--cpu-sample-n", type=int, default=64)
     ap.add_argument("--cpu-sample-steps", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-jfnk", action="store_true")
+    ap.add_argument("--jfnk-mesh", default="160,40,13")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-cases", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (torch) whose HBM "
                          "byte counters calibrate FETCH_SIZE/WRITE_SIZE")
@@ -74,7 +90,7 @@ def main():
     ach_stage = B_STAGE * N / k_stage_s / 1e9
     ach_grad = B_GRAD * N / k_grad_s / 1e9
     step_gbs = B_CELL_STAGE * N * stages * args.steps / sec / 1e9
-    roofline = {"bound": "hbm", "kernel": "fv_residual_kernel<AUSM,linear,viscous>",
+    roofline = {"bound": "hbm", "kernel": "fv_residual_group_kernel<AUSM,face states,viscous,4>",
                 "achieved": round(ach_stage, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(ach_stage / PEAK_HBM_GBS, 4), "traffic": None,
                 "bytes_per_cell": B_STAGE, "avg_us": round(k_us[1], 2)}
@@ -83,6 +99,8 @@ def main():
                                  "frac": round(ach_grad / PEAK_HBM_GBS, 4)},
              "step": {"achieved": round(step_gbs, 1), "bytes_per_cell_stage": B_CELL_STAGE,
                       "frac": round(step_gbs / PEAK_HBM_GBS, 4)}}
+    print("explicit: %.3f ms/step, %.3e cell-stage/s" % (ms_per_step, value), file=sys.stderr,
+          flush=True)
     case.close()
 
     if args.copy_calibration:
@@ -103,12 +121,17 @@ def main():
               file=sys.stderr, flush=True)
         del src, dst
 
+    jfnk = None if args.no_jfnk else bench_jfnk([int(x) for x in args.jfnk_mesh.split(",")])
+    batched = None if args.no_batch else bench_batch(args.batch_cases, args.batch_steps)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.cpu_sample_n, args.cpu_sample_steps)
+        if jfnk is not None:
+            cpu["jfnk"] = cpu_baseline_jfnk()
 
     line = {
-        "metric": "cell-stage updates/s (FP64 RK)",
+        "metric": "cell-stage updates/s (FP64 RK) + JFNK time-to-1e-8 residual, learned vs plain",
         "value": value,
         "unit": "cell-stage/s",
         "n_gpus": 1,
@@ -125,15 +148,120 @@ def main():
                    "cells": N, "stages_per_step": stages},
         "roofline": roofline,
         "roofline_detail": extra,
+        "jfnk": jfnk,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
 
 
+def bench_jfnk(n):
+    """Plain FP64 block-Jacobi JFNK against the learned-preconditioner,
+    FP32-Krylov JFNK on the bump channel, both from the free stream to the
+    1e-8 stop norm."""
+    import numpy as np
+    from fv3d_amd import configs, learned
+    from fv3d_amd import solver as S
+
+    cfg = configs.bump_channel_steady(nx=n[0], ny=n[1], nz=n[2])
+    case = S.Case(**cfg)
+    U0 = case.state()
+    out = {"workload": "bump channel %dx%dx%d hex x 6 tets (%d cells), Ma 0.5, AUSM, linear "
+                       "reconstruction, slip walls, state outlet; stop norm 1e-8"
+                       % (n[0], n[1], n[2], case.n_cells), "cells": case.n_cells}
+    st, _ = case.jfnk(tol=1e-8, precision="fp64", precond="block")
+    U64 = case.state()
+    out["plain_fp64_block"] = _jfnk_row(st)
+    print("jfnk plain: %.2f s, %d newton, %d gmres" % (st["seconds"], st["newton_steps"],
+                                                        st["gmres_iterations"]),
+          file=sys.stderr, flush=True)
+    variants = [("fp32_block", dict(precision="fp32", precond="block"))]
+    have_net = os.path.exists(learned.PRETRAINED)
+    if have_net:
+        learned.load_pretrained()
+        variants.append(("learned_fp32", dict(precision="fp32", precond="learned")))
+    else:
+        out["learned_fp32"] = None
+        out["note"] = "no pretrained network at %s" % learned.PRETRAINED
+    for name, kw in variants:
+        case.set_state(U0)
+        st, _ = case.jfnk(tol=1e-8, **kw)
+        U = case.state()
+        row = _jfnk_row(st)
+        row["solution_rel_diff_vs_plain"] = float(np.linalg.norm(U - U64) / np.linalg.norm(U64))
+        row["speedup_vs_plain"] = out["plain_fp64_block"]["seconds"] / st["seconds"]
+        out[name] = row
+        print("jfnk %s: %.2f s, %d newton, %d gmres, fallback %d" % (
+            name, st["seconds"], st["newton_steps"], st["gmres_iterations"],
+            st["precond_fallbacks"]), file=sys.stderr, flush=True)
+    case.close()
+    return out
+
+
+def _jfnk_row(st):
+    return {"seconds": st["seconds"], "status": st["status"], "newton": st["newton_steps"],
+            "gmres": st["gmres_iterations"], "residual_evals": st["residual_evals"],
+            "norm": st["norm"], "l2_drop": st["l2_drop"],
+            "precond_seconds": st["precond_seconds"],
+            "precond_fallbacks": st["precond_fallbacks"]}
+
+
+def bench_batch(n_cases, steps):
+    import numpy as np
+    from fv3d_amd import batch as B
+
+    cases = B.c5_cases(n_cases)
+    t0 = time.time()
+    b = B.Batch(cases, tile=8)
+    setup = time.time() - t0
+    b.run(3, graph=False)
+    b.run(2, graph=True)
+    sec = b.run(steps, graph=True)
+    sec_eager = b.run(steps, graph=False)
+    cells = b.n_cells
+    ms = sec / steps * 1e3
+    out = {"workload": "%d periodic cases, 2e4-4e5 tets log-uniform (seeds 0..%d), Mach 0.2-2.0, "
+                       "Rusanov/AUSM by seed parity, Navier-Stokes every 4th, first order, RK3-3"
+                       % (n_cases, n_cases - 1),
+           "cells": cells, "cases": n_cases, "setup_s": round(setup, 1),
+           "ms_per_step_graph": ms, "ms_per_step_eager": sec_eager / steps * 1e3,
+           "cell_updates_per_s": cells * steps / sec,
+           "cell_stage_updates_per_s": 3 * cells * steps / sec,
+           "roofline_first_order_frac": round(B_CELL_STAGE_FO * 3 * cells * steps / sec / 1e9
+                                              / PEAK_HBM_GBS, 4)}
+    b.close()
+    # per-case step time, each case alone (graph replay)
+    per = []
+    for c in cases:
+        bb = B.Batch([c], tile=8)
+        bb.run(2, graph=False)
+        bb.run(1, graph=True)
+        per.append(bb.run(10, graph=True) / 10 * 1e3)
+        bb.close()
+    per = np.array(per)
+    out["per_case_step_ms_alone"] = {"p50": float(np.percentile(per, 50)),
+                                     "p99": float(np.percentile(per, 99)),
+                                     "sum": float(per.sum())}
+    print("batched: %.3f ms/step (%d cells), alone p50 %.3f p99 %.3f sum %.3f ms" % (
+        ms, cells, out["per_case_step_ms_alone"]["p50"], out["per_case_step_ms_alone"]["p99"],
+        per.sum()), file=sys.stderr, flush=True)
+    return out
+
+
+def _cpu_name():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(n, steps):
     """The oracle (one core, the reference's data model) on a smaller mesh of
     the same workload; cell-stage updates per second."""
-    import os as _os
     from fv3d_amd import configs
     from fv3d_amd import solver as S
     from oracle import fvref
@@ -149,19 +277,37 @@ def cpu_baseline(n, steps):
     done, _ = o.timedisc("RK3-3", steps, True, 1e-10, 0.0, 1e300)
     sec = time.perf_counter() - t0
     N = mesh["n_cells"]
-    cpu_name = "unknown"
-    try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    cpu_name = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
-        pass
     return {"value": N * 3 * done / sec, "unit": "cell-stage/s", "cores": 1, "kind": "port",
             "sample": "oracle (C99, reference AoS data model, gcc -O3) on the same workload "
                       "at %d^3 x 6 = %d cells, %d RK3-3 steps, %.1f s on %s (nproc %d)"
-                      % (n, N, done, sec, cpu_name, _os.cpu_count() or 0)}
+                      % (n, N, done, sec, _cpu_name(), os.cpu_count() or 0)}
+
+
+def cpu_baseline_jfnk(n=(16, 4, 3)):
+    """The oracle's JFNK (numpy driver over the C residual, one core) on a
+    small mesh of the same channel: seconds to the 1e-8 stop norm."""
+    import numpy as np
+    from fv3d_amd import configs
+    from fv3d_amd import solver as S
+    from fv3d_amd._lib import lib
+    from oracle.jfnk import OracleJfnk
+
+    cfg = configs.bump_channel_steady(nx=n[0], ny=n[1], nz=n[2])
+    L = lib()
+    L.init_solver(b"cpu-baseline")
+    S.check(L.fvh_mesh_channel(n[0], n[1], n[2], 0.05, 20261203))
+    mesh = S.mesh_arrays()
+    oj = OracleJfnk(mesh, cfg["physics"], cfg["regions"], cfg["numerics"])
+    u0 = 0.5 * math.sqrt(1.4)
+    U = np.zeros((mesh["n_cells"], 5))
+    U[:, 0] = 1.0
+    U[:, 1] = u0
+    U[:, 4] = 1 / 0.4 + 0.5 * u0 * u0
+    _, st, _ = oj.solve(U, tol=1e-8)
+    return {"seconds": st["seconds"], "cells": mesh["n_cells"], "newton": st["newton_steps"],
+            "gmres": st["gmres_iterations"], "status": st["status"], "cores": 1, "kind": "port",
+            "sample": "oracle JFNK (numpy GMRES over the C99 residual, FP64 block Jacobi) on the "
+                      "%dx%dx%d channel, %s" % (n[0], n[1], n[2], _cpu_name())}
 
 
 if __name__ == "__main__":

@@ -107,3 +107,22 @@ class Batch:
 
     def close(self):
         self.case.close()
+
+
+def autotune(cases, tiles=(0, 4, 8, 16), steps=10, warmup=2, graph_options=(False, True),
+             **batch_kwargs):
+    """Times the batch step for every cell-ordering tile (hexahedra per side
+    of a locality tile; 0 = lexicographic) and with and without graph replay;
+    returns (best dict, table of dicts with ms_per_step)."""
+    table = []
+    for tile in tiles:
+        b = Batch(cases, tile=tile, **batch_kwargs)
+        b.run(warmup, graph=False)
+        for graph in graph_options:
+            b.run(1, graph=graph)
+            sec = b.run(steps, graph=graph)
+            table.append(dict(tile=tile, graph=graph, ms_per_step=sec / steps * 1e3,
+                              cells=b.n_cells))
+        b.close()
+    best = min(table, key=lambda r: r["ms_per_step"])
+    return best, table

@@ -224,10 +224,13 @@ __device__ __forceinline__ double limited(double phi, double gx, double gy,
     double lim = 1.0;
     if (bj)
     {
+        // y >= 1 exactly when the numerator reaches the slope (division is
+        // monotonic), and then min(1, y) = 1: the divide is only needed
+        // below that
         double y = 1.0;
-        if (slope > 0)
+        if (slope > 0 && mx - phi < slope)
             y = (mx - phi) / slope;
-        else if (slope < 0)
+        else if (slope < 0 && mn - phi > slope)
             y = (mn - phi) / slope;
         lim = rmin(1.0, y);
     }
