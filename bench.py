@@ -57,11 +57,17 @@ def main():
     ap.add_argument("--cpu-sample-n", type=int, default=64)
     ap.add_argument("--cpu-sample-steps", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-explicit", action="store_true",
+                    help="skip the headline explicit run (profiling the other legs)")
     ap.add_argument("--no-jfnk", action="store_true")
     ap.add_argument("--jfnk-mesh", default="160,40,13")
+    ap.add_argument("--jfnk-max-newton", type=int, default=400,
+                    help="Newton step limit of the JFNK legs (a short profiling run)")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-cases", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--no-batch-alone", action="store_true",
+                    help="skip the per-case runs of the batched leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (torch) whose HBM "
                          "byte counters calibrate FETCH_SIZE/WRITE_SIZE")
@@ -75,33 +81,37 @@ def main():
     from fv3d_amd import configs
     from fv3d_amd import solver as S
 
-    t0 = time.time()
-    cfg = configs.taylor_green(n=args.n, steps=args.steps + args.warmup + 100, tile=args.tile)
-    case = S.Case(**cfg)
-    print("mesh + init: %.1f s, %d cells, %d faces" % (time.time() - t0, case.n_cells,
-                                                    case.n_faces), file=sys.stderr, flush=True)
-    sec, k_us = case.bench(args.steps, args.warmup)
-    N = case.n_cells
+    value = ms_per_step = N = None
+    roofline = extra = None
     stages = 3
-    value = N * stages * args.steps / sec
-    ms_per_step = sec * 1e3 / args.steps
+    if not args.no_explicit:
+        t0 = time.time()
+        cfg = configs.taylor_green(n=args.n, steps=args.steps + args.warmup + 100, tile=args.tile)
+        case = S.Case(**cfg)
+        print("mesh + init: %.1f s, %d cells, %d faces" % (time.time() - t0, case.n_cells,
+                                                        case.n_faces), file=sys.stderr, flush=True)
+        sec, k_us = case.bench(args.steps, args.warmup)
+        N = case.n_cells
+        value = N * stages * args.steps / sec
+        ms_per_step = sec * 1e3 / args.steps
 
-    k_grad_s, k_stage_s = k_us[0] * 1e-6, k_us[1] * 1e-6
-    ach_stage = B_STAGE * N / k_stage_s / 1e9
-    ach_grad = B_GRAD * N / k_grad_s / 1e9
-    step_gbs = B_CELL_STAGE * N * stages * args.steps / sec / 1e9
-    roofline = {"bound": "hbm", "kernel": "fv_residual_group_kernel<AUSM,face states,viscous,4>",
-                "achieved": round(ach_stage, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": round(ach_stage / PEAK_HBM_GBS, 4), "traffic": None,
-                "bytes_per_cell": B_STAGE, "avg_us": round(k_us[1], 2)}
-    extra = {"gradient_kernel": {"achieved": round(ach_grad, 1), "bytes_per_cell": B_GRAD,
-                                 "avg_us": round(k_us[0], 2),
-                                 "frac": round(ach_grad / PEAK_HBM_GBS, 4)},
-             "step": {"achieved": round(step_gbs, 1), "bytes_per_cell_stage": B_CELL_STAGE,
-                      "frac": round(step_gbs / PEAK_HBM_GBS, 4)}}
-    print("explicit: %.3f ms/step, %.3e cell-stage/s" % (ms_per_step, value), file=sys.stderr,
-          flush=True)
-    case.close()
+        k_grad_s, k_stage_s = k_us[0] * 1e-6, k_us[1] * 1e-6
+        ach_stage = B_STAGE * N / k_stage_s / 1e9
+        ach_grad = B_GRAD * N / k_grad_s / 1e9
+        step_gbs = B_CELL_STAGE * N * stages * args.steps / sec / 1e9
+        roofline = {"bound": "hbm",
+                    "kernel": "fv_residual_group_kernel<AUSM,face states,viscous,4>",
+                    "achieved": round(ach_stage, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                    "frac": round(ach_stage / PEAK_HBM_GBS, 4), "traffic": None,
+                    "bytes_per_cell": B_STAGE, "avg_us": round(k_us[1], 2)}
+        extra = {"gradient_kernel": {"achieved": round(ach_grad, 1), "bytes_per_cell": B_GRAD,
+                                     "avg_us": round(k_us[0], 2),
+                                     "frac": round(ach_grad / PEAK_HBM_GBS, 4)},
+                 "step": {"achieved": round(step_gbs, 1), "bytes_per_cell_stage": B_CELL_STAGE,
+                          "frac": round(step_gbs / PEAK_HBM_GBS, 4)}}
+        print("explicit: %.3f ms/step, %.3e cell-stage/s" % (ms_per_step, value),
+              file=sys.stderr, flush=True)
+        case.close()
 
     if args.copy_calibration:
         import torch
@@ -121,11 +131,13 @@ def main():
               file=sys.stderr, flush=True)
         del src, dst
 
-    jfnk = None if args.no_jfnk else bench_jfnk([int(x) for x in args.jfnk_mesh.split(",")])
-    batched = None if args.no_batch else bench_batch(args.batch_cases, args.batch_steps)
+    jfnk = None if args.no_jfnk else bench_jfnk([int(x) for x in args.jfnk_mesh.split(",")],
+                                                 args.jfnk_max_newton)
+    batched = None if args.no_batch else bench_batch(args.batch_cases, args.batch_steps,
+                                                     not args.no_batch_alone)
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and not args.no_explicit:
         cpu = cpu_baseline(args.cpu_sample_n, args.cpu_sample_steps)
         if jfnk is not None:
             cpu["jfnk"] = cpu_baseline_jfnk()
@@ -155,7 +167,7 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def bench_jfnk(n):
+def bench_jfnk(n, max_newton=400):
     """Plain FP64 block-Jacobi JFNK against the learned-preconditioner,
     FP32-Krylov JFNK on the bump channel, both from the free stream to the
     1e-8 stop norm."""
@@ -169,7 +181,7 @@ def bench_jfnk(n):
     out = {"workload": "bump channel %dx%dx%d hex x 6 tets (%d cells), Ma 0.5, AUSM, linear "
                        "reconstruction, slip walls, state outlet; stop norm 1e-8"
                        % (n[0], n[1], n[2], case.n_cells), "cells": case.n_cells}
-    st, _ = case.jfnk(tol=1e-8, precision="fp64", precond="block")
+    st, _ = case.jfnk(tol=1e-8, precision="fp64", precond="block", max_newton=max_newton)
     U64 = case.state()
     out["plain_fp64_block"] = _jfnk_row(st)
     print("jfnk plain: %.2f s, %d newton, %d gmres" % (st["seconds"], st["newton_steps"],
@@ -185,7 +197,7 @@ def bench_jfnk(n):
         out["note"] = "no pretrained network at %s" % learned.PRETRAINED
     for name, kw in variants:
         case.set_state(U0)
-        st, _ = case.jfnk(tol=1e-8, **kw)
+        st, _ = case.jfnk(tol=1e-8, max_newton=max_newton, **kw)
         U = case.state()
         row = _jfnk_row(st)
         row["solution_rel_diff_vs_plain"] = float(np.linalg.norm(U - U64) / np.linalg.norm(U64))
@@ -206,7 +218,7 @@ def _jfnk_row(st):
             "precond_fallbacks": st["precond_fallbacks"]}
 
 
-def bench_batch(n_cases, steps):
+def bench_batch(n_cases, steps, alone=True):
     import numpy as np
     from fv3d_amd import batch as B
 
@@ -230,6 +242,8 @@ def bench_batch(n_cases, steps):
            "roofline_first_order_frac": round(B_CELL_STAGE_FO * 3 * cells * steps / sec / 1e9
                                               / PEAK_HBM_GBS, 4)}
     b.close()
+    if not alone:
+        return out
     # per-case step time, each case alone (graph replay)
     per = []
     for c in cases:

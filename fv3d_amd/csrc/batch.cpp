@@ -160,6 +160,7 @@ void enqueue_step(const RuntimeView &v, const StageBuffers &sb)
                 o.vg = g_b.vg;
                 o.c_begin = g.c_begin;
                 o.c_end = g.c_end;
+                o.prims = sb.prims[*sb.cur];
                 dev::launch_gradient(s, m, P, Uin, nullptr, o, 0, nullptr);
             }
             dev::StageParams sp{};
@@ -176,6 +177,8 @@ void enqueue_step(const RuntimeView &v, const StageBuffers &sb)
             sp.c_end = g.c_end;
             sp.case_dt = g_b.d_dt;
             sp.cell_case = g_b.cell_case;
+            sp.prims_in = sb.prims[*sb.cur];
+            sp.prims_out = sb.prims[1 - *sb.cur];
             dev::launch_residual(s, m, P, sp, g.flux, false, g.viscous != 0);
         }
         *sb.cur = 1 - *sb.cur;
@@ -330,6 +333,8 @@ int fvh_batch_run(int steps, int use_graph, double *seconds)
         if (sb.stages % 2 == 0 && use_graph)
             throw std::runtime_error("fvh_batch_run: graphs assume an odd stage count");
         hipStream_t s = v.stream;
+        // primitive rows of the current state (the steps keep them current)
+        dev::launch_prims(s, v.m, v.P, sb.U[*sb.cur], sb.prims[*sb.cur]);
         if (use_graph)
             for (int par = 0; par < 2; ++par)
             {

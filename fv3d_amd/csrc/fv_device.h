@@ -75,6 +75,7 @@ struct GradOut
     double *grad; // [N*30] all gradients (tests)
     double *mnmx; // [N*20] neighbourhood min/max (tests)
     int c_begin, c_end; // cells [c_begin, c_end); c_end = 0: all cells
+    const double *prims; // input: primitives [N][5] of U (null: computed)
 };
 
 struct StageParams
@@ -96,6 +97,8 @@ struct StageParams
     int c_begin, c_end;    // cells [c_begin, c_end); c_end = 0: all cells
     const double *case_dt; // batched cases: dt of case cell_case[c] (sc unused)
     const int *cell_case;
+    const double *prims_in; // primitives [N][5] of U_in (null: computed)
+    double *prims_out;      // primitives of U_out, written by the stage update
 };
 
 constexpr int kBlock = 256;
@@ -130,6 +133,9 @@ void launch_residual(hipStream_t s, const DevMesh &m, const PhysDev &P,
                      bool viscous);
 void launch_dt_partials(hipStream_t s, const DevMesh &m, const PhysDev &P,
                         const double *U, double *partials, int partials_cap);
+// Primitive rows [N][5] (u, v, w, p, T) of U (con_to_prim).
+void launch_prims(hipStream_t s, const DevMesh &m, const PhysDev &P, const double *U,
+                  double *prims);
 // Host-mirror rows: AoS phi_total[(N+B)*10] (primitives from con_to_prim,
 // ghost rows from the boundary conditions) and phi_dt[N*5] (R may be null).
 void launch_mirror(hipStream_t s, const DevMesh &m, const PhysDev &P,

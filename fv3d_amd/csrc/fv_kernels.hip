@@ -100,8 +100,7 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
         return;
 
     double pc[10];
-    load_cons(U, c, pc);
-    con_to_prim(P, pc);
+    load_phi(P, U, out.prims, c, pc);
 
     double gx[10], gy[10], gz[10], mn[10], mx[10];
 #pragma unroll
@@ -133,8 +132,7 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
         double pn[10];
         if (nb < N)
         {
-            load_cons(U, nb, pn);
-            con_to_prim(P, pn);
+            load_phi(P, U, out.prims, nb, pn);
         }
         else
         {
@@ -326,8 +324,7 @@ __global__ __launch_bounds__(kBlock, FVH_K2_MINW) void fv_residual_group_kernel(
         else
         {
             double pc[10], pn[10];
-            load_cons(sp.U_in, c, pc);
-            con_to_prim(P, pc);
+            load_phi(P, sp.U_in, sp.prims_in, c, pc);
             if (ghost)
             {
                 const double *gp = sp.gphi + (size_t)(nb - N) * 10;
@@ -337,8 +334,7 @@ __global__ __launch_bounds__(kBlock, FVH_K2_MINW) void fv_residual_group_kernel(
             }
             else
             {
-                load_cons(sp.U_in, nb, pn);
-                con_to_prim(P, pn);
+                load_phi(P, sp.U_in, sp.prims_in, nb, pn);
             }
 #pragma unroll
             for (int j = 0; j < 10; ++j)
@@ -429,11 +425,27 @@ __global__ __launch_bounds__(kBlock, FVH_K2_MINW) void fv_residual_group_kernel(
                 unew[j] = pc[j] + kj * dt * sp.rk_g;
                 sp.U_out[(size_t)c * 5 + j] = unew[j];
             }
+            if (sp.prims_out)
+            {
+                double q[10];
+#pragma unroll
+                for (int j = 0; j < 5; ++j)
+                    q[j] = unew[j];
+                con_to_prim(P, q);
+#pragma unroll
+                for (int j = 0; j < 5; ++j)
+                    sp.prims_out[(size_t)c * 5 + j] = q[5 + j];
+            }
             if (sp.do_reduce)
             {
                 // time step with the new density and the primitives of this
                 // stage's input (navier_stokes.c:171-187)
-                con_to_prim(P, pc);
+                if (sp.prims_in)
+#pragma unroll
+                    for (int j = 0; j < 5; ++j)
+                        pc[5 + j] = sp.prims_in[(size_t)c * 5 + j];
+                else
+                    con_to_prim(P, pc);
 #pragma unroll
                 for (int j = 0; j < 5; ++j)
                     part[j] = rabs(R5[j]) * vol;
@@ -784,6 +796,28 @@ void launch_dt_partials(hipStream_t s, const DevMesh &m, const PhysDev &P,
         throw std::runtime_error("reduction partials buffer too small");
     hipLaunchKernelGGL(fv_dt_partials_kernel, dim3(dt_partials_grid(m)),
                        dim3(kBlock), 0, s, m, P, U, partials);
+}
+
+__global__ __launch_bounds__(kBlock) void fv_prims_kernel(int N, PhysDev P,
+                                                         const double *__restrict__ U,
+                                                         double *__restrict__ prims)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N)
+        return;
+    double q[10];
+    load_cons(U, c, q);
+    con_to_prim(P, q);
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+        prims[(size_t)c * 5 + j] = q[5 + j];
+}
+
+void launch_prims(hipStream_t s, const DevMesh &m, const PhysDev &P, const double *U,
+                  double *prims)
+{
+    hipLaunchKernelGGL(fv_prims_kernel, dim3(cell_blocks(m.N)), dim3(kBlock), 0, s, m.N, P, U,
+                       prims);
 }
 
 void launch_mirror(hipStream_t s, const DevMesh &m, const PhysDev &P,

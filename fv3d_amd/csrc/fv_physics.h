@@ -180,6 +180,25 @@ __device__ __forceinline__ void load_cons(const double *__restrict__ U, int c,
         phi[j] = p[j];
 }
 
+// Conservative state and its primitives: read from the primitive rows
+// [N][5] (u, v, w, p, T) when they are kept beside the state (the values
+// con_to_prim produced from the same state, so bit-identical), else computed.
+__device__ __forceinline__ void load_phi(const PhysDev &P, const double *__restrict__ U,
+                                         const double *__restrict__ prims, int c,
+                                         double *phi)
+{
+    load_cons(U, c, phi);
+    if (prims)
+    {
+        const double *q = prims + (size_t)c * 5;
+#pragma unroll
+        for (int j = 0; j < 5; ++j)
+            phi[5 + j] = q[j];
+    }
+    else
+        con_to_prim(P, phi);
+}
+
 // Workgroup -> cell tile mapping.  The hardware hands consecutive
 // workgroups to the 8 XCDs in turn; this gives each XCD one contiguous range
 // of cell tiles so that the neighbour data a tile gathers is mostly in that

@@ -114,7 +114,7 @@ void residual_norms(const RuntimeView &v, double &norm, double &l2)
 
 struct GmresResult
 {
-    int its = 0, evals = 0;
+    int its = 0, evals = 0, reorth = 0;
     double rel = 1.0;
 };
 
@@ -133,9 +133,8 @@ GmresResult gmres(const RuntimeView &v, const fvh_jfnk_options_t &o, double fd_n
     FVH_HIP_CHECK(hipMemsetAsync(g_ws.xacc, 0, (size_t)n * sizeof(double), s));
     double beta0 = 0.0;
 
-    // z = M^-1 (vector in T), w = A z into out
+    // w = A z into out, with |z|^2 already in dots[kZZ]
     auto apply_op = [&](const T *zin, T *out) {
-        kry::multidot<T>(s, n, 1, zin, ld, zin, g_ws.partials, dots + kZZ);
         kry::fd_eps(s, dots + kZZ, fd_num, dots + kEps);
         kry::perturb<T>(s, n, v.U, dots + kEps, zin, g_ws.Ut);
         eval_residual(g_ws.Ut, g_ws.Rt);
@@ -151,7 +150,7 @@ GmresResult gmres(const RuntimeView &v, const fvh_jfnk_options_t &o, double fd_n
         else
         {
             kry::convert<T>(s, n, g_ws.xacc, t);
-            kry::precond_apply<T>(s, N, g_ws.Minv, t, z);
+            kry::precond_apply_norm<T>(s, N, g_ws.Minv, t, z, g_ws.partials, dots + kZZ);
             apply_op(z, V);
             kry::rhs_minus<T>(s, n, g_ws.R0, V);
         }
@@ -173,20 +172,38 @@ GmresResult gmres(const RuntimeView &v, const fvh_jfnk_options_t &o, double fd_n
         for (int j = 0; j < m; ++j)
         {
             T *w = V + (size_t)(j + 1) * ld;
-            kry::precond_apply<T>(s, N, g_ws.Minv, V + (size_t)j * ld, z);
+            kry::precond_apply_norm<T>(s, N, g_ws.Minv, V + (size_t)j * ld, z, g_ws.partials,
+                                       dots + kZZ);
             apply_op(z, w);
-            // classical Gram-Schmidt, twice
-            kry::multidot<T>(s, n, j + 1, V, ld, w, g_ws.partials, dots + kH1);
+            // classical Gram-Schmidt with one pass and the norm by Pythagoras
+            // (w is V_{j+1}, so the same multi-dot gives <w, w>); a second
+            // pass when the new vector lost more than half its square norm
+            // to cancellation (the 1/sqrt(2) criterion of Daniel, Gragg,
+            // Kaufman and Stewart)
+            kry::multidot<T>(s, n, j + 2, V, ld, w, g_ws.partials, dots + kH1);
             kry::multiaxpy<T>(s, n, j + 1, V, ld, dots + kH1, w);
-            kry::multidot<T>(s, n, j + 1, V, ld, w, g_ws.partials, dots + kH2);
-            kry::multiaxpy<T>(s, n, j + 1, V, ld, dots + kH2, w);
-            kry::multidot<T>(s, n, 1, w, ld, w, g_ws.partials, dots + kWW);
-            fetch(s, 0, kWW + 1);
+            fetch(s, 0, j + 2);
             ++res.its;
+            const double ww0 = host[kH1 + j + 1];
+            double hh = 0.0;
+            for (int i = 0; i <= j; ++i)
+                hh += host[kH1 + i] * host[kH1 + i];
+            double hn2 = ww0 - hh;
+            for (int i = 0; i <= j; ++i)
+                host[kH2 + i] = 0.0;
+            if (!(hn2 > 0.5 * ww0))
+            {
+                kry::multidot<T>(s, n, j + 2, V, ld, w, g_ws.partials, dots + kH2);
+                kry::multiaxpy<T>(s, n, j + 1, V, ld, dots + kH2, w);
+                kry::multidot<T>(s, n, 1, w, ld, w, g_ws.partials, dots + kWW);
+                fetch(s, kH2, kry::kMaxDots + 1);
+                hn2 = host[kWW];
+                ++res.reorth;
+            }
             double *Hj = &H[(size_t)j * (m + 1)]; // column j
             for (int i = 0; i <= j; ++i)
                 Hj[i] = host[kH1 + i] + host[kH2 + i];
-            const double hn = std::sqrt(host[kWW]);
+            const double hn = std::sqrt(hn2 > 0.0 ? hn2 : 0.0);
             Hj[j + 1] = hn;
             for (int i = 0; i < j; ++i)
             {
@@ -375,7 +392,7 @@ int fvh_jfnk_default_options(fvh_jfnk_options_t *o)
     o->precond = FVH_PRECOND_BLOCK;
     o->verbose = 0;
     o->learned_check = 10;
-    o->learned_tol = 0.5;
+    o->learned_tol = 0.2;
     return FVH_OK;
 }
 

@@ -33,6 +33,11 @@ void precond_blocks(hipStream_t s, const dev::DevMesh &m, const dev::PhysDev &P,
 template <typename T>
 void precond_apply(hipStream_t s, int N, const double *Minv, const T *v, T *z);
 
+// z = Minv v and zz[0] = |z|_2^2 (fixed-order FP64 sum of the stored values)
+template <typename T>
+void precond_apply_norm(hipStream_t s, int N, const double *Minv, const T *v, T *z,
+                        double *partials, double *zz);
+
 // eps[0] = num / sqrt(zz[0]) (num / 1e-300 for zz = 0)
 void fd_eps(hipStream_t s, const double *zz, double num, double *eps);
 

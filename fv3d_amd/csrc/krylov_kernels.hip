@@ -174,6 +174,41 @@ __global__ void k_fd_eps(const double *__restrict__ zz, double num, double *__re
     }
 }
 
+// z = Minv v and the per-block partial sums of |z|^2 (grid-stride over
+// cells on the fixed dot grid; z^2 of the stored T values)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_precond_apply_norm(int N,
+                                                               const double *__restrict__ Minv,
+                                                               const T *__restrict__ v,
+                                                               T *__restrict__ z,
+                                                               double *__restrict__ partials)
+{
+    __shared__ double sh[kBlock];
+    double acc = 0.0;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < N; c += gridDim.x * blockDim.x)
+    {
+        double x[5];
+#pragma unroll
+        for (int j = 0; j < 5; ++j)
+            x[j] = (double)v[(size_t)c * 5 + j];
+        const double *Mi = Minv + (size_t)c * 25;
+#pragma unroll
+        for (int r = 0; r < 5; ++r)
+        {
+            double a = 0.0;
+#pragma unroll
+            for (int q = 0; q < 5; ++q)
+                a += Mi[r * 5 + q] * x[q];
+            const T zt = (T)a;
+            z[(size_t)c * 5 + r] = zt;
+            acc += (double)zt * (double)zt;
+        }
+    }
+    dev::block_sum(sh, acc);
+    if (threadIdx.x == 0)
+        partials[(size_t)blockIdx.x * kMaxDots] = acc;
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_perturb(int n, const double *__restrict__ U,
                                                     const double *__restrict__ eps,
@@ -212,22 +247,24 @@ __global__ __launch_bounds__(kBlock) void k_rhs_minus(int n, const double *__res
 // Deterministic multi-dot: a fixed grid of kDotBlocks blocks, grid-stride
 // loop, per-thread FP64 sums, then a per-block column reduction in thread
 // order; the fold adds the block rows in block order.
-template <typename T>
+// KMAX >= k accumulators per thread (16, 32 or 64: fewer registers, more
+// waves in flight for the short Krylov bases).
+template <typename T, int KMAX>
 __global__ __launch_bounds__(kBlock) void k_multidot(int n, int k, const T *__restrict__ V,
                                                      size_t ld, const T *__restrict__ w,
                                                      double *__restrict__ partials)
 {
     __shared__ double sh[kBlock * 17];
-    double acc[kMaxDots];
+    double acc[KMAX];
 #pragma unroll
-    for (int j = 0; j < kMaxDots; ++j)
+    for (int j = 0; j < KMAX; ++j)
         acc[j] = 0.0;
     const int stride = gridDim.x * blockDim.x;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
     {
         const double wi = (double)w[i];
 #pragma unroll
-        for (int j = 0; j < kMaxDots; ++j)
+        for (int j = 0; j < KMAX; ++j)
             if (j < k)
                 acc[j] += (double)V[(size_t)j * ld + i] * wi;
     }
@@ -236,7 +273,7 @@ __global__ __launch_bounds__(kBlock) void k_multidot(int n, int k, const T *__re
     // sub-sums in order
     __shared__ double sub[kBlock];
 #pragma unroll
-    for (int j0 = 0; j0 < kMaxDots; j0 += 16)
+    for (int j0 = 0; j0 < KMAX; j0 += 16)
     {
         if (j0 >= k)
             break;
@@ -443,6 +480,15 @@ void precond_apply(hipStream_t s, int N, const double *Minv, const T *v, T *z)
                        Minv, v, z);
 }
 
+template <typename T>
+void precond_apply_norm(hipStream_t s, int N, const double *Minv, const T *v, T *z,
+                        double *partials, double *zz)
+{
+    hipLaunchKernelGGL(k_precond_apply_norm<T>, dim3(kDotBlocks), dim3(kBlock), 0, s, N, Minv,
+                       v, z, partials);
+    hipLaunchKernelGGL(k_fold_dots, dim3(1), dim3(kBlock), 0, s, kDotBlocks, partials, zz);
+}
+
 void fd_eps(hipStream_t s, const double *zz, double num, double *eps)
 {
     hipLaunchKernelGGL(k_fd_eps, dim3(1), dim3(64), 0, s, zz, num, eps);
@@ -476,8 +522,15 @@ void multidot(hipStream_t s, int n, int k, const T *V, size_t ld, const T *w,
 {
     if (k < 1 || k > kMaxDots)
         throw std::runtime_error("multidot: bad vector count");
-    hipLaunchKernelGGL(k_multidot<T>, dim3(kDotBlocks), dim3(kBlock), 0, s, n, k, V, ld, w,
-                       partials);
+    if (k <= 16)
+        hipLaunchKernelGGL((k_multidot<T, 16>), dim3(kDotBlocks), dim3(kBlock), 0, s, n, k, V, ld,
+                           w, partials);
+    else if (k <= 32)
+        hipLaunchKernelGGL((k_multidot<T, 32>), dim3(kDotBlocks), dim3(kBlock), 0, s, n, k, V, ld,
+                           w, partials);
+    else
+        hipLaunchKernelGGL((k_multidot<T, 64>), dim3(kDotBlocks), dim3(kBlock), 0, s, n, k, V, ld,
+                           w, partials);
     hipLaunchKernelGGL(k_fold_dots, dim3(k), dim3(kBlock), 0, s, kDotBlocks, partials, dots);
 }
 
@@ -536,6 +589,8 @@ void update(hipStream_t s, int n, double *U, const double *dU, double alpha)
 
 #define FVH_KRY_INST(T)                                                                    \
     template void precond_apply<T>(hipStream_t, int, const double *, const T *, T *);      \
+    template void precond_apply_norm<T>(hipStream_t, int, const double *, const T *, T *,  \
+                                        double *, double *);                               \
     template void perturb<T>(hipStream_t, int, const double *, const double *, const T *,  \
                              double *);                                                    \
     template void op_finish<T>(hipStream_t, int, const T *, const double *,                \

@@ -46,6 +46,7 @@ RuntimeView runtime_view();
 struct StageBuffers
 {
     double *U[2];
+    double *prims[2]; // primitive rows of U[i] (kept by the stage updates)
     int *cur;
     double *k, *R;
     int stages;

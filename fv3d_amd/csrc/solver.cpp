@@ -149,6 +149,7 @@ struct Device
     std::vector<void *> allocs;
     dev::DevMesh m{};
     double *U[2] = {nullptr, nullptr};
+    double *prims[2] = {nullptr, nullptr}; // primitives of U[i] during stepping
     int cur = 0;
     double *k = nullptr;
     double *R = nullptr;
@@ -294,16 +295,63 @@ dev::GradOut grad_out()
 }
 
 // Residual of U_in into R (and, for a stage, into U_out/k).
+// Per-kernel event timing of enqueue_residual (the benchmark's
+// instrumented pass): gradient and residual/stage kernel durations.
+struct KernelTimer
+{
+    hipEvent_t a = nullptr, b = nullptr;
+    double acc[3] = {0, 0, 0};
+    int cnt[3] = {0, 0, 0};
+    void start() { HIP_CHECK(hipEventRecord(a, g_dev.stream)); }
+    void stop(int q)
+    {
+        HIP_CHECK(hipEventRecord(b, g_dev.stream));
+        HIP_CHECK(hipEventSynchronize(b));
+        float t = 0;
+        HIP_CHECK(hipEventElapsedTime(&t, a, b));
+        acc[q] += t * 1e3;
+        cnt[q]++;
+    }
+};
+
+void ensure_prims()
+{
+    if (!g_dev.prims[0])
+    {
+        g_dev.prims[0] = dmalloc<double>((size_t)5 * g_dev.m.N);
+        g_dev.prims[1] = dmalloc<double>((size_t)5 * g_dev.m.N);
+    }
+}
+
+// Primitive rows of the current state, kept up to date by the stage updates
+// from here on (called at the start of every explicit run).
+void refresh_prims()
+{
+    ensure_prims();
+    dev::launch_prims(g_dev.stream, g_dev.m, phys(), g_dev.U[g_dev.cur],
+                      g_dev.prims[g_dev.cur]);
+}
+
 void enqueue_residual(const double *U_in, double *U_out, double *R_out,
-                      int stage, bool update, bool reduce, dev::DevScalars *sc)
+                      int stage, bool update, bool reduce, dev::DevScalars *sc,
+                      const double *prims_in = nullptr, double *prims_out = nullptr,
+                      KernelTimer *tm = nullptr)
 {
     const dev::PhysDev P = phys();
     ensure_gradient_buffers();
     dev::launch_ghost(g_dev.stream, g_dev.m, P, U_in, g_dev.gphi,
                       is_linear() ? g_dev.fs : nullptr, sc);
     if (needs_gradients())
-        dev::launch_gradient(g_dev.stream, g_dev.m, P, U_in, g_dev.gphi,
-                             grad_out(), is_linear(), sc);
+    {
+        dev::GradOut go = grad_out();
+        go.prims = prims_in;
+        if (tm)
+            tm->start();
+        dev::launch_gradient(g_dev.stream, g_dev.m, P, U_in, g_dev.gphi, go,
+                             is_linear(), sc);
+        if (tm)
+            tm->stop(0);
+    }
     dev::StageParams sp{};
     sp.U_in = U_in;
     sp.U_out = U_out;
@@ -320,8 +368,14 @@ void enqueue_residual(const double *U_in, double *U_out, double *R_out,
     sp.sc = sc;
     sp.partials = g_dev.partials;
     sp.partials_cap = g_dev.partials_cap;
+    sp.prims_in = prims_in;
+    sp.prims_out = update ? prims_out : nullptr;
+    if (tm)
+        tm->start();
     dev::launch_residual(g_dev.stream, g_dev.m, P, sp, g_flux, is_linear(),
                          is_viscous());
+    if (tm)
+        tm->stop(1);
 }
 
 void sync_mirror(bool with_dt)
@@ -635,7 +689,9 @@ void print_row(int iter, const double *row, bool out, int inner = 0, int lsoe = 
 }
 
 // One explicit step (timestep_lserkw2, explicit.c:96-141) + bookkeeping.
-void enqueue_explicit_step(dev::DevScalars *sc)
+// One explicit step; the primitive rows of U[cur] must be current
+// (refresh_prims at the start of a run).
+void enqueue_explicit_step(dev::DevScalars *sc, KernelTimer *tm = nullptr)
 {
     for (int s = 0; s < g_rk_stages; ++s)
     {
@@ -643,11 +699,15 @@ void enqueue_explicit_step(dev::DevScalars *sc)
         double *Uin = g_dev.U[g_dev.cur];
         double *Uout = g_dev.U[1 - g_dev.cur];
         enqueue_residual(Uin, Uout, last ? g_dev.R : nullptr, s, true, last,
-                         sc);
+                         sc, g_dev.prims[g_dev.cur], g_dev.prims[1 - g_dev.cur], tm);
         g_dev.cur = 1 - g_dev.cur;
     }
+    if (tm)
+        tm->start();
     dev::launch_finalize(g_dev.stream, sc, g_dev.partials, dev::residual_grid(g_dev.m), g_dev.fold, g_dev.fold_cap, phys(), 1, g_dev.log,
                          g_dev.log_cap);
+    if (tm)
+        tm->stop(2);
 }
 
 void init_scalars(int iter0, double t0, int max_iter, bool transient,
@@ -705,6 +765,7 @@ int run_explicit(int max_iter)
     }
     g_log.assign((size_t)max_iter * 8, 0.0);
     init_scalars(iter0, t0, max_iter, g_transient, g_t_end);
+    refresh_prims();
 
     int printed = 0;
     int status = 0;
@@ -964,8 +1025,11 @@ StageBuffers stage_buffers()
     if (g_rk_stages <= 0)
         throw std::runtime_error("explicit scheme not initialised");
     StageBuffers b{};
+    ensure_prims();
     b.U[0] = g_dev.U[0];
     b.U[1] = g_dev.U[1];
+    b.prims[0] = g_dev.prims[0];
+    b.prims[1] = g_dev.prims[1];
     b.cur = &g_dev.cur;
     b.k = g_dev.k;
     b.R = g_dev.R;
@@ -1950,6 +2014,7 @@ int fvh_bench_explicit(int n_steps, int n_warmup, double *seconds,
             g_dev.log = dmalloc<double>((size_t)total * 8);
         }
         init_scalars(0, 0.0, 1 << 30, true, DBL_MAX);
+        refresh_prims();
         for (int i = 0; i < n_warmup; ++i)
             enqueue_explicit_step(g_dev.sc);
         hipEvent_t e0, e1;
@@ -1968,73 +2033,18 @@ int fvh_bench_explicit(int n_steps, int n_warmup, double *seconds,
         if (g_dev.sc_host->done < 0)
             throw std::runtime_error("NaN/Inf residual during the benchmark");
 
-        // instrumented pass: events around each kernel of n_steps steps
+        // instrumented pass: events around each kernel of up to 20 steps
         if (k_us)
         {
             const int ev_steps = std::max(1, std::min(n_steps, 20));
-            double acc[3] = {0, 0, 0};
-            int cnt[3] = {0, 0, 0};
-            hipEvent_t a, b;
-            HIP_CHECK(hipEventCreate(&a));
-            HIP_CHECK(hipEventCreate(&b));
-            const dev::PhysDev P = phys();
+            KernelTimer tm;
+            HIP_CHECK(hipEventCreate(&tm.a));
+            HIP_CHECK(hipEventCreate(&tm.b));
             for (int st = 0; st < ev_steps; ++st)
-            {
-                for (int s = 0; s < g_rk_stages; ++s)
-                {
-                    const bool last = s == g_rk_stages - 1;
-                    double *Uin = g_dev.U[g_dev.cur];
-                    double *Uout = g_dev.U[1 - g_dev.cur];
-                    ensure_gradient_buffers();
-                    dev::launch_ghost(g_dev.stream, g_dev.m, P, Uin, g_dev.gphi,
-                                      is_linear() ? g_dev.fs : nullptr, g_dev.sc);
-                    if (needs_gradients())
-                    {
-                        HIP_CHECK(hipEventRecord(a, g_dev.stream));
-                        dev::launch_gradient(g_dev.stream, g_dev.m, P, Uin,
-                                             g_dev.gphi, grad_out(), is_linear(),
-                                             g_dev.sc);
-                        HIP_CHECK(hipEventRecord(b, g_dev.stream));
-                        HIP_CHECK(hipEventSynchronize(b));
-                        float t = 0;
-                        HIP_CHECK(hipEventElapsedTime(&t, a, b));
-                        acc[0] += t * 1e3, cnt[0]++;
-                    }
-                    dev::StageParams sp{};
-                    sp.U_in = Uin;
-                    sp.U_out = Uout;
-                    sp.k = g_dev.k;
-                    sp.R_out = last ? g_dev.R : nullptr;
-                    sp.vg = g_dev.vg;
-                    sp.fs = g_dev.fs;
-                    sp.gphi = g_dev.gphi;
-                    sp.rk_a = g_rk_a[s];
-                    sp.rk_g = g_rk_g[s];
-                    sp.stage0 = s == 0;
-                    sp.do_update = 1;
-                    sp.do_reduce = last;
-                    sp.sc = g_dev.sc;
-                    sp.partials = g_dev.partials;
-                    sp.partials_cap = g_dev.partials_cap;
-                    HIP_CHECK(hipEventRecord(a, g_dev.stream));
-                    dev::launch_residual(g_dev.stream, g_dev.m, P, sp, g_flux,
-                                         is_linear(), is_viscous());
-                    HIP_CHECK(hipEventRecord(b, g_dev.stream));
-                    HIP_CHECK(hipEventSynchronize(b));
-                    float t = 0;
-                    HIP_CHECK(hipEventElapsedTime(&t, a, b));
-                    acc[1] += t * 1e3, cnt[1]++;
-                    g_dev.cur = 1 - g_dev.cur;
-                }
-                HIP_CHECK(hipEventRecord(a, g_dev.stream));
-                dev::launch_finalize(g_dev.stream, g_dev.sc, g_dev.partials, dev::residual_grid(g_dev.m), g_dev.fold, g_dev.fold_cap, P, 1, g_dev.log,
-                                     g_dev.log_cap);
-                HIP_CHECK(hipEventRecord(b, g_dev.stream));
-                HIP_CHECK(hipEventSynchronize(b));
-                float t = 0;
-                HIP_CHECK(hipEventElapsedTime(&t, a, b));
-                acc[2] += t * 1e3, cnt[2]++;
-            }
+                enqueue_explicit_step(g_dev.sc, &tm);
+            double *acc = tm.acc;
+            int *cnt = tm.cnt;
+            hipEvent_t a = tm.a, b = tm.b;
             for (int q = 0; q < 3; ++q)
                 k_us[q] = cnt[q] ? acc[q] / cnt[q] : 0.0;
             hipEventDestroy(a);

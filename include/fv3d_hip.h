@@ -157,7 +157,7 @@ typedef struct
                             with the block-Jacobi blocks every learned_check
                             Newton steps (and at the first); 0 = first only */
     double learned_tol;  /* mean per-cell relative Frobenius error above which
-                            the solve falls back to block Jacobi (default 0.5) */
+                            the solve falls back to block Jacobi (default 0.2) */
 } fvh_jfnk_options_t;
 
 typedef struct

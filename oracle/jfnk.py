@@ -19,7 +19,8 @@ written plainly in FP64 numpy:
     differences;
   - right-preconditioned GMRES(m) with restarts on
     A z = z/dtau - [R(U + eps z) - R(U)]/eps, eps = fd_eps (1+|U|)/|z|,
-    classical Gram-Schmidt twice, Givens rotations, relative tolerance rtol;
+    classical Gram-Schmidt with selective reorthogonalisation, Givens
+    rotations, relative tolerance rtol;
   - update U += alpha M^-1 x with alpha = min(1, max_change / max relative
     change of rho and rho*E);
   - CFL times cfl_cut (not below 1) when the new stop norm exceeds 3x the
@@ -28,6 +29,7 @@ written plainly in FP64 numpy:
 Stop norm: max_j (1/V_glob) sum_i |R_ij| V_i (src/Solver/src/analyze/
 analyze.c:37-61 without the dt factor).
 """
+import math
 import time
 
 import numpy as np
@@ -59,9 +61,12 @@ def colouring(mesh):
 
 
 def gmres(op, b, m, max_restarts, rtol):
-    """Restarted GMRES(m) for op(x) = b from x = 0: classical Gram-Schmidt
-    twice, Givens rotations; stops when the estimated residual is at most
-    rtol |b|.  Returns (x, Arnoldi steps, estimated relative residual)."""
+    """Restarted GMRES(m) for op(x) = b from x = 0: classical Gram-Schmidt in
+    one pass with the new vector's norm by Pythagoras (|w|^2 - |h|^2), and a
+    second pass when that lost more than half of |w|^2 (the 1/sqrt(2) test
+    of Daniel, Gragg, Kaufman and Stewart); Givens rotations; stops when the
+    estimated residual is at most rtol |b|.  Returns (x, Arnoldi steps,
+    estimated relative residual)."""
     x = np.zeros(b.size)
     beta0 = None
     its = 0
@@ -85,12 +90,17 @@ def gmres(op, b, m, max_restarts, rtol):
         for j in range(m):
             w = op(Vk[j])
             h1 = Vk[: j + 1] @ w
+            ww = w @ w
             w = w - h1 @ Vk[: j + 1]
-            h2 = Vk[: j + 1] @ w
-            w = w - h2 @ Vk[: j + 1]
+            hn2 = ww - h1 @ h1
+            h2 = np.zeros_like(h1)
+            if not hn2 > 0.5 * ww:
+                h2 = Vk[: j + 1] @ w
+                w = w - h2 @ Vk[: j + 1]
+                hn2 = w @ w
             its += 1
             H[: j + 1, j] = h1 + h2
-            hn = np.linalg.norm(w)
+            hn = math.sqrt(max(hn2, 0.0))
             H[j + 1, j] = hn
             for i in range(j):
                 a_, b_ = H[i, j], H[i + 1, j]
