@@ -18,7 +18,8 @@ class JfnkOptions(ctypes.Structure):
                 ("cfl_growth", c_double), ("cfl_cut", c_double), ("cfl_stall", c_double),
                 ("cfl_stall_cut", c_double), ("cfl_max", c_double),
                 ("krylov_dims", c_int), ("max_restarts", c_int), ("rtol", c_double),
-                ("fd_eps", c_double), ("max_change", c_double), ("precision", c_int),
+                ("fd_eps", c_double), ("fd_eps32", c_double), ("max_change", c_double),
+                ("precision", c_int),
                 ("precond", c_int), ("verbose", c_int), ("learned_check", c_int),
                 ("learned_tol", c_double)]
 

@@ -123,14 +123,16 @@ void launch_ghost(hipStream_t s, const DevMesh &m, const PhysDev &P,
                   const double *U, double *gphi, double *fs,
                   const DevScalars *sc);
 // linear: limited linear face states; otherwise the cell values.
+// fp32: the arithmetic in FP32 (inputs and outputs stay FP64 arrays), for
+// the FP32 Jacobian-vector products of the Newton-Krylov solve.
 void launch_gradient(hipStream_t s, const DevMesh &m, const PhysDev &P,
                      const double *U, const double *gphi, const GradOut &out,
-                     int linear, const DevScalars *sc);
+                     int linear, const DevScalars *sc, bool fp32 = false);
 // face_states: read the face states written by launch_gradient; otherwise
 // (first order) take the cell values of U directly.
 void launch_residual(hipStream_t s, const DevMesh &m, const PhysDev &P,
                      const StageParams &sp, int flux, bool face_states,
-                     bool viscous);
+                     bool viscous, bool fp32 = false);
 void launch_dt_partials(hipStream_t s, const DevMesh &m, const PhysDev &P,
                         const double *U, double *partials, int partials_cap);
 // Primitive rows [N][5] (u, v, w, p, T) of U (con_to_prim).

@@ -86,7 +86,7 @@ __global__ __launch_bounds__(kBlock) void fv_ghost_kernel(
 // (reconstruction.c:168-255 / 142-163).  Writes the face states by face id
 // and side, and the gradient subset the viscous flux needs.
 // ---------------------------------------------------------------------------
-template <int MAXF>
+template <int MAXF, class Real>
 __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
     DevMesh m, PhysDev P, const double *__restrict__ U,
     const double *__restrict__ gphi, GradOut out, int linear,
@@ -99,16 +99,16 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
     if (c >= out.c_end)
         return;
 
-    double pc[10];
+    Real pc[10];
     load_phi(P, U, out.prims, c, pc);
 
-    double gx[10], gy[10], gz[10], mn[10], mx[10];
+    Real gx[10], gy[10], gz[10], mn[10], mx[10];
 #pragma unroll
     for (int j = 0; j < 10; ++j)
     {
-        gx[j] = 0.0;
-        gy[j] = 0.0;
-        gz[j] = 0.0;
+        gx[j] = Real(0.0);
+        gy[j] = Real(0.0);
+        gz[j] = Real(0.0);
         mn[j] = pc[j];
         mx[j] = pc[j];
     }
@@ -126,10 +126,10 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
         const int nb = own0 ? fc.y : fc.x;
         const double2 *g = reinterpret_cast<const double2 *>(m.fgB + (size_t)f * kFgB);
         const double2 g0 = g[0], g1 = g[1], g2 = g[2];
-        const double n0 = g0.x, n1 = g0.y, n2 = g1.x;
-        const double area = g1.y, lam = g2.x;
+        const Real n0 = Real(g0.x), n1 = Real(g0.y), n2 = Real(g1.x);
+        const Real area = Real(g1.y), lam = Real(g2.x);
 
-        double pn[10];
+        Real pn[10];
         if (nb < N)
         {
             load_phi(P, U, out.prims, nb, pn);
@@ -140,18 +140,18 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
             const double *gp = gphi + (size_t)(nb - N) * 10;
 #pragma unroll
             for (int j = 0; j < 10; ++j)
-                pn[j] = gp[j];
+                pn[j] = Real(gp[j]);
         }
 
 #pragma unroll
         for (int j = 0; j < 10; ++j)
         {
-            const double p0 = own0 ? pc[j] : pn[j];
-            const double p1 = own0 ? pn[j] : pc[j];
-            const double pm = p0 + lam * (p1 - p0);
-            const double ax = pm * n0 * area;
-            const double ay = pm * n1 * area;
-            const double az = pm * n2 * area;
+            const Real p0 = own0 ? pc[j] : pn[j];
+            const Real p1 = own0 ? pn[j] : pc[j];
+            const Real pm = p0 + lam * (p1 - p0);
+            const Real ax = pm * n0 * area;
+            const Real ay = pm * n1 * area;
+            const Real az = pm * n2 * area;
             if (own0)
             {
                 gx[j] += ax;
@@ -169,7 +169,7 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
         }
     }
 
-    const double sv = m.svol[c];
+    const Real sv = Real(m.svol[c]);
 #pragma unroll
     for (int j = 0; j < 10; ++j)
     {
@@ -181,12 +181,12 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
     if (out.vg)
     {
         double2 *d = reinterpret_cast<double2 *>(out.vg + (size_t)c * 12);
-        d[0] = make_double2(gx[1], gx[2]);
-        d[1] = make_double2(gx[3], gx[9]);
-        d[2] = make_double2(gy[1], gy[2]);
-        d[3] = make_double2(gy[3], gy[9]);
-        d[4] = make_double2(gz[1], gz[2]);
-        d[5] = make_double2(gz[3], gz[9]);
+        d[0] = make_double2(double(gx[1]), double(gx[2]));
+        d[1] = make_double2(double(gx[3]), double(gx[9]));
+        d[2] = make_double2(double(gy[1]), double(gy[2]));
+        d[3] = make_double2(double(gy[3]), double(gy[9]));
+        d[4] = make_double2(double(gz[1]), double(gz[2]));
+        d[5] = make_double2(double(gz[3]), double(gz[9]));
     }
     if (out.grad)
     {
@@ -224,24 +224,24 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
             break;
         const int f = cfc[s];
         const bool own0 = m.fcell[2 * f] == c;
-        double st[8];
+        Real st[8];
         if (linear)
         {
             const double2 *g = reinterpret_cast<const double2 *>(m.fgB + (size_t)f * kFgB);
             const double2 g2 = g[2], g3 = g[3];
-            double r[3];
+            Real r[3];
             if (own0)
             {
-                r[0] = g2.y - xc0;
-                r[1] = g3.x - xc1;
-                r[2] = g3.y - xc2;
+                r[0] = Real(g2.y - xc0);
+                r[1] = Real(g3.x - xc1);
+                r[2] = Real(g3.y - xc2);
             }
             else
             {
                 const double *sh = m.shtab + (size_t)m.fsh[f] * 3;
-                r[0] = g2.y - (xc0 + sh[0]);
-                r[1] = g3.x - (xc1 + sh[1]);
-                r[2] = g3.y - (xc2 + sh[2]);
+                r[0] = Real(g2.y - (xc0 + sh[0]));
+                r[1] = Real(g3.x - (xc1 + sh[1]));
+                r[2] = Real(g3.y - (xc2 + sh[2]));
             }
 #pragma unroll
             for (int j = 0; j < 8; ++j)
@@ -254,10 +254,10 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
                 st[j] = pc[j];
         }
         double2 *d = reinterpret_cast<double2 *>(out.fs + (size_t)f * 16 + (own0 ? 0 : 8));
-        d[0] = make_double2(st[0], st[1]);
-        d[1] = make_double2(st[2], st[3]);
-        d[2] = make_double2(st[4], st[5]);
-        d[3] = make_double2(st[6], st[7]);
+        d[0] = make_double2(double(st[0]), double(st[1]));
+        d[1] = make_double2(double(st[2]), double(st[3]));
+        d[2] = make_double2(double(st[4]), double(st[5]));
+        d[3] = make_double2(double(st[6]), double(st[7]));
     }
 }
 
@@ -273,7 +273,7 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
 // FS: face states from K1 (reconstructed path); otherwise first order, the
 // face states are the cell values of U.
 // ---------------------------------------------------------------------------
-template <int FLUX, bool FS, bool VISC, int LANES>
+template <int FLUX, bool FS, bool VISC, int LANES, class Real>
 __global__ __launch_bounds__(kBlock, FVH_K2_MINW) void fv_residual_group_kernel(
     DevMesh m, PhysDev P, StageParams sp)
 {
@@ -295,7 +295,7 @@ __global__ __launch_bounds__(kBlock, FVH_K2_MINW) void fv_residual_group_kernel(
     }
 
     const int nf = cell_ok ? m.ncf[c] : 0;
-    double contrib[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    Real contrib[5] = {Real(0), Real(0), Real(0), Real(0), Real(0)};
     if (s < nf)
     {
         const int f = m.cf[(size_t)c * m.maxf + s];
@@ -303,10 +303,10 @@ __global__ __launch_bounds__(kBlock, FVH_K2_MINW) void fv_residual_group_kernel(
         const bool own0 = fc.x == c;
         const int nb = own0 ? fc.y : fc.x;
         const bool ghost = nb >= N;
-        Frame fr;
+        FrameT<Real> fr;
         load_frame(m, f, fr);
 
-        double L[10], R[10];
+        Real L[10], R[10];
         if (FS)
         {
             const double2 *p = reinterpret_cast<const double2 *>(sp.fs + (size_t)f * 16);
@@ -314,23 +314,23 @@ __global__ __launch_bounds__(kBlock, FVH_K2_MINW) void fv_residual_group_kernel(
             for (int q = 0; q < 4; ++q)
             {
                 const double2 a = p[q], b = p[4 + q];
-                L[2 * q] = a.x;
-                L[2 * q + 1] = a.y;
-                R[2 * q] = b.x;
-                R[2 * q + 1] = b.y;
+                L[2 * q] = Real(a.x);
+                L[2 * q + 1] = Real(a.y);
+                R[2 * q] = Real(b.x);
+                R[2 * q + 1] = Real(b.y);
             }
-            L[8] = L[9] = R[8] = R[9] = 0.0;
+            L[8] = L[9] = R[8] = R[9] = Real(0.0);
         }
         else
         {
-            double pc[10], pn[10];
+            Real pc[10], pn[10];
             load_phi(P, sp.U_in, sp.prims_in, c, pc);
             if (ghost)
             {
                 const double *gp = sp.gphi + (size_t)(nb - N) * 10;
 #pragma unroll
                 for (int j = 0; j < 10; ++j)
-                    pn[j] = gp[j];
+                    pn[j] = Real(gp[j]);
             }
             else
             {
@@ -344,10 +344,10 @@ __global__ __launch_bounds__(kBlock, FVH_K2_MINW) void fv_residual_group_kernel(
             }
         }
 
-        ViscGrad gL, gR;
+        ViscGradT<Real> gL, gR;
         if (VISC)
         {
-            ViscGrad go, gn;
+            ViscGradT<Real> go, gn;
             load_vg(sp.vg, c, go);
             if (ghost)
             {
@@ -369,26 +369,26 @@ __global__ __launch_bounds__(kBlock, FVH_K2_MINW) void fv_residual_group_kernel(
             }
         }
 
-        double flux[5];
+        Real flux[5];
         face_flux<FLUX, VISC>(P, fr, L, R, gL, gR, flux);
 #pragma unroll
         for (int j = 0; j < 5; ++j)
         {
-            const double v = flux[j] * fr.area;
+            const Real v = flux[j] * fr.area;
             contrib[j] = own0 ? v : -v;
         }
     }
 
     // ordered sum over the group's faces
-    double acc[5];
+    Real acc[5];
 #pragma unroll
     for (int j = 0; j < 5; ++j)
     {
-        double a = 0.0;
+        Real a = Real(0.0);
 #pragma unroll
         for (int k = 0; k < LANES; ++k)
         {
-            const double v = __shfl(contrib[j], k, LANES);
+            const Real v = __shfl(contrib[j], k, LANES);
             if (k < nf)
                 a = a + v;
         }
@@ -402,7 +402,7 @@ __global__ __launch_bounds__(kBlock, FVH_K2_MINW) void fv_residual_group_kernel(
         double R5[5];
 #pragma unroll
         for (int j = 0; j < 5; ++j)
-            R5[j] = -acc[j] / vol;
+            R5[j] = -double(acc[j]) / vol;
         if (sp.R_out)
         {
 #pragma unroll
@@ -715,7 +715,7 @@ void launch_ghost(hipStream_t s, const DevMesh &m, const PhysDev &P,
 
 void launch_gradient(hipStream_t s, const DevMesh &m, const PhysDev &P,
                      const double *U, const double *gphi, const GradOut &out_in,
-                     int linear, const DevScalars *sc)
+                     int linear, const DevScalars *sc, bool fp32)
 {
     check_faces(m);
     GradOut out = out_in;
@@ -727,33 +727,53 @@ void launch_gradient(hipStream_t s, const DevMesh &m, const PhysDev &P,
     if (out.c_begin < 0 || out.c_end > m.N || out.c_begin >= out.c_end)
         throw std::runtime_error("launch_gradient: bad cell range");
     const int grid = cell_blocks(out.c_end - out.c_begin);
+    if (fp32)
+    {
+        if (m.maxf <= 4)
+            hipLaunchKernelGGL((fv_gradient_kernel<4, float>), dim3(grid), dim3(kBlock),
+                               FVH_K1_LDS, s, m, P, U, gphi, out, linear, sc);
+        else
+            hipLaunchKernelGGL((fv_gradient_kernel<6, float>), dim3(grid), dim3(kBlock),
+                               FVH_K1_LDS, s, m, P, U, gphi, out, linear, sc);
+        return;
+    }
     if (m.maxf <= 4)
-        hipLaunchKernelGGL(fv_gradient_kernel<4>, dim3(grid),
+        hipLaunchKernelGGL((fv_gradient_kernel<4, double>), dim3(grid),
                            dim3(kBlock), FVH_K1_LDS, s, m, P, U, gphi, out, linear,
                            sc);
     else
-        hipLaunchKernelGGL(fv_gradient_kernel<6>, dim3(grid),
+        hipLaunchKernelGGL((fv_gradient_kernel<6, double>), dim3(grid),
                            dim3(kBlock), FVH_K1_LDS, s, m, P, U, gphi, out, linear,
                            sc);
 }
 
 template <int FLUX, bool FS, bool VISC>
 static void launch_res_t(hipStream_t s, const DevMesh &m, const PhysDev &P,
-                         const StageParams &sp)
+                         const StageParams &sp, bool fp32)
 {
     const int lanes = m.maxf <= 4 ? 4 : 8;
     const int grid = cell_blocks((sp.c_end - sp.c_begin) * lanes);
+    if (fp32)
+    {
+        if (m.maxf <= 4)
+            hipLaunchKernelGGL((fv_residual_group_kernel<FLUX, FS, VISC, 4, float>),
+                               dim3(grid), dim3(kBlock), 0, s, m, P, sp);
+        else
+            hipLaunchKernelGGL((fv_residual_group_kernel<FLUX, FS, VISC, 8, float>),
+                               dim3(grid), dim3(kBlock), 0, s, m, P, sp);
+        return;
+    }
     if (m.maxf <= 4)
-        hipLaunchKernelGGL((fv_residual_group_kernel<FLUX, FS, VISC, 4>),
+        hipLaunchKernelGGL((fv_residual_group_kernel<FLUX, FS, VISC, 4, double>),
                            dim3(grid), dim3(kBlock), 0, s, m, P, sp);
     else
-        hipLaunchKernelGGL((fv_residual_group_kernel<FLUX, FS, VISC, 8>),
+        hipLaunchKernelGGL((fv_residual_group_kernel<FLUX, FS, VISC, 8, double>),
                            dim3(grid), dim3(kBlock), 0, s, m, P, sp);
 }
 
 void launch_residual(hipStream_t s, const DevMesh &m, const PhysDev &P,
                      const StageParams &sp_in, int flux, bool face_states,
-                     bool viscous)
+                     bool viscous, bool fp32)
 {
     check_faces(m);
     StageParams sp = sp_in;
@@ -766,26 +786,28 @@ void launch_residual(hipStream_t s, const DevMesh &m, const PhysDev &P,
         throw std::runtime_error("launch_residual: bad cell range");
     if (sp.do_update && !sp.case_dt && !sp.sc)
         throw std::runtime_error("launch_residual: stage update without a time step");
+    if (fp32 && (sp.do_update || sp.do_reduce))
+        throw std::runtime_error("launch_residual: FP32 arithmetic is for residuals only");
     if (sp.do_reduce && cell_blocks((sp.c_end - sp.c_begin) * (m.maxf <= 4 ? 4 : 8)) >
                             sp.partials_cap)
         throw std::runtime_error("reduction partials buffer too small");
     if (flux == FluxRusanov)
     {
         if (face_states)
-            viscous ? launch_res_t<FluxRusanov, true, true>(s, m, P, sp)
-                    : launch_res_t<FluxRusanov, true, false>(s, m, P, sp);
+            viscous ? launch_res_t<FluxRusanov, true, true>(s, m, P, sp, fp32)
+                    : launch_res_t<FluxRusanov, true, false>(s, m, P, sp, fp32);
         else
-            viscous ? launch_res_t<FluxRusanov, false, true>(s, m, P, sp)
-                    : launch_res_t<FluxRusanov, false, false>(s, m, P, sp);
+            viscous ? launch_res_t<FluxRusanov, false, true>(s, m, P, sp, fp32)
+                    : launch_res_t<FluxRusanov, false, false>(s, m, P, sp, fp32);
     }
     else
     {
         if (face_states)
-            viscous ? launch_res_t<FluxAUSM, true, true>(s, m, P, sp)
-                    : launch_res_t<FluxAUSM, true, false>(s, m, P, sp);
+            viscous ? launch_res_t<FluxAUSM, true, true>(s, m, P, sp, fp32)
+                    : launch_res_t<FluxAUSM, true, false>(s, m, P, sp, fp32);
         else
-            viscous ? launch_res_t<FluxAUSM, false, true>(s, m, P, sp)
-                    : launch_res_t<FluxAUSM, false, false>(s, m, P, sp);
+            viscous ? launch_res_t<FluxAUSM, false, true>(s, m, P, sp, fp32)
+                    : launch_res_t<FluxAUSM, false, false>(s, m, P, sp, fp32);
     }
 }
 

@@ -2,6 +2,12 @@
 // Newton-Krylov kernels: the reference's EOS, boundary conditions, fluxes
 // and reductions restated operation by operation (compiled with
 // -ffp-contract=off everywhere).
+//
+// The arithmetic type Real is double for every path that is compared with
+// the reference (the instantiation is the plain FP64 code, constant for
+// constant); Real = float serves the FP32 Jacobian-vector products of the
+// mixed-precision Newton-Krylov solve.  Arrays in memory stay FP64 and are
+// converted on load.
 #pragma once
 
 #include <cfloat>
@@ -13,15 +19,15 @@ namespace fvh {
 namespace dev {
 
 // reference BM_MIN / BM_MAX / BM_ABS semantics (ternaries, not fmin/fmax)
-__device__ __forceinline__ double rmin(double a, double b) { return a < b ? a : b; }
-__device__ __forceinline__ double rmax(double a, double b) { return a > b ? a : b; }
-__device__ __forceinline__ double rabs(double a) { return a < 0 ? -a : a; }
+template <class Real> __device__ __forceinline__ Real rmin(Real a, Real b) { return a < b ? a : b; }
+template <class Real> __device__ __forceinline__ Real rmax(Real a, Real b) { return a > b ? a : b; }
+template <class Real> __device__ __forceinline__ Real rabs(Real a) { return a < 0 ? -a : a; }
 
 // basec dot_n: sequential sum from 0.0
-__device__ __forceinline__ double dot3(double a0, double a1, double a2,
-                                       double b0, double b1, double b2)
+template <class Real>
+__device__ __forceinline__ Real dot3(Real a0, Real a1, Real a2, Real b0, Real b1, Real b2)
 {
-    double s = 0.0;
+    Real s = Real(0.0);
     s += a0 * b0;
     s += a1 * b1;
     s += a2 * b2;
@@ -41,64 +47,70 @@ enum
 };
 
 // eos.c:81-92
-__device__ __forceinline__ void con_to_prim(const PhysDev &P, double *phi)
+template <class Real>
+__device__ __forceinline__ void con_to_prim(const PhysDev &P, Real *phi)
 {
     phi[5] = phi[1] / phi[0];
     phi[6] = phi[2] / phi[0];
     phi[7] = phi[3] / phi[0];
-    double p = P.kappa_m1 * (phi[4] - 0.5 * dot3(phi[1], phi[2], phi[3],
-                                                 phi[5], phi[6], phi[7]));
-    p = rmax(1.00E-10, p);
+    Real p = Real(P.kappa_m1) * (phi[4] - Real(0.5) * dot3(phi[1], phi[2], phi[3],
+                                                             phi[5], phi[6], phi[7]));
+    p = rmax(Real(1.00E-10), p);
     phi[8] = p;
-    phi[9] = p / (phi[0] * P.R);
+    phi[9] = p / (phi[0] * Real(P.R));
 }
 
 // eos.c:120-127
-__device__ __forceinline__ void prim_to_con(const PhysDev &P, double *phi)
+template <class Real>
+__device__ __forceinline__ void prim_to_con(const PhysDev &P, Real *phi)
 {
     phi[1] = phi[5] * phi[0];
     phi[2] = phi[6] * phi[0];
     phi[3] = phi[7] * phi[0];
-    phi[4] = P.s_kappa_m1 * phi[8] +
-             0.5 * dot3(phi[5], phi[6], phi[7], phi[1], phi[2], phi[3]);
+    phi[4] = Real(P.s_kappa_m1) * phi[8] +
+             Real(0.5) * dot3(phi[5], phi[6], phi[7], phi[1], phi[2], phi[3]);
 }
 
 // eos.c:55-75 (compression branch with the reference's A = 2/((k+1) u_n))
-__device__ __forceinline__ double riemann_p(const PhysDev &P, const double *phi)
+template <class Real>
+__device__ __forceinline__ Real riemann_p(const PhysDev &P, const Real *phi)
 {
-    if (phi[5] <= 0.0)
+    if (phi[5] <= Real(0.0))
     {
-        double c = sqrt(P.kappa * phi[8] / phi[0]);
-        return phi[8] * pow(rmax(1e-4, 1. + 0.5 * P.kappa_m1 * phi[5] / c),
-                            2 * P.kappa * P.s_kappa_m1);
+        Real c = sqrt(Real(P.kappa) * phi[8] / phi[0]);
+        return phi[8] * pow(rmax(Real(1e-4), Real(1.) + Real(0.5) * Real(P.kappa_m1) * phi[5] / c),
+                            Real(2) * Real(P.kappa) * Real(P.s_kappa_m1));
     }
-    double ar = 2 * P.s_kappa_p1 / phi[5];
-    double br = P.kappa_m1 * P.s_kappa_p1 * phi[8];
-    return phi[8] + phi[5] / ar * 0.5 *
-                        (phi[5] + sqrt(phi[5] * phi[5] + 4. * ar * (phi[8] + br)));
+    Real ar = Real(2) * Real(P.s_kappa_p1) / phi[5];
+    Real br = Real(P.kappa_m1) * Real(P.s_kappa_p1) * phi[8];
+    return phi[8] + phi[5] / ar * Real(0.5) *
+                        (phi[5] + sqrt(phi[5] * phi[5] + Real(4.) * ar * (phi[8] + br)));
 }
 
-struct Frame
+template <class Real> struct FrameT
 {
-    double n[3], t1[3], t2[3];
-    double area;
+    Real n[3], t1[3], t2[3];
+    Real area;
 };
+using Frame = FrameT<double>;
 
-__device__ __forceinline__ void load_frame(const DevMesh &m, int f, Frame &fr)
+template <class Real>
+__device__ __forceinline__ void load_frame(const DevMesh &m, int f, FrameT<Real> &fr)
 {
     const double2 *g = reinterpret_cast<const double2 *>(m.fgA + (size_t)f * kFgA);
     const double2 a = g[0], b = g[1], c = g[2], d = g[3], e = g[4];
-    fr.n[0] = a.x, fr.n[1] = a.y, fr.n[2] = b.x;
-    fr.t1[0] = b.y, fr.t1[1] = c.x, fr.t1[2] = c.y;
-    fr.t2[0] = d.x, fr.t2[1] = d.y, fr.t2[2] = e.x;
-    fr.area = e.y;
+    fr.n[0] = Real(a.x), fr.n[1] = Real(a.y), fr.n[2] = Real(b.x);
+    fr.t1[0] = Real(b.y), fr.t1[1] = Real(c.x), fr.t1[2] = Real(c.y);
+    fr.t2[0] = Real(d.x), fr.t2[1] = Real(d.y), fr.t2[2] = Real(e.x);
+    fr.area = Real(e.y);
 }
 
-__device__ __forceinline__ void rotate_back(const Frame &fr, double *g)
+template <class Real>
+__device__ __forceinline__ void rotate_back(const FrameT<Real> &fr, Real *g)
 {
-    double tu = g[5] * fr.n[0] + g[6] * fr.t1[0] + g[7] * fr.t2[0];
-    double tv = g[5] * fr.n[1] + g[6] * fr.t1[1] + g[7] * fr.t2[1];
-    double tw = g[5] * fr.n[2] + g[6] * fr.t1[2] + g[7] * fr.t2[2];
+    Real tu = g[5] * fr.n[0] + g[6] * fr.t1[0] + g[7] * fr.t2[0];
+    Real tv = g[5] * fr.n[1] + g[6] * fr.t1[1] + g[7] * fr.t2[1];
+    Real tw = g[5] * fr.n[2] + g[6] * fr.t1[2] + g[7] * fr.t2[2];
     g[5] = tu;
     g[6] = tv;
     g[7] = tw;
@@ -106,9 +118,9 @@ __device__ __forceinline__ void rotate_back(const Frame &fr, double *g)
 
 // Ghost state of boundary row N+b from the interior state pc
 // (boundary.c:202-319).
-__device__ __forceinline__ void ghost_state(const DevMesh &m, const PhysDev &P,
-                                            int b, const Frame &fr,
-                                            const double *pc, double *g)
+template <class Real>
+__device__ __forceinline__ void ghost_state(const DevMesh &m, const PhysDev &P, int b,
+                                            const FrameT<Real> &fr, const Real *pc, Real *g)
 {
     const int region = m.bnd_region[b];
     const int type = m.region_type[region];
@@ -119,7 +131,7 @@ __device__ __forceinline__ void ghost_state(const DevMesh &m, const PhysDev &P,
     {
 #pragma unroll
         for (int j = 0; j < 10; ++j)
-            g[j] = m.region_phi[region * 10 + j];
+            g[j] = Real(m.region_phi[region * 10 + j]);
     }
     else if (type == BC_WALL_ADIABATIC || type == BC_WALL_ISOTHERMAL)
     {
@@ -127,12 +139,12 @@ __device__ __forceinline__ void ghost_state(const DevMesh &m, const PhysDev &P,
         g[6] = dot3(pc[5], pc[6], pc[7], fr.t1[0], fr.t1[1], fr.t1[2]);
         g[7] = dot3(pc[5], pc[6], pc[7], fr.t2[0], fr.t2[1], fr.t2[2]);
         g[8] = riemann_p(P, g);
-        g[5] = 0.0;
-        g[6] = 0.0;
-        g[7] = 0.0;
+        g[5] = Real(0.0);
+        g[6] = Real(0.0);
+        g[7] = Real(0.0);
         if (type == BC_WALL_ISOTHERMAL)
-            g[9] = m.region_phi[region * 10 + 9];
-        g[0] = g[8] / (P.R * g[9]);
+            g[9] = Real(m.region_phi[region * 10 + 9]);
+        g[0] = g[8] / (Real(P.R) * g[9]);
         rotate_back(fr, g);
     }
     else if (type == BC_WALL_SLIP || type == BC_SYMMETRY)
@@ -141,8 +153,8 @@ __device__ __forceinline__ void ghost_state(const DevMesh &m, const PhysDev &P,
         g[6] = dot3(pc[5], pc[6], pc[7], fr.t1[0], fr.t1[1], fr.t1[2]);
         g[7] = dot3(pc[5], pc[6], pc[7], fr.t2[0], fr.t2[1], fr.t2[2]);
         g[8] = riemann_p(P, g);
-        g[5] = 0.0;
-        g[9] = g[8] / (g[0] * P.R);
+        g[5] = Real(0.0);
+        g[9] = g[8] / (g[0] * Real(P.R));
         rotate_back(fr, g);
     }
     prim_to_con(P, g);
@@ -150,15 +162,15 @@ __device__ __forceinline__ void ghost_state(const DevMesh &m, const PhysDev &P,
 
 // Ghost gradient (boundary.c:361-406): a copy of the interior gradient, with
 // the normal component removed for slip walls and symmetry.
-__device__ __forceinline__ void ghost_grad(int type, const Frame &fr, double gx,
-                                           double gy, double gz, double &ox,
-                                           double &oy, double &oz)
+template <class Real>
+__device__ __forceinline__ void ghost_grad(int type, const FrameT<Real> &fr, Real gx, Real gy,
+                                           Real gz, Real &ox, Real &oy, Real &oz)
 {
     if (type == BC_WALL_SLIP || type == BC_SYMMETRY)
     {
-        double ty = gx * fr.t1[0] + gy * fr.t1[1] + gz * fr.t1[2];
-        double tz = gx * fr.t2[0] + gy * fr.t2[1] + gz * fr.t2[2];
-        double tx = 0.0;
+        Real ty = gx * fr.t1[0] + gy * fr.t1[1] + gz * fr.t1[2];
+        Real tz = gx * fr.t2[0] + gy * fr.t2[1] + gz * fr.t2[2];
+        Real tx = Real(0.0);
         ox = tx * fr.n[0] + ty * fr.t1[0] + tz * fr.t2[0];
         oy = tx * fr.n[1] + ty * fr.t1[1] + tz * fr.t2[1];
         oz = tx * fr.n[2] + ty * fr.t1[2] + tz * fr.t2[2];
@@ -171,21 +183,21 @@ __device__ __forceinline__ void ghost_grad(int type, const Frame &fr, double gx,
     }
 }
 
-__device__ __forceinline__ void load_cons(const double *__restrict__ U, int c,
-                                          double *phi)
+template <class Real>
+__device__ __forceinline__ void load_cons(const double *__restrict__ U, int c, Real *phi)
 {
     const double *p = U + (size_t)c * 5;
 #pragma unroll
     for (int j = 0; j < 5; ++j)
-        phi[j] = p[j];
+        phi[j] = Real(p[j]);
 }
 
 // Conservative state and its primitives: read from the primitive rows
 // [N][5] (u, v, w, p, T) when they are kept beside the state (the values
 // con_to_prim produced from the same state, so bit-identical), else computed.
+template <class Real>
 __device__ __forceinline__ void load_phi(const PhysDev &P, const double *__restrict__ U,
-                                         const double *__restrict__ prims, int c,
-                                         double *phi)
+                                         const double *__restrict__ prims, int c, Real *phi)
 {
     load_cons(U, c, phi);
     if (prims)
@@ -193,7 +205,7 @@ __device__ __forceinline__ void load_phi(const PhysDev &P, const double *__restr
         const double *q = prims + (size_t)c * 5;
 #pragma unroll
         for (int j = 0; j < 5; ++j)
-            phi[5 + j] = q[j];
+            phi[5 + j] = Real(q[j]);
     }
     else
         con_to_prim(P, phi);
@@ -213,45 +225,46 @@ __device__ __forceinline__ int xcd_tile(int nblk)
 
 // Viscous gradient subset: momentum (vars 1,2,3) and T (9) per direction,
 // v[d*4 + q] = d/dx_d of var {1,2,3,9}[q].
-struct ViscGrad
+template <class Real> struct ViscGradT
 {
-    double v[12];
+    Real v[12];
 };
+using ViscGrad = ViscGradT<double>;
 
 __device__ __forceinline__ int visc_var(int q) { return q < 3 ? q + 1 : 9; }
 
-__device__ __forceinline__ void load_vg(const double *__restrict__ vg, int c,
-                                        ViscGrad &g)
+template <class Real>
+__device__ __forceinline__ void load_vg(const double *__restrict__ vg, int c, ViscGradT<Real> &g)
 {
     const double2 *p = reinterpret_cast<const double2 *>(vg + (size_t)c * 12);
 #pragma unroll
     for (int q = 0; q < 6; ++q)
     {
         const double2 v = p[q];
-        g.v[2 * q] = v.x;
-        g.v[2 * q + 1] = v.y;
+        g.v[2 * q] = Real(v.x);
+        g.v[2 * q + 1] = Real(v.y);
     }
 }
 
 // Limited linear extrapolation to a face (reconstruction.c:193-200 with
 // limiter.c:86-105: min(1, y) of the per-face-side Barth-Jespersen ratio).
-__device__ __forceinline__ double limited(double phi, double gx, double gy,
-                                          double gz, const double *r, double mn,
-                                          double mx, int bj)
+template <class Real>
+__device__ __forceinline__ Real limited(Real phi, Real gx, Real gy, Real gz, const Real *r,
+                                        Real mn, Real mx, int bj)
 {
-    const double slope = r[0] * gx + r[1] * gy + r[2] * gz;
-    double lim = 1.0;
+    const Real slope = r[0] * gx + r[1] * gy + r[2] * gz;
+    Real lim = Real(1.0);
     if (bj)
     {
         // y >= 1 exactly when the numerator reaches the slope (division is
         // monotonic), and then min(1, y) = 1: the divide is only needed
         // below that
-        double y = 1.0;
-        if (slope > 0 && mx - phi < slope)
+        Real y = Real(1.0);
+        if (slope > Real(0) && mx - phi < slope)
             y = (mx - phi) / slope;
-        else if (slope < 0 && mn - phi > slope)
+        else if (slope < Real(0) && mn - phi > slope)
             y = (mn - phi) / slope;
-        lim = rmin(1.0, y);
+        lim = rmin(Real(1.0), y);
     }
     return phi + lim * slope;
 }
@@ -261,7 +274,8 @@ __device__ __forceinline__ double limited(double phi, double gx, double gy,
 // ---------------------------------------------------------------------------
 
 // flux.c:203-210
-__device__ __forceinline__ void euler_flux_1d(const double *phi, double *f)
+template <class Real>
+__device__ __forceinline__ void euler_flux_1d(const Real *phi, Real *f)
 {
     f[0] = phi[1];
     f[1] = phi[1] * phi[5] + phi[8];
@@ -271,63 +285,65 @@ __device__ __forceinline__ void euler_flux_1d(const double *phi, double *f)
 }
 
 // flux.c:320-336
-__device__ __forceinline__ void riemann_rusanov(const PhysDev &P, const double *l,
-                                                const double *r, double *f)
+template <class Real>
+__device__ __forceinline__ void riemann_rusanov(const PhysDev &P, const Real *l, const Real *r,
+                                                Real *f)
 {
-    const double c_l = sqrt(P.kappa * l[8] / l[0]);
-    const double c_r = sqrt(P.kappa * r[8] / r[0]);
-    const double eig = rmax(rabs(l[5]) + c_l, rabs(r[5]) + c_r);
-    double fl[5], fr[5];
+    const Real c_l = sqrt(Real(P.kappa) * l[8] / l[0]);
+    const Real c_r = sqrt(Real(P.kappa) * r[8] / r[0]);
+    const Real eig = rmax(rabs(l[5]) + c_l, rabs(r[5]) + c_r);
+    Real fl[5], fr[5];
     euler_flux_1d(l, fl);
     euler_flux_1d(r, fr);
 #pragma unroll
     for (int j = 0; j < 5; ++j)
-        f[j] = 0.5 * (fl[j] + fr[j]) - 0.5 * eig * (r[j] - l[j]);
+        f[j] = Real(0.5) * (fl[j] + fr[j]) - Real(0.5) * eig * (r[j] - l[j]);
 }
 
 // flux.c:344-406
-__device__ __forceinline__ void riemann_ausm(const PhysDev &P, const double *l,
-                                             const double *r, double *f)
+template <class Real>
+__device__ __forceinline__ void riemann_ausm(const PhysDev &P, const Real *l, const Real *r,
+                                             Real *f)
 {
-    const double c_l = sqrt(P.kappa * l[8] / l[0]);
-    const double c_r = sqrt(P.kappa * r[8] / r[0]);
-    const double M_l = l[5] / c_l;
-    const double M_r = r[5] / c_r;
-    const double H_l = (l[4] + l[8]) / l[0];
-    const double H_r = (r[4] + r[8]) / r[0];
-    double M_p, P_p, M_m, P_m;
-    if (M_l <= -1.0)
+    const Real c_l = sqrt(Real(P.kappa) * l[8] / l[0]);
+    const Real c_r = sqrt(Real(P.kappa) * r[8] / r[0]);
+    const Real M_l = l[5] / c_l;
+    const Real M_r = r[5] / c_r;
+    const Real H_l = (l[4] + l[8]) / l[0];
+    const Real H_r = (r[4] + r[8]) / r[0];
+    Real M_p, P_p, M_m, P_m;
+    if (M_l <= Real(-1.0))
     {
-        M_p = 0.0;
-        P_p = 0.0;
+        M_p = Real(0.0);
+        P_p = Real(0.0);
     }
-    else if (M_l < 1.0)
+    else if (M_l < Real(1.0))
     {
-        M_p = 0.25 * (M_l + 1.0) * (M_l + 1.0);
-        P_p = 0.25 * l[8] * (1.0 + M_l) * (1.0 + M_l) * (2.0 - M_l);
+        M_p = Real(0.25) * (M_l + Real(1.0)) * (M_l + Real(1.0));
+        P_p = Real(0.25) * l[8] * (Real(1.0) + M_l) * (Real(1.0) + M_l) * (Real(2.0) - M_l);
     }
     else
     {
         M_p = M_l;
         P_p = l[8];
     }
-    if (M_r <= -1.0)
+    if (M_r <= Real(-1.0))
     {
         M_m = M_r;
         P_m = r[8];
     }
-    else if (M_r < 1.0)
+    else if (M_r < Real(1.0))
     {
-        M_m = -0.25 * (M_r - 1.0) * (M_r - 1.0);
-        P_m = 0.25 * r[8] * (1.0 - M_r) * (1.0 - M_r) * (2.0 + M_r);
+        M_m = Real(-0.25) * (M_r - Real(1.0)) * (M_r - Real(1.0));
+        P_m = Real(0.25) * r[8] * (Real(1.0) - M_r) * (Real(1.0) - M_r) * (Real(2.0) + M_r);
     }
     else
     {
-        M_m = 0.0;
-        P_m = 0.0;
+        M_m = Real(0.0);
+        P_m = Real(0.0);
     }
-    const double mp = rmax(0.0, M_p + M_m);
-    const double mm = rmin(0.0, M_p + M_m);
+    const Real mp = rmax(Real(0.0), M_p + M_m);
+    const Real mm = rmin(Real(0.0), M_p + M_m);
     f[0] = mp * c_l * l[0];
     f[1] = mp * c_l * l[0] * l[5] + P_p;
     f[2] = mp * c_l * l[0] * l[6];
@@ -342,47 +358,48 @@ __device__ __forceinline__ void riemann_ausm(const PhysDev &P, const double *l,
 
 // flux.c:215-263 with the reference's use of momentum gradients as velocity
 // gradients; phi holds the face state (reconstructed primitives at 5..7).
-__device__ __forceinline__ void viscous_1d(const PhysDev &P, const double *phi,
-                                           const ViscGrad &g, double *f,
-                                           double *gg, double *h)
+template <class Real>
+__device__ __forceinline__ void viscous_1d(const PhysDev &P, const Real *phi,
+                                           const ViscGradT<Real> &g, Real *f, Real *gg, Real *h)
 {
-    const double s_23 = 2.0 / 3.0;
-    const double s_43 = 4.0 / 3.0;
-    const double mu = P.mu;
-    const double gx1 = g.v[0], gx2 = g.v[1], gx3 = g.v[2], gxT = g.v[3];
-    const double gy1 = g.v[4], gy2 = g.v[5], gy3 = g.v[6], gyT = g.v[7];
-    const double gz1 = g.v[8], gz2 = g.v[9], gz3 = g.v[10], gzT = g.v[11];
-    const double tau_xx = mu * (s_43 * gx1 - s_23 * gy2 - s_23 * gz3);
-    const double tau_yy = mu * (-s_23 * gx1 + s_43 * gy2 - s_23 * gz3);
-    const double tau_zz = mu * (-s_23 * gx1 - s_23 * gy2 + s_43 * gz3);
-    const double tau_xy = mu * (gy1 + gx2);
-    const double tau_xz = mu * (gz1 + gx3);
-    const double tau_yz = mu * (gz2 + gy3);
-    f[0] = 0.0;
+    const Real s_23 = Real(2.0) / Real(3.0);
+    const Real s_43 = Real(4.0) / Real(3.0);
+    const Real mu = Real(P.mu);
+    const Real lam = Real(P.lam);
+    const Real gx1 = g.v[0], gx2 = g.v[1], gx3 = g.v[2], gxT = g.v[3];
+    const Real gy1 = g.v[4], gy2 = g.v[5], gy3 = g.v[6], gyT = g.v[7];
+    const Real gz1 = g.v[8], gz2 = g.v[9], gz3 = g.v[10], gzT = g.v[11];
+    const Real tau_xx = mu * (s_43 * gx1 - s_23 * gy2 - s_23 * gz3);
+    const Real tau_yy = mu * (-s_23 * gx1 + s_43 * gy2 - s_23 * gz3);
+    const Real tau_zz = mu * (-s_23 * gx1 - s_23 * gy2 + s_43 * gz3);
+    const Real tau_xy = mu * (gy1 + gx2);
+    const Real tau_xz = mu * (gz1 + gx3);
+    const Real tau_yz = mu * (gz2 + gy3);
+    f[0] = Real(0.0);
     f[1] = -tau_xx;
     f[2] = -tau_xy;
     f[3] = -tau_xz;
-    f[4] = -tau_xx * phi[5] - tau_xy * phi[6] - tau_xz * phi[7] - P.lam * gxT;
-    gg[0] = 0.0;
+    f[4] = -tau_xx * phi[5] - tau_xy * phi[6] - tau_xz * phi[7] - lam * gxT;
+    gg[0] = Real(0.0);
     gg[1] = -tau_xy;
     gg[2] = -tau_yy;
     gg[3] = -tau_yz;
-    gg[4] = -tau_xy * phi[5] - tau_yy * phi[6] - tau_yz * phi[7] - P.lam * gyT;
-    h[0] = 0.0;
+    gg[4] = -tau_xy * phi[5] - tau_yy * phi[6] - tau_yz * phi[7] - lam * gyT;
+    h[0] = Real(0.0);
     h[1] = -tau_xz;
     h[2] = -tau_yz;
     h[3] = -tau_zz;
-    h[4] = -tau_xz * phi[5] - tau_yz * phi[6] - tau_zz * phi[7] - P.lam * gzT;
+    h[4] = -tau_xz * phi[5] - tau_yz * phi[6] - tau_zz * phi[7] - lam * gzT;
 }
 
 // One face flux in the face's own orientation (flux.c:42-111/113-197).
-template <int FLUX, bool VISC>
-__device__ __forceinline__ void face_flux(const PhysDev &P, const Frame &fr,
-                                          const double *L, const double *R,
-                                          const ViscGrad &gL, const ViscGrad &gR,
-                                          double *out)
+template <int FLUX, bool VISC, class Real>
+__device__ __forceinline__ void face_flux(const PhysDev &P, const FrameT<Real> &fr,
+                                          const Real *L, const Real *R,
+                                          const ViscGradT<Real> &gL, const ViscGradT<Real> &gR,
+                                          Real *out)
 {
-    double lr[10], rr[10];
+    Real lr[10], rr[10];
     lr[0] = L[0];
     lr[1] = dot3(L[1], L[2], L[3], fr.n[0], fr.n[1], fr.n[2]);
     lr[2] = dot3(L[1], L[2], L[3], fr.t1[0], fr.t1[1], fr.t1[2]);
@@ -396,7 +413,7 @@ __device__ __forceinline__ void face_flux(const PhysDev &P, const Frame &fr,
     rr[4] = R[4];
     con_to_prim(P, rr);
 
-    double fc[5];
+    Real fc[5];
     if (FLUX == FluxRusanov)
         riemann_rusanov(P, lr, rr, fc);
     else
@@ -410,15 +427,15 @@ __device__ __forceinline__ void face_flux(const PhysDev &P, const Frame &fr,
 
     if (VISC)
     {
-        double fl[5], gl[5], hl[5], fr5[5], gr[5], hr[5];
+        Real fl[5], gl[5], hl[5], fr5[5], gr[5], hr[5];
         viscous_1d(P, L, gL, fl, gl, hl);
         viscous_1d(P, R, gR, fr5, gr, hr);
 #pragma unroll
         for (int j = 0; j < 5; ++j)
         {
-            const double fx = 0.5 * (fl[j] + fr5[j]);
-            const double fy = 0.5 * (gl[j] + gr[j]);
-            const double fz = 0.5 * (hl[j] + hr[j]);
+            const Real fx = Real(0.5) * (fl[j] + fr5[j]);
+            const Real fy = Real(0.5) * (gl[j] + gr[j]);
+            const Real fz = Real(0.5) * (hl[j] + hr[j]);
             out[j] += fx * fr.n[0] + fy * fr.n[1] + fz * fr.n[2];
         }
     }

@@ -26,7 +26,7 @@ struct Workspace
     int N = 0, m = 0, prec = -1;
     size_t ld = 0;
     std::vector<void *> allocs;
-    double *R0 = nullptr, *Rt = nullptr, *Ut = nullptr, *dtau = nullptr,
+    double *R0 = nullptr, *R0j = nullptr, *Rt = nullptr, *Ut = nullptr, *dtau = nullptr,
            *Minv = nullptr, *xacc = nullptr, *dU = nullptr, *partials = nullptr,
            *dots = nullptr, *ydev = nullptr, *Mex = nullptr;
     void *V = nullptr, *z = nullptr, *t = nullptr; // T storage
@@ -71,6 +71,7 @@ void prepare(int N, int m, int prec)
     g_ws.ld = (n + 63) / 64 * 64;
     const size_t es = prec == FVH_JFNK_FP32 ? sizeof(float) : sizeof(double);
     g_ws.R0 = walloc<double>(n);
+    g_ws.R0j = prec == FVH_JFNK_FP32 ? walloc<double>(n) : g_ws.R0;
     g_ws.Rt = walloc<double>(n);
     g_ws.Ut = walloc<double>(n);
     g_ws.dtau = walloc<double>(N);
@@ -133,12 +134,15 @@ GmresResult gmres(const RuntimeView &v, const fvh_jfnk_options_t &o, double fd_n
     FVH_HIP_CHECK(hipMemsetAsync(g_ws.xacc, 0, (size_t)n * sizeof(double), s));
     double beta0 = 0.0;
 
-    // w = A z into out, with |z|^2 already in dots[kZZ]
+    // w = A z into out, with |z|^2 already in dots[kZZ]; in FP32 mode the
+    // difference quotient takes two FP32-arithmetic residuals (R0j is the
+    // FP32 residual of U)
+    const bool fp32 = o.precision == FVH_JFNK_FP32;
     auto apply_op = [&](const T *zin, T *out) {
         kry::fd_eps(s, dots + kZZ, fd_num, dots + kEps);
         kry::perturb<T>(s, n, v.U, dots + kEps, zin, g_ws.Ut);
-        eval_residual(g_ws.Ut, g_ws.Rt);
-        kry::op_finish<T>(s, N, zin, g_ws.dtau, g_ws.Rt, g_ws.R0, dots + kEps, out);
+        eval_residual(g_ws.Ut, g_ws.Rt, fp32);
+        kry::op_finish<T>(s, N, zin, g_ws.dtau, g_ws.Rt, g_ws.R0j, dots + kEps, out);
         ++res.evals;
     };
 
@@ -323,7 +327,13 @@ int solve(const fvh_jfnk_options_t &o, fvh_jfnk_stats_t &st)
             std::chrono::duration<double>(std::chrono::steady_clock::now() - tp).count();
         kry::multidot<double>(s, n, 1, v.U, g_ws.ld, v.U, g_ws.partials, g_ws.dots + kUU);
         fetch(s, kUU, 1);
-        const double fd_num = o.fd_eps * (1.0 + std::sqrt(g_ws.host[kUU]));
+        const bool fp32 = o.precision == FVH_JFNK_FP32;
+        const double fd_num = (fp32 ? o.fd_eps32 : o.fd_eps) * (1.0 + std::sqrt(g_ws.host[kUU]));
+        if (fp32)
+        {
+            eval_residual(v.U, g_ws.R0j, true);
+            ++st.residual_evals;
+        }
         GmresResult gr = o.precision == FVH_JFNK_FP32 ? gmres<float>(v, o, fd_num)
                                                       : gmres<double>(v, o, fd_num);
         st.gmres_iterations += gr.its;
@@ -387,6 +397,7 @@ int fvh_jfnk_default_options(fvh_jfnk_options_t *o)
     o->max_restarts = 3;
     o->rtol = 1e-2;
     o->fd_eps = 1e-7;
+    o->fd_eps32 = 3e-4;
     o->max_change = 0.2;
     o->precision = FVH_JFNK_FP64;
     o->precond = FVH_PRECOND_BLOCK;

@@ -55,8 +55,9 @@ struct StageBuffers
 StageBuffers stage_buffers();
 
 // Enqueues R = dU/dt(U) (finite_volume_time_derivative semantics, the
-// reference's ghost rows recomputed from U) on the runtime stream.
-void eval_residual(const double *U, double *R);
+// reference's ghost rows recomputed from U) on the runtime stream.  fp32:
+// the gradient and flux arithmetic in FP32 (for Jacobian-vector products).
+void eval_residual(const double *U, double *R, bool fp32 = false);
 
 // The current state was changed outside the time loop: the cached residual
 // and host mirror are stale.

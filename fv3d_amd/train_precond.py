@@ -1,7 +1,8 @@
 """Train the learned block preconditioner on the solver's own data.
 
-Training set: bump-channel solves (the JFNK benchmark's geometry) at Mach
-numbers drawn uniformly from [0.3, 0.7] with jitter seeds 1..n_solves.  Each
+Training set: bump-channel solves (the JFNK benchmark's geometry at several
+resolutions, the benchmark's own among them) at Mach numbers drawn uniformly
+from [0.3, 0.7] with jitter seeds 1..n_solves.  Each
 solve starts from the free stream and runs block-Jacobi JFNK in short
 bursts; after each burst the features and the block-Jacobi targets of the
 current state are appended to the device training set (fvh_nn_collect).
@@ -45,17 +46,23 @@ def collect_solve(n, seed, mach, bursts, burst_steps):
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--solves", type=int, default=200)
-    ap.add_argument("--mesh", default="16,4,3", help="hexahedra nx,ny,nz of the channel")
-    ap.add_argument("--bursts", type=int, default=12)
-    ap.add_argument("--burst-steps", type=int, default=3)
-    ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--mesh", default="16,4,3;32,8,3;64,16,6;160,40,13",
+                    help="channel meshes (hexahedra nx,ny,nz; ';'-separated), used in turn "
+                         "by the solves in the proportions of --mesh-weights")
+    ap.add_argument("--mesh-weights", default="10,6,3,1")
+    ap.add_argument("--bursts", type=int, default=6)
+    ap.add_argument("--burst-steps", type=int, default=4)
+    ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--holdout", type=int, default=4)
     ap.add_argument("--out", default=learned.PRETRAINED)
     a = ap.parse_args(argv)
-    n = tuple(int(x) for x in a.mesh.split(","))
+    meshes = [tuple(int(x) for x in m.split(",")) for m in a.mesh.split(";")]
+    weights = [int(w) for w in a.mesh_weights.split(",")][:len(meshes)]
+    cycle = [m for m, w in zip(meshes, weights) for _ in range(w)]
+    n = meshes[0]
     rng = np.random.default_rng(a.seed)
     machs = rng.uniform(0.3, 0.7, a.solves + a.holdout)
     t0 = time.time()
@@ -66,7 +73,8 @@ def main(argv=None):
     first.close()
     total = 0
     for i in range(a.solves):
-        total = collect_solve(n, i + 1, float(machs[i]), a.bursts, a.burst_steps)
+        total = collect_solve(cycle[i % len(cycle)], i + 1, float(machs[i]), a.bursts,
+                              a.burst_steps)
         if (i + 1) % 20 == 0 or i + 1 == a.solves:
             print("solves %d  samples %d  %.1f s" % (i + 1, total, time.time() - t0), flush=True)
     t1 = time.time()
@@ -79,7 +87,7 @@ def main(argv=None):
     errs = []
     for j in range(a.holdout):
         seed = 10000 + j
-        case = channel_case(n, seed, float(machs[a.solves + j]))
+        case = channel_case(meshes[j % len(meshes)], seed, float(machs[a.solves + j]))
         case.jfnk(max_newton=10, tol=1e-8, precond="block")
         for cfl in (10.0, 1e3):
             errs.append(learned.validate(case, cfl))

@@ -123,8 +123,9 @@ int fvh_device_info(char *name, int name_len, int *cu_count,
  * No counterpart in the reference (its implicit mode is the lumped-Jacobian
  * block solve of init_implicit / timedisc). */
 #define FVH_JFNK_FP64 0      /* Krylov vectors and J v in FP64 */
-#define FVH_JFNK_FP32 1      /* Krylov vectors and J v results stored in FP32;
-                                residuals, dots, update in FP64 */
+#define FVH_JFNK_FP32 1      /* Krylov vectors in FP32 and J v from two FP32-
+                                arithmetic residuals (step fd_eps32); Newton
+                                residuals, dots and the update in FP64 */
 #define FVH_PRECOND_SCALAR 0 /* M = I / dtau */
 #define FVH_PRECOND_BLOCK 1  /* block Jacobi: exact 5x5 blocks of the
                                 first-order inviscid residual */
@@ -148,6 +149,8 @@ typedef struct
     int max_restarts;    /* GMRES cycles per Newton step minus one (default 3) */
     double rtol;         /* GMRES relative tolerance (default 1e-2) */
     double fd_eps;       /* finite-difference scale (default 1e-7) */
+    double fd_eps32;     /* the same for FP32 J v (default 3e-4 ~ sqrt of the
+                            FP32 unit roundoff) */
     double max_change;   /* damping: largest relative change of rho and
                             rho*E per step (default 0.2) */
     int precision;       /* FVH_JFNK_FP64 / FVH_JFNK_FP32 */
