@@ -1,0 +1,98 @@
+"""The JSON driver reads the reference's parameter-file layout (values under
+"value", boundary regions with type and u, v, w, p, T) into the case
+configuration (CPU), and a run from a parameter file equals the same case
+built directly, writing reference-layout output files that restart (GPU)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from fv3d_amd import driver
+from fv3d_amd import solver as S
+from fv3d_amd._lib import lib
+
+
+def _params(mesh_file, timestep="Explicit", i_output=0, restart=None, max_iter=6):
+    p = {
+        "General": {"title": {"value": "drv"}},
+        "Mesh": {"mesh_file": {"value": mesh_file}},
+        "Equation": {"equation": {"value": "Navier-Stokes"}, "Navier-Stokes": {
+            "cfl_scale": {"value": 0.5}, "dfl_scale": {"value": 1.0},
+            "mu_mix": {"value": 0.0}, "R_mix": {"value": 1.0}, "Pr": {"value": 0.718},
+            "kappa": {"value": 1.4},
+            "Flux": {"flux_scheme": {"value": "Rusanov"}},
+            "Boundary": {
+                "flow": {"type": "flow", "u": 0.5, "v": 0.0, "w": 0.0, "p": 1.0, "T": 1.0},
+                "inflow": {"type": "inflow", "u": 0.5, "v": 0.0, "w": 0.0, "p": 1.0, "T": 1.0},
+                "outflow": {"type": "outflow"},
+                "bottom": {"type": "wall-slip"}, "top": {"type": "wall-slip"},
+                "zmin": {"type": "symmetry"}, "zmax": {"type": "symmetry"}}}},
+        "FV": {"Reconstruction": {"reconstruction": {"value": "Linear"}},
+               "Limiter": {"limiter": {"value": "Barth-Jespersenn"}}},
+        "Output": {"i_output_data": {"value": i_output}},
+        "Restart": {"use_restart": {"value": restart is not None},
+                    "restart_file": {"value": restart or ""}},
+        "TimeDisc": {"timestep": {"value": timestep}, "max_iter": {"value": max_iter},
+                     "transient": {"value": True}, "abort_residual": {"value": 1e-10},
+                     "t_start": {"value": 0.0}, "t_end": {"value": 1e300},
+                     "Explicit": {"scheme": {"value": "RK3-3"}},
+                     "Implicit": {"scheme": {"value": "BDF-2"}, "max_iter": {"value": 50},
+                                  "solver": {"value": "GMRes"}}}}
+    return p
+
+
+def test_config_from_reference_json(tmp_path):
+    cfg, restart = driver.config_from_json(_params("m.h5", "Implicit"), str(tmp_path))
+    assert cfg["mesh"] == ("file", os.path.join(str(tmp_path), "m.h5"))
+    assert cfg["physics"]["flux"] == "Rusanov" and cfg["physics"]["R_mix"] == 1.0
+    assert cfg["regions"]["inflow"] == ("inflow", (0.5, 0.0, 0.0, 1.0, 1.0))
+    assert cfg["regions"]["outflow"] == ("outflow", None)
+    n = cfg["numerics"]
+    assert n["timestep"] == "Implicit" and n["scheme"] == "BDF2" and n["max_iter_inner"] == 50
+    assert n["reconstruction"] == "Linear" and n["limiter"] == "Barth-Jespersenn"
+    assert cfg["title"] == "drv" and restart is None
+
+
+@pytest.mark.gpu
+def test_driver_run_equals_direct_case(tmp_path):
+    L = lib()
+    L.init_solver(b"drv")
+    S.check(L.fvh_mesh_channel(12, 4, 2, 0.05, 3))
+    ext = ".h5" if L.fvh_hdf5_available() else ".fvm"
+    mesh = os.path.join(str(tmp_path), "mesh" + ext)
+    S.check(L.fvh_mesh_write(mesh.encode()))
+    pfile = os.path.join(str(tmp_path), "case.json")
+    cwd = os.getcwd()
+    os.chdir(str(tmp_path))
+    try:
+        with open(pfile, "w") as f:
+            json.dump(_params("mesh" + ext, i_output=3), f)
+        cfg, _ = driver.config_from_json(json.load(open(pfile)), str(tmp_path))
+        case = S.Case(**cfg)
+        rc, _ = case.run()
+        U_direct = case.state()
+        case.close()
+        assert driver.main([pfile, "--quiet"]) == 0
+        out = "drv.%09d%s" % (6, ".h5" if ext == ".h5" else ".fvs")
+        assert os.path.exists(out)
+        # restart from the step-3 file and run to step 6 again
+        first = "drv.%09d%s" % (3, ".h5" if ext == ".h5" else ".fvs")
+        with open(pfile, "w") as f:
+            json.dump(_params("mesh" + ext, i_output=0, restart=first, max_iter=3), f)
+        cfg2, restart = driver.config_from_json(json.load(open(pfile)), str(tmp_path))
+        case = S.Case(**cfg2)
+        L.init_restart(1, restart.encode())
+        rc2, log = case.run()
+        U_restart = case.state()
+        case.close()
+    finally:
+        os.chdir(cwd)
+    # the restarted run does steps 4..6 (timedisc.c: max_iter more steps from
+    # the restart iteration).  Its first dt uses the primitives of the
+    # restored state, where the uninterrupted run used those of step 3's last
+    # stage input (the reference keeps that stale copy in phi_total), so the
+    # states agree to the size of that dt change, not bitwise.
+    assert rc == 0 and rc2 == 0 and len(log) == 3
+    e = np.abs(U_restart - U_direct).max() / np.abs(U_direct).max()
+    assert e < 1e-4, e
