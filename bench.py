@@ -44,6 +44,11 @@ B_STAGE = 80 + 80 + 224 + 24 + 8 + 24 + 20 + 240 + 160
 B_CELL_STAGE = 1276
 # Tetrahedra, first order, inviscid (batched Euler cases)
 B_CELL_STAGE_FO = 388
+# HBM bytes per launch of the residual (K2) and gradient (K1) kernels on the
+# default 96^3 x 6 mesh, from rocprofv3 FETCH_SIZE and WRITE_SIZE per
+# dispatch scaled by a 1 GiB device copy's counters (read = 2.0 x FETCH_SIZE,
+# written = 1.0 x WRITE_SIZE): profiles/explicit_tgv96_rk3_kernels.md
+TRAFFIC_N96 = {"residual": 3.656e9 + 0.498e9, "gradient": 5.137e9 + 1.932e9}
 
 
 def main():
@@ -102,9 +107,11 @@ def main():
         roofline = {"bound": "hbm",
                     "kernel": "fv_residual_group_kernel<AUSM,face states,viscous,4>",
                     "achieved": round(ach_stage, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                    "frac": round(ach_stage / PEAK_HBM_GBS, 4), "traffic": None,
+                    "frac": round(ach_stage / PEAK_HBM_GBS, 4),
+                    "traffic": TRAFFIC_N96["residual"] if args.n == 96 else None,
                     "bytes_per_cell": B_STAGE, "avg_us": round(k_us[1], 2)}
         extra = {"gradient_kernel": {"achieved": round(ach_grad, 1), "bytes_per_cell": B_GRAD,
+                                     "traffic": TRAFFIC_N96["gradient"] if args.n == 96 else None,
                                      "avg_us": round(k_us[0], 2),
                                      "frac": round(ach_grad / PEAK_HBM_GBS, 4)},
                  "step": {"achieved": round(step_gbs, 1), "bytes_per_cell_stage": B_CELL_STAGE,
@@ -155,8 +162,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "taylor-green Re1600 Ma0.1, %d^3 hex x 6 tets (%d cells), "
-                               "AUSM + Green-Gauss + Barth-Jespersen, RK3-3" % (args.n, N),
+        "config": {"workload": "taylor-green Re1600 Ma0.1, %d^3 hex x 6 tets (%s cells), "
+                               "AUSM + Green-Gauss + Barth-Jespersen, RK3-3"
+                               % (args.n, N if N is not None else 6 * args.n ** 3),
                    "cells": N, "stages_per_step": stages},
         "roofline": roofline,
         "roofline_detail": extra,
