@@ -68,6 +68,9 @@ def main():
     ap.add_argument("--jfnk-mesh", default="160,40,13")
     ap.add_argument("--jfnk-max-newton", type=int, default=400,
                     help="Newton step limit of the JFNK legs (a short profiling run)")
+    ap.add_argument("--jfnk-variants", default="",
+                    help="comma-separated subset of the JFNK variants after the plain "
+                         "FP64 block-Jacobi run (profiling)")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-cases", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=20)
@@ -139,7 +142,9 @@ def main():
         del src, dst
 
     jfnk = None if args.no_jfnk else bench_jfnk([int(x) for x in args.jfnk_mesh.split(",")],
-                                                 args.jfnk_max_newton)
+                                                 args.jfnk_max_newton,
+                                                 args.jfnk_variants.split(",")
+                                                 if args.jfnk_variants else None)
     batched = None if args.no_batch else bench_batch(args.batch_cases, args.batch_steps,
                                                      not args.no_batch_alone)
 
@@ -175,7 +180,7 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def bench_jfnk(n, max_newton=400):
+def bench_jfnk(n, max_newton=400, only=None):
     """Plain FP64 block-Jacobi JFNK against the learned-preconditioner,
     FP32-Krylov JFNK on the bump channel, both from the free stream to the
     1e-8 stop norm."""
@@ -195,14 +200,20 @@ def bench_jfnk(n, max_newton=400):
     print("jfnk plain: %.2f s, %d newton, %d gmres" % (st["seconds"], st["newton_steps"],
                                                         st["gmres_iterations"]),
           file=sys.stderr, flush=True)
-    variants = [("fp32_block", dict(precision="fp32", precond="block"))]
+    variants = [("fp32_block", dict(precision="fp32", precond="block")),
+                ("fp64_dilu", dict(precision="fp64", precond="block", coupling="dilu")),
+                ("fp32_dilu", dict(precision="fp32", precond="block", coupling="dilu"))]
     have_net = os.path.exists(learned.PRETRAINED)
     if have_net:
         learned.load_pretrained()
-        variants.append(("learned_fp32", dict(precision="fp32", precond="learned")))
+        variants += [("learned_fp32", dict(precision="fp32", precond="learned")),
+                     ("learned_fp32_dilu", dict(precision="fp32", precond="learned",
+                                                coupling="dilu"))]
     else:
-        out["learned_fp32"] = None
+        out["learned_fp32"] = out["learned_fp32_dilu"] = None
         out["note"] = "no pretrained network at %s" % learned.PRETRAINED
+    if only is not None:
+        variants = [v for v in variants if v[0] in only]
     for name, kw in variants:
         case.set_state(U0)
         st, _ = case.jfnk(tol=1e-8, max_newton=max_newton, **kw)

@@ -21,7 +21,7 @@ class JfnkOptions(ctypes.Structure):
                 ("fd_eps", c_double), ("fd_eps32", c_double), ("max_change", c_double),
                 ("precision", c_int),
                 ("precond", c_int), ("verbose", c_int), ("learned_check", c_int),
-                ("learned_tol", c_double)]
+                ("learned_tol", c_double), ("coupling", c_int)]
 
 
 class JfnkStats(ctypes.Structure):
@@ -115,8 +115,10 @@ SIGNATURES = [
     ("fvh_jfnk_solve", c_int, [ctypes.POINTER(JfnkOptions), ctypes.POINTER(JfnkStats)]),
     ("fvh_jfnk_history", c_int, [P_DOUBLE, c_int]),
     ("fvh_jfnk_blocks", c_int, [c_double, c_int, P_DOUBLE, P_DOUBLE]),
+    ("fvh_jfnk_precond_apply", c_int, [c_double, c_int, c_int, P_DOUBLE, P_DOUBLE]),
     ("fvh_hdf5_available", c_int, []),
     ("fvh_set_output_format", c_int, [c_int]),
+    ("fvh_mesh_gmsh", c_int, [c_char_p, c_double, c_double, c_double]),
     ("fvh_mesh_batch", c_int, [c_int, P_INT, c_double, c_int]),
     ("fvh_batch_setup", c_int, [c_int, P_INT, P_INT, P_DOUBLE]),
     ("fvh_batch_run", c_int, [c_int, c_int, P_DOUBLE]),

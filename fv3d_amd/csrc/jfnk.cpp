@@ -42,6 +42,28 @@ constexpr int kH1 = 0, kH2 = kry::kMaxDots, kWW = 2 * kry::kMaxDots,
 Workspace g_ws;
 std::vector<double> g_hist;
 
+// Multicolour block-ILU structures of the current mesh (colouring and
+// neighbour tables built on the host from the device connectivity) and the
+// per-Newton-step blocks; the buffers belong to g_ws.
+struct DiluState
+{
+    const int *mesh_cf = nullptr; // the mesh they were built for
+    int N = 0, maxf = 0, ncol = 0;
+    std::vector<int> coff;
+    int *ccell = nullptr, *nbr = nullptr, *back = nullptr, *flags = nullptr;
+    double *off = nullptr, *dinv = nullptr;
+    float *off32 = nullptr, *dinv32 = nullptr;
+    kry::DiluMesh view() const { return {N, maxf, ncol, ccell, nbr, back, flags}; }
+};
+DiluState g_dilu;
+
+template <class T> const T *dilu_off();
+template <> const float *dilu_off<float>() { return g_dilu.off32; }
+template <> const double *dilu_off<double>() { return g_dilu.off; }
+template <class T> const T *dilu_dinv();
+template <> const float *dilu_dinv<float>() { return g_dilu.dinv32; }
+template <> const double *dilu_dinv<double>() { return g_dilu.dinv; }
+
 template <class X> X *walloc(size_t n)
 {
     void *p = nullptr;
@@ -57,6 +79,7 @@ void release()
     if (g_ws.host)
         (void)hipHostFree(g_ws.host);
     g_ws = Workspace();
+    g_dilu = DiluState();
 }
 
 void prepare(int N, int m, int prec)
@@ -93,6 +116,136 @@ void fetch(hipStream_t s, int off, int count)
     FVH_HIP_CHECK(hipMemcpyAsync(g_ws.host + off, g_ws.dots + off, count * sizeof(double),
                                  hipMemcpyDeviceToHost, s));
     FVH_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+// Greedy distance-1 colouring of the cell graph in cell order (at most
+// maxf + 1 colours), colour-sorted positions, the neighbour / transposed-slot
+// tables and the block buffers (FP32 copies when fp32).
+void dilu_prepare(const RuntimeView &v, bool fp32)
+{
+    const dev::DevMesh &m = v.m;
+    if (g_dilu.mesh_cf != m.cf || g_dilu.N != m.N)
+    {
+        const int N = m.N, F = m.F, maxf = m.maxf;
+        if (maxf > 8)
+            throw std::runtime_error("DILU: cells with more than 8 faces are not supported");
+        std::vector<int> cf((size_t)N * maxf), ncf(N), fcell((size_t)2 * F);
+        FVH_HIP_CHECK(hipMemcpy(cf.data(), m.cf, cf.size() * sizeof(int),
+                                hipMemcpyDeviceToHost));
+        FVH_HIP_CHECK(hipMemcpy(ncf.data(), m.ncf, ncf.size() * sizeof(int),
+                                hipMemcpyDeviceToHost));
+        FVH_HIP_CHECK(hipMemcpy(fcell.data(), m.fcell, fcell.size() * sizeof(int),
+                                hipMemcpyDeviceToHost));
+        std::vector<int> nb((size_t)N * maxf, -1);
+        for (int c = 0; c < N; ++c)
+            for (int t = 0; t < ncf[c]; ++t)
+            {
+                const int f = cf[(size_t)c * maxf + t];
+                const int c0 = fcell[2 * (size_t)f], c1 = fcell[2 * (size_t)f + 1];
+                const int o = c0 == c ? c1 : c0;
+                if (o >= N)
+                    continue; // boundary face
+                if (o == c)
+                    throw std::runtime_error("DILU: a cell is its own face neighbour");
+                for (int u = 0; u < t; ++u)
+                    if (nb[(size_t)c * maxf + u] == o)
+                        throw std::runtime_error("DILU: two cells share more than one face");
+                nb[(size_t)c * maxf + t] = o;
+            }
+        std::vector<int> col(N, -1);
+        int ncol = 0;
+        for (int c = 0; c < N; ++c)
+        {
+            unsigned used = 0;
+            for (int t = 0; t < maxf; ++t)
+            {
+                const int o = nb[(size_t)c * maxf + t];
+                if (o >= 0 && col[o] >= 0)
+                    used |= 1u << col[o];
+            }
+            int q = 0;
+            while ((used >> q) & 1u)
+                ++q;
+            col[c] = q;
+            ncol = std::max(ncol, q + 1);
+        }
+        std::vector<int> coff(ncol + 1, 0), ccell(N), pos(N);
+        for (int c = 0; c < N; ++c)
+            ++coff[col[c] + 1];
+        for (int q = 0; q < ncol; ++q)
+            coff[q + 1] += coff[q];
+        {
+            std::vector<int> fill(coff.begin(), coff.end() - 1);
+            for (int c = 0; c < N; ++c)
+            {
+                pos[c] = fill[col[c]]++;
+                ccell[pos[c]] = c;
+            }
+        }
+        std::vector<int> nbr((size_t)N * maxf, -1), back((size_t)N * maxf, -1), flags(N, 0);
+        for (int p = 0; p < N; ++p)
+        {
+            const int c = ccell[p];
+            for (int t = 0; t < maxf; ++t)
+            {
+                const int o = nb[(size_t)c * maxf + t];
+                if (o < 0)
+                    continue;
+                const int f = cf[(size_t)c * maxf + t];
+                int to = -1;
+                for (int u = 0; u < ncf[o]; ++u)
+                    if (cf[(size_t)o * maxf + u] == f)
+                        to = u;
+                if (to < 0)
+                    throw std::runtime_error("DILU: inconsistent face lists");
+                nbr[(size_t)p * maxf + t] = o;
+                back[(size_t)p * maxf + t] = pos[o] * maxf + to;
+                flags[p] |= col[o] < col[c] ? (1 << t) : (1 << (8 + t));
+            }
+        }
+        g_dilu = DiluState();
+        g_dilu.mesh_cf = m.cf;
+        g_dilu.N = N;
+        g_dilu.maxf = maxf;
+        g_dilu.ncol = ncol;
+        g_dilu.coff = coff;
+        g_dilu.ccell = walloc<int>(N);
+        g_dilu.nbr = walloc<int>((size_t)N * maxf);
+        g_dilu.back = walloc<int>((size_t)N * maxf);
+        g_dilu.flags = walloc<int>(N);
+        FVH_HIP_CHECK(hipMemcpy(g_dilu.ccell, ccell.data(), (size_t)N * sizeof(int),
+                                hipMemcpyHostToDevice));
+        FVH_HIP_CHECK(hipMemcpy(g_dilu.nbr, nbr.data(), nbr.size() * sizeof(int),
+                                hipMemcpyHostToDevice));
+        FVH_HIP_CHECK(hipMemcpy(g_dilu.back, back.data(), back.size() * sizeof(int),
+                                hipMemcpyHostToDevice));
+        FVH_HIP_CHECK(hipMemcpy(g_dilu.flags, flags.data(), (size_t)N * sizeof(int),
+                                hipMemcpyHostToDevice));
+        g_dilu.off = walloc<double>((size_t)N * maxf * 25);
+        g_dilu.dinv = walloc<double>((size_t)N * 25);
+    }
+    if (fp32 && !g_dilu.off32)
+    {
+        g_dilu.off32 = walloc<float>((size_t)g_dilu.N * g_dilu.maxf * 25);
+        g_dilu.dinv32 = walloc<float>((size_t)g_dilu.N * 25);
+    }
+}
+
+// The coupled preconditioner at U from the inverse diagonal blocks in Minv.
+void dilu_build(const RuntimeView &v, bool fp32)
+{
+    dilu_prepare(v, fp32);
+    const kry::DiluMesh d = g_dilu.view();
+    kry::dilu_offdiag(v.stream, v.m, v.P, v.flux, d, v.U, g_dilu.off);
+    for (int q = 0; q < g_dilu.ncol; ++q)
+    {
+        if (fp32)
+            kry::dilu_factor<float>(v.stream, d, g_dilu.coff[q], g_dilu.coff[q + 1], g_ws.Minv,
+                                    g_dilu.off, g_dilu.dinv, g_dilu.off32, g_dilu.dinv32);
+        else
+            kry::dilu_factor<double>(v.stream, d, g_dilu.coff[q], g_dilu.coff[q + 1],
+                                     g_ws.Minv, g_dilu.off, g_dilu.dinv, nullptr, nullptr);
+    }
 }
 
 // stop norm max_j (1/V_glob) sum_i |R_ij| V_i and |R|_2 of g_ws.R0
@@ -138,6 +291,18 @@ GmresResult gmres(const RuntimeView &v, const fvh_jfnk_options_t &o, double fd_n
     // difference quotient takes two FP32-arithmetic residuals (R0j is the
     // FP32 residual of U)
     const bool fp32 = o.precision == FVH_JFNK_FP32;
+    // z = M^-1 in with |z|^2 into dots[kZZ]
+    const bool coupled = o.coupling == FVH_COUPLING_DILU;
+    auto apply_M = [&](const T *in, T *out) {
+        if (coupled)
+        {
+            kry::dilu_apply<T, T>(s, g_dilu.view(), g_dilu.coff.data(), dilu_off<T>(),
+                                  dilu_dinv<T>(), in, out);
+            kry::multidot<T>(s, n, 1, out, ld, out, g_ws.partials, dots + kZZ);
+        }
+        else
+            kry::precond_apply_norm<T>(s, N, g_ws.Minv, in, out, g_ws.partials, dots + kZZ);
+    };
     auto apply_op = [&](const T *zin, T *out) {
         kry::fd_eps(s, dots + kZZ, fd_num, dots + kEps);
         kry::perturb<T>(s, n, v.U, dots + kEps, zin, g_ws.Ut);
@@ -154,7 +319,7 @@ GmresResult gmres(const RuntimeView &v, const fvh_jfnk_options_t &o, double fd_n
         else
         {
             kry::convert<T>(s, n, g_ws.xacc, t);
-            kry::precond_apply_norm<T>(s, N, g_ws.Minv, t, z, g_ws.partials, dots + kZZ);
+            apply_M(t, z);
             apply_op(z, V);
             kry::rhs_minus<T>(s, n, g_ws.R0, V);
         }
@@ -176,8 +341,7 @@ GmresResult gmres(const RuntimeView &v, const fvh_jfnk_options_t &o, double fd_n
         for (int j = 0; j < m; ++j)
         {
             T *w = V + (size_t)(j + 1) * ld;
-            kry::precond_apply_norm<T>(s, N, g_ws.Minv, V + (size_t)j * ld, z, g_ws.partials,
-                                       dots + kZZ);
+            apply_M(V + (size_t)j * ld, z);
             apply_op(z, w);
             // classical Gram-Schmidt with one pass and the norm by Pythagoras
             // (w is V_{j+1}, so the same multi-dot gives <w, w>); a second
@@ -266,6 +430,10 @@ int solve(const fvh_jfnk_options_t &o, fvh_jfnk_stats_t &st)
     if (o.precond == FVH_PRECOND_LEARNED && !learned_ready())
         throw std::runtime_error("learned preconditioner: no network (fvh_nn_init / "
                                  "fvh_nn_set_params)");
+    if (o.coupling != FVH_COUPLING_NONE && o.coupling != FVH_COUPLING_DILU)
+        throw std::runtime_error("unknown coupling");
+    if (o.coupling == FVH_COUPLING_DILU && o.precond == FVH_PRECOND_SCALAR)
+        throw std::runtime_error("DILU coupling needs block or learned diagonal blocks");
     RuntimeView v = runtime_view();
     const int N = v.m.N, n = 5 * N;
     prepare(N, o.krylov_dims, o.precision);
@@ -322,6 +490,8 @@ int solve(const fvh_jfnk_options_t &o, fvh_jfnk_stats_t &st)
         else
             kry::precond_blocks(s, v.m, v.P, v.flux, v.U, g_ws.dtau,
                                 o.precond == FVH_PRECOND_SCALAR ? 0 : 1, g_ws.Minv);
+        if (o.coupling == FVH_COUPLING_DILU)
+            dilu_build(v, o.precision == FVH_JFNK_FP32);
         FVH_HIP_CHECK(hipStreamSynchronize(s));
         st.precond_seconds +=
             std::chrono::duration<double>(std::chrono::steady_clock::now() - tp).count();
@@ -338,7 +508,17 @@ int solve(const fvh_jfnk_options_t &o, fvh_jfnk_stats_t &st)
                                                       : gmres<double>(v, o, fd_num);
         st.gmres_iterations += gr.its;
         st.residual_evals += gr.evals;
-        kry::precond_apply<double>(s, N, g_ws.Minv, g_ws.xacc, g_ws.dU);
+        if (o.coupling == FVH_COUPLING_DILU)
+        {
+            if (fp32)
+                kry::dilu_apply<double, float>(s, g_dilu.view(), g_dilu.coff.data(),
+                                               g_dilu.off32, g_dilu.dinv32, g_ws.xacc, g_ws.dU);
+            else
+                kry::dilu_apply<double, double>(s, g_dilu.view(), g_dilu.coff.data(),
+                                                g_dilu.off, g_dilu.dinv, g_ws.xacc, g_ws.dU);
+        }
+        else
+            kry::precond_apply<double>(s, N, g_ws.Minv, g_ws.xacc, g_ws.dU);
         kry::update_ratio(s, N, v.U, g_ws.dU, g_ws.partials, g_ws.dots + kRatio);
         fetch(s, kRatio, 1);
         const double ratio = g_ws.host[kRatio];
@@ -404,6 +584,7 @@ int fvh_jfnk_default_options(fvh_jfnk_options_t *o)
     o->verbose = 0;
     o->learned_check = 10;
     o->learned_tol = 0.2;
+    o->coupling = FVH_COUPLING_NONE;
     return FVH_OK;
 }
 
@@ -444,6 +625,55 @@ int fvh_jfnk_blocks(double cfl, int precond, double *dtau, double *minv)
         if (minv)
             FVH_HIP_CHECK(hipMemcpy(minv, g_ws.Minv, (size_t)25 * N * sizeof(double),
                                     hipMemcpyDeviceToHost));
+    }
+    catch (const std::exception &e)
+    {
+        set_error(e.what());
+        return FVH_ERR_STATE;
+    }
+    return FVH_OK;
+}
+
+int fvh_jfnk_precond_apply(double cfl, int precond, int coupling, const double *v, double *z)
+{
+    if (!v || !z)
+    {
+        set_error("fvh_jfnk_precond_apply: null argument");
+        return FVH_ERR_ARG;
+    }
+    try
+    {
+        if (precond < FVH_PRECOND_SCALAR || precond > FVH_PRECOND_LEARNED)
+            throw std::runtime_error("fvh_jfnk_precond_apply: unknown preconditioner");
+        if (coupling != FVH_COUPLING_NONE && coupling != FVH_COUPLING_DILU)
+            throw std::runtime_error("fvh_jfnk_precond_apply: unknown coupling");
+        if (coupling == FVH_COUPLING_DILU && precond == FVH_PRECOND_SCALAR)
+            throw std::runtime_error("DILU coupling needs block or learned diagonal blocks");
+        if (precond == FVH_PRECOND_LEARNED && !learned_ready())
+            throw std::runtime_error("learned preconditioner: no network");
+        RuntimeView rv = runtime_view();
+        const int N = rv.m.N, n = 5 * N;
+        prepare(N, g_ws.m > 0 ? g_ws.m : 30, g_ws.prec >= 0 ? g_ws.prec : FVH_JFNK_FP64);
+        hipStream_t s = rv.stream;
+        kry::local_dt(s, rv.m, rv.P, rv.U, cfl, g_ws.dtau);
+        if (precond == FVH_PRECOND_LEARNED)
+            learned_blocks(rv, cfl, g_ws.dtau, g_ws.Minv);
+        else
+            kry::precond_blocks(s, rv.m, rv.P, rv.flux, rv.U, g_ws.dtau,
+                                precond == FVH_PRECOND_BLOCK ? 1 : 0, g_ws.Minv);
+        FVH_HIP_CHECK(hipMemcpyAsync(g_ws.xacc, v, (size_t)n * sizeof(double),
+                                     hipMemcpyHostToDevice, s));
+        if (coupling == FVH_COUPLING_DILU)
+        {
+            dilu_build(rv, false);
+            kry::dilu_apply<double, double>(s, g_dilu.view(), g_dilu.coff.data(), g_dilu.off,
+                                            g_dilu.dinv, g_ws.xacc, g_ws.dU);
+        }
+        else
+            kry::precond_apply<double>(s, N, g_ws.Minv, g_ws.xacc, g_ws.dU);
+        FVH_HIP_CHECK(hipMemcpyAsync(z, g_ws.dU, (size_t)n * sizeof(double),
+                                     hipMemcpyDeviceToHost, s));
+        FVH_HIP_CHECK(hipStreamSynchronize(s));
     }
     catch (const std::exception &e)
     {

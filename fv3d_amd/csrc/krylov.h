@@ -38,6 +38,48 @@ template <typename T>
 void precond_apply_norm(hipStream_t s, int N, const double *Minv, const T *v, T *z,
                         double *partials, double *zz);
 
+// ---- coupled preconditioner: multicolour block ILU(0) ("DILU") -------------
+// M = (D + L) D^-1 (D + U) over the cells in colour order (a distance-1
+// colouring: no two cells of one colour share a face), L and U the
+// off-diagonal 5x5 blocks of A1 = I/dtau - dR1/dU (R1 the first-order
+// inviscid residual) below and above the diagonal in that order, and
+// D_i = A1_ii - sum_{k lower neighbour of i} A1_ik D_k^-1 A1_ki, so that M
+// agrees with A1 on the diagonal blocks and on every face-neighbour block.
+//
+// Blocks are stored by colour-sorted position p (cell ccell[p]) and face slot
+// t (the slot order of the cell's face list): off[(p*maxf + t)*25 + r*5 + k]
+// = A1_{i,nbr} row r, column k.  nbr[p*maxf + t] is the neighbour cell or -1
+// (boundary face, or t beyond the cell's face count); back[p*maxf + t] the
+// flat slot index (pk*maxf + t') of the transposed coupling A1_{nbr,i};
+// flags[p] has bit t set for a neighbour of a lower colour and bit 8 + t for
+// one of a higher colour.
+struct DiluMesh
+{
+    int N, maxf, ncol;
+    const int *ccell, *nbr, *back, *flags;
+};
+
+// off = the off-diagonal blocks of A1 at U, by one-sided differences of the
+// shared face flux in the column cell's state (h = 1e-7 max(|U|, 1), the
+// steps of precond_blocks).
+void dilu_offdiag(hipStream_t s, const dev::DevMesh &m, const dev::PhysDev &P, int flux,
+                  const DiluMesh &d, const double *U, double *off);
+
+// Factorisation of the positions [p0, p1) of one colour (colours in
+// ascending order): dinv[p] = D_i^-1 with A1_ii = Minv[i]^-1 (Minv by cell,
+// the block-Jacobi or learned inverse blocks).  With B = float the
+// position's blocks are also converted into offB / dinvB.
+template <typename B>
+void dilu_factor(hipStream_t s, const DiluMesh &d, int p0, int p1, const double *Minv,
+                 const double *off, double *dinv, B *offB, B *dinvB);
+
+// z = M^-1 v: the forward sweep over the colours in ascending order, then
+// the backward sweep in descending order (z in place); coff[q] is the first
+// position of colour q (host array of ncol + 1 entries).
+template <typename T, typename B>
+void dilu_apply(hipStream_t s, const DiluMesh &d, const int *coff, const B *off,
+                const B *dinv, const T *v, T *z);
+
 // eps[0] = num / sqrt(zz[0]) (num / 1e-300 for zz = 0)
 void fd_eps(hipStream_t s, const double *zz, double num, double *eps);
 

@@ -438,6 +438,256 @@ __global__ __launch_bounds__(kBlock) void k_update(int n, double *__restrict__ U
         U[i] += alpha * dU[i];
 }
 
+// ---- multicolour block ILU(0) -----------------------------------------------
+
+// Off-diagonal blocks A1_{nb,c} = -dR1_nb/dU_c = (s_cf A_f / V_nb) dF_f/dU_c
+// (s_cf = +1 when c is the face's first cell), one thread per position; the
+// column block of each interior face is written into the neighbour's row
+// slot (every slot is written by exactly one thread).
+template <int FLUX, int MAXF>
+__global__ __launch_bounds__(kBlock) void k_dilu_offdiag(DevMesh m, PhysDev P,
+                                                         DiluMesh d,
+                                                         const double *__restrict__ U,
+                                                         double *__restrict__ off)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= d.N)
+        return;
+    const int c = d.ccell[p];
+    double uc[10];
+    dev::load_cons(U, c, uc);
+    const int nf = m.ncf[c];
+    const dev::ViscGrad none{};
+#pragma unroll 1
+    for (int s = 0; s < MAXF; ++s)
+    {
+        if (s >= nf)
+            break;
+        const int dst = d.back[p * d.maxf + s];
+        if (dst < 0)
+            continue; // boundary face
+        const int f = m.cf[(size_t)c * m.maxf + s];
+        const int c0 = m.fcell[2 * f], c1 = m.fcell[2 * f + 1];
+        const bool own0 = c0 == c;
+        const int nb = own0 ? c1 : c0;
+        dev::Frame fr;
+        dev::load_frame(m, f, fr);
+        double pn[10];
+        dev::load_cons(U, nb, pn);
+        dev::con_to_prim(P, pn);
+        const double w = (own0 ? -fr.area : fr.area) / m.vol[nb];
+        double f0[5];
+        {
+            double u[10];
+#pragma unroll
+            for (int j = 0; j < 5; ++j)
+                u[j] = uc[j];
+            dev::con_to_prim(P, u);
+            double L[10], R[10];
+#pragma unroll
+            for (int j = 0; j < 10; ++j)
+            {
+                L[j] = own0 ? u[j] : pn[j];
+                R[j] = own0 ? pn[j] : u[j];
+            }
+            dev::face_flux<FLUX, false>(P, fr, L, R, none, none, f0);
+        }
+        double *o = off + (size_t)dst * 25;
+#pragma unroll 1
+        for (int k = 0; k < 5; ++k)
+        {
+            double u[10];
+#pragma unroll
+            for (int j = 0; j < 5; ++j)
+                u[j] = uc[j];
+            const double h = 1e-7 * dev::rmax(fabs(uc[k]), 1.0);
+            u[k] = uc[k] + h;
+            const double hk = u[k] - uc[k];
+            dev::con_to_prim(P, u);
+            double L[10], R[10];
+#pragma unroll
+            for (int j = 0; j < 10; ++j)
+            {
+                L[j] = own0 ? u[j] : pn[j];
+                R[j] = own0 ? pn[j] : u[j];
+            }
+            double fk[5];
+            dev::face_flux<FLUX, false>(P, fr, L, R, none, none, fk);
+#pragma unroll
+            for (int j = 0; j < 5; ++j)
+                o[j * 5 + k] = w * ((fk[j] - f0[j]) / hk);
+        }
+    }
+}
+
+__device__ __forceinline__ void mm5(const double *A, const double *B, double *C)
+{
+#pragma unroll
+    for (int r = 0; r < 5; ++r)
+#pragma unroll
+        for (int k = 0; k < 5; ++k)
+        {
+            double a = 0.0;
+#pragma unroll
+            for (int q = 0; q < 5; ++q)
+                a += A[r * 5 + q] * B[q * 5 + k];
+            C[r * 5 + k] = a;
+        }
+}
+
+template <typename B>
+__global__ __launch_bounds__(kBlock) void k_dilu_factor(DiluMesh d, int p0, int p1,
+                                                        const double *__restrict__ Minv,
+                                                        const double *__restrict__ off,
+                                                        double *__restrict__ dinv,
+                                                        B *__restrict__ offB,
+                                                        B *__restrict__ dinvB)
+{
+    const int p = p0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= p1)
+        return;
+    const int i = d.ccell[p];
+    double D[25], X[25], Y[25];
+#pragma unroll
+    for (int q = 0; q < 25; ++q)
+        X[q] = Minv[(size_t)i * 25 + q];
+    dev::invert5(X, D);
+    const int lo = d.flags[p] & 0xff;
+#pragma unroll 1
+    for (int t = 0; t < d.maxf; ++t)
+    {
+        if (!((lo >> t) & 1))
+            continue;
+        const int kt = d.back[p * d.maxf + t];
+        const int pk = kt / d.maxf;
+        const double *Aik = off + ((size_t)p * d.maxf + t) * 25;
+        const double *Aki = off + (size_t)kt * 25;
+        const double *Dk = dinv + (size_t)pk * 25;
+        double Ak[25];
+#pragma unroll
+        for (int q = 0; q < 25; ++q)
+            Ak[q] = Aik[q];
+        mm5(Ak, Dk, X); // A_ik D_k^-1
+#pragma unroll
+        for (int q = 0; q < 25; ++q)
+            Ak[q] = Aki[q];
+        mm5(X, Ak, Y); // A_ik D_k^-1 A_ki
+#pragma unroll
+        for (int q = 0; q < 25; ++q)
+            D[q] -= Y[q];
+    }
+    dev::invert5(D, X);
+#pragma unroll
+    for (int q = 0; q < 25; ++q)
+        dinv[(size_t)p * 25 + q] = X[q];
+    if (dinvB)
+    {
+#pragma unroll
+        for (int q = 0; q < 25; ++q)
+            dinvB[(size_t)p * 25 + q] = (B)X[q];
+        for (int q = 0; q < d.maxf * 25; ++q)
+            offB[(size_t)p * d.maxf * 25 + q] = (B)off[(size_t)p * d.maxf * 25 + q];
+    }
+}
+
+// forward sweep of one colour: z_i = D_i^-1 (v_i - sum_{k lower} A_ik z_k)
+template <typename T, typename B>
+__global__ __launch_bounds__(kBlock) void k_dilu_forward(DiluMesh d, int p0, int p1,
+                                                         const B *__restrict__ off,
+                                                         const B *__restrict__ dinv,
+                                                         const T *__restrict__ v,
+                                                         T *__restrict__ z)
+{
+    const int p = p0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= p1)
+        return;
+    const int i = d.ccell[p];
+    double r[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+        r[j] = (double)v[(size_t)i * 5 + j];
+    const int lo = d.flags[p] & 0xff;
+#pragma unroll 1
+    for (int t = 0; t < d.maxf; ++t)
+    {
+        if (!((lo >> t) & 1))
+            continue;
+        const int k = d.nbr[p * d.maxf + t];
+        const B *A = off + ((size_t)p * d.maxf + t) * 25;
+        double y[5];
+#pragma unroll
+        for (int j = 0; j < 5; ++j)
+            y[j] = (double)z[(size_t)k * 5 + j];
+#pragma unroll
+        for (int a = 0; a < 5; ++a)
+        {
+            double acc = 0.0;
+#pragma unroll
+            for (int b = 0; b < 5; ++b)
+                acc += (double)A[a * 5 + b] * y[b];
+            r[a] -= acc;
+        }
+    }
+    const B *Di = dinv + (size_t)p * 25;
+#pragma unroll
+    for (int a = 0; a < 5; ++a)
+    {
+        double acc = 0.0;
+#pragma unroll
+        for (int b = 0; b < 5; ++b)
+            acc += (double)Di[a * 5 + b] * r[b];
+        z[(size_t)i * 5 + a] = (T)acc;
+    }
+}
+
+// backward sweep of one colour: z_i -= D_i^-1 sum_{k higher} A_ik z_k
+template <typename T, typename B>
+__global__ __launch_bounds__(kBlock) void k_dilu_backward(DiluMesh d, int p0, int p1,
+                                                          const B *__restrict__ off,
+                                                          const B *__restrict__ dinv,
+                                                          T *__restrict__ z)
+{
+    const int p = p0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= p1)
+        return;
+    const int hi = (d.flags[p] >> 8) & 0xff;
+    if (!hi)
+        return;
+    const int i = d.ccell[p];
+    double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+    for (int t = 0; t < d.maxf; ++t)
+    {
+        if (!((hi >> t) & 1))
+            continue;
+        const int k = d.nbr[p * d.maxf + t];
+        const B *A = off + ((size_t)p * d.maxf + t) * 25;
+        double x[5];
+#pragma unroll
+        for (int j = 0; j < 5; ++j)
+            x[j] = (double)z[(size_t)k * 5 + j];
+#pragma unroll
+        for (int a = 0; a < 5; ++a)
+        {
+            double acc = 0.0;
+#pragma unroll
+            for (int b = 0; b < 5; ++b)
+                acc += (double)A[a * 5 + b] * x[b];
+            r[a] += acc;
+        }
+    }
+    const B *Di = dinv + (size_t)p * 25;
+#pragma unroll
+    for (int a = 0; a < 5; ++a)
+    {
+        double acc = 0.0;
+#pragma unroll
+        for (int b = 0; b < 5; ++b)
+            acc += (double)Di[a * 5 + b] * r[b];
+        z[(size_t)i * 5 + a] = (T)((double)z[(size_t)i * 5 + a] - acc);
+    }
+}
+
 } // namespace
 
 void local_dt(hipStream_t s, const DevMesh &m, const PhysDev &P, const double *U,
@@ -586,6 +836,69 @@ void update(hipStream_t s, int n, double *U, const double *dU, double alpha)
 {
     hipLaunchKernelGGL(k_update, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, U, dU, alpha);
 }
+
+void dilu_offdiag(hipStream_t s, const DevMesh &m, const PhysDev &P, int flux,
+                  const DiluMesh &d, const double *U, double *off)
+{
+    if (m.maxf > 6)
+        throw std::runtime_error("cells with more than 6 faces are not supported");
+    const dim3 g(blocks_for(d.N)), b(kBlock);
+    if (flux == dev::FluxRusanov)
+    {
+        if (m.maxf <= 4)
+            hipLaunchKernelGGL((k_dilu_offdiag<dev::FluxRusanov, 4>), g, b, 0, s, m, P, d, U,
+                               off);
+        else
+            hipLaunchKernelGGL((k_dilu_offdiag<dev::FluxRusanov, 6>), g, b, 0, s, m, P, d, U,
+                               off);
+    }
+    else
+    {
+        if (m.maxf <= 4)
+            hipLaunchKernelGGL((k_dilu_offdiag<dev::FluxAUSM, 4>), g, b, 0, s, m, P, d, U, off);
+        else
+            hipLaunchKernelGGL((k_dilu_offdiag<dev::FluxAUSM, 6>), g, b, 0, s, m, P, d, U, off);
+    }
+}
+
+template <typename B>
+void dilu_factor(hipStream_t s, const DiluMesh &d, int p0, int p1, const double *Minv,
+                 const double *off, double *dinv, B *offB, B *dinvB)
+{
+    if (p1 <= p0)
+        return;
+    hipLaunchKernelGGL(k_dilu_factor<B>, dim3(blocks_for(p1 - p0)), dim3(kBlock), 0, s, d, p0,
+                       p1, Minv, off, dinv, offB, dinvB);
+}
+
+template <typename T, typename B>
+void dilu_apply(hipStream_t s, const DiluMesh &d, const int *coff, const B *off,
+                const B *dinv, const T *v, T *z)
+{
+    for (int q = 0; q < d.ncol; ++q)
+        if (coff[q + 1] > coff[q])
+            hipLaunchKernelGGL((k_dilu_forward<T, B>), dim3(blocks_for(coff[q + 1] - coff[q])),
+                               dim3(kBlock), 0, s, d, coff[q], coff[q + 1], off, dinv, v, z);
+    // the highest colour has no higher neighbours
+    for (int q = d.ncol - 2; q >= 0; --q)
+        if (coff[q + 1] > coff[q])
+            hipLaunchKernelGGL((k_dilu_backward<T, B>),
+                               dim3(blocks_for(coff[q + 1] - coff[q])), dim3(kBlock), 0, s, d,
+                               coff[q], coff[q + 1], off, dinv, z);
+}
+
+template void dilu_factor<double>(hipStream_t, const DiluMesh &, int, int, const double *,
+                                  const double *, double *, double *, double *);
+template void dilu_factor<float>(hipStream_t, const DiluMesh &, int, int, const double *,
+                                 const double *, double *, float *, float *);
+template void dilu_apply<double, double>(hipStream_t, const DiluMesh &, const int *,
+                                         const double *, const double *, const double *,
+                                         double *);
+template void dilu_apply<float, float>(hipStream_t, const DiluMesh &, const int *,
+                                       const float *, const float *, const float *, float *);
+template void dilu_apply<double, float>(hipStream_t, const DiluMesh &, const int *,
+                                        const float *, const float *, const double *,
+                                        double *);
 
 #define FVH_KRY_INST(T)                                                                    \
     template void precond_apply<T>(hipStream_t, int, const double *, const T *, T *);      \

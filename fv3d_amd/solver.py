@@ -152,6 +152,11 @@ def get_param(name):
     return ctypes.c_double.in_dll(lib(), name).value
 
 
+_JFNK_NAMES = {"precision": {"fp64": 0, "fp32": 1},
+               "precond": {"scalar": 0, "block": 1, "learned": 2},
+               "coupling": {"none": 0, "dilu": 1}}
+
+
 class Case:
     """One solver set-up in the reference's initialisation order.
 
@@ -298,13 +303,13 @@ class Case:
 
     def jfnk(self, **options):
         """Steady Newton-Krylov solve of the current state (fvh_jfnk_solve).
-        options override fvh_jfnk_default_options fields; precision and
-        precond also take the names "fp64"/"fp32" and
-        "scalar"/"block"/"learned".  Returns (stats dict, history rows)."""
+        options override fvh_jfnk_default_options fields; precision,
+        precond and coupling also take the names "fp64"/"fp32",
+        "scalar"/"block"/"learned" and "none"/"dilu".  Returns (stats dict,
+        history rows)."""
         o = JfnkOptions()
         check(self.L.fvh_jfnk_default_options(ctypes.byref(o)))
-        names = {"precision": {"fp64": 0, "fp32": 1},
-                 "precond": {"scalar": 0, "block": 1, "learned": 2}}
+        names = _JFNK_NAMES
         for k, v in options.items():
             if not hasattr(o, k):
                 raise KeyError("unknown JFNK option %r" % k)
@@ -331,6 +336,18 @@ class Case:
         check(self.L.fvh_jfnk_blocks(cfl, kinds.get(precond, precond),
                                      dtau.ctypes.data_as(P_DOUBLE),
                                      minv.ctypes.data_as(P_DOUBLE)))
+
+    def jfnk_precond_apply(self, cfl, v, precond="block", coupling="none"):
+        """z = M^-1 v (fvh_jfnk_precond_apply) for the preconditioner a
+        Newton step at the current state and pseudo CFL builds; v [N,5]."""
+        v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+        assert v.size == 5 * self.n_cells
+        z = np.zeros_like(v)
+        check(self.L.fvh_jfnk_precond_apply(
+            cfl, _JFNK_NAMES["precond"].get(precond, precond),
+            _JFNK_NAMES["coupling"].get(coupling, coupling),
+            v.ctypes.data_as(P_DOUBLE), z.ctypes.data_as(P_DOUBLE)))
+        return z.reshape(self.n_cells, 5)
 
     def host_phi_total(self):
         check(self.L.fvh_sync_host())

@@ -47,6 +47,14 @@ int fvh_mesh_box(int nx, int ny, int nz, double x0, double y0, double z0,
 int fvh_mesh_channel(int nx, int ny, int nz, double jitter,
                      unsigned long long seed);
 
+/* Read a Gmsh 4.1 ASCII mesh (tetrahedra, hexahedra, prisms, pyramids as
+ * cells; triangles and quadrangles as boundaries; physical names as
+ * regions), scale the vertices by (sx, sy, sz) and preprocess it as the
+ * reference's pyMeshFV3D does (replaces tools/pyMeshFV3D/pymeshfv3d/reader/
+ * gmsh.py:35-139 + process.py:23-214 followed by init_mesh of the written
+ * file).  Builds solver_mesh like fvh_mesh_box; fvh_mesh_write saves it. */
+int fvh_mesh_gmsh(cstring_t path, double sx, double sy, double sz);
+
 /* Write the mesh in the reference's HDF5 layout (path ending in .h5; libhdf5
  * 1.10 is loaded at run time) or in a raw little-endian container with the
  * same groups and datasets (any other name).  init_mesh() reads either,
@@ -131,6 +139,14 @@ int fvh_device_info(char *name, int name_len, int *cu_count,
                                 first-order inviscid residual */
 #define FVH_PRECOND_LEARNED 2 /* 5x5 blocks predicted by the trained cell
                                  network; block Jacobi where rejected */
+#define FVH_COUPLING_NONE 0  /* the preconditioner is block-diagonal */
+#define FVH_COUPLING_DILU 1  /* multicolour block ILU(0) of the first-order
+                                inviscid Jacobian A1 = I/dtau - dR1/dU: the
+                                precond blocks on the diagonal, face-
+                                neighbour blocks by differences of the shared
+                                face flux, cells ordered by a distance-1
+                                colouring; M agrees with A1 on every diagonal
+                                and face-neighbour block */
 
 typedef struct
 {
@@ -161,6 +177,8 @@ typedef struct
                             Newton steps (and at the first); 0 = first only */
     double learned_tol;  /* mean per-cell relative Frobenius error above which
                             the solve falls back to block Jacobi (default 0.2) */
+    int coupling;        /* FVH_COUPLING_* (default FVH_COUPLING_NONE); DILU
+                            needs precond BLOCK or LEARNED */
 } fvh_jfnk_options_t;
 
 typedef struct
@@ -187,6 +205,11 @@ int fvh_jfnk_history(double *rows, int max_rows);
  * minv[N][5][5] (row-major) of M_c = I/dtau_c - dR1_c/dU_c (precond
  * FVH_PRECOND_BLOCK) or of I/dtau_c (FVH_PRECOND_SCALAR). */
 int fvh_jfnk_blocks(double cfl, int precond, double *dtau, double *minv);
+/* z = M^-1 v (host arrays of 5N values, FP64) for the preconditioner that a
+ * Newton step at the current state and the given CFL builds with precond /
+ * coupling (FVH_PRECOND_LEARNED needs a network; it is applied without the
+ * block-error check). */
+int fvh_jfnk_precond_apply(double cfl, int precond, int coupling, const double *v, double *z);
 
 /* ---- batched cases ------------------------------------------------------ */
 /* Many independent cases in one device state and one launch per kernel

@@ -17,6 +17,13 @@ written plainly in FP64 numpy:
     perturbed at once: a first-order residual sees no two of them), a
     different route to the same derivatives as the device's per-cell
     differences;
+  - optionally (coupling "dilu") the multicolour block ILU(0) of
+    A1 = I/dtau - dR1/dU: cells ordered by the greedy distance-1 colouring
+    (ascending cell index within a colour), off-diagonal face-neighbour
+    blocks by one-sided differences over a distance-2 colouring (every
+    neighbour of a perturbed cell sees exactly one perturbed cell), the
+    block-Jacobi blocks on the diagonal, D_i = A1_ii - sum_{k lower}
+    A1_ik D_k^-1 A1_ki, M = (D + L) D^-1 (D + U);
   - right-preconditioned GMRES(m) with restarts on
     A z = z/dtau - [R(U + eps z) - R(U)]/eps, eps = fd_eps (1+|U|)/|z|,
     classical Gram-Schmidt with selective reorthogonalisation, Givens
@@ -38,7 +45,7 @@ from . import fvref
 
 DEFAULTS = dict(max_newton=300, tol=1e-8, cfl0=10.0, cfl_growth=1.5, cfl_cut=0.3,
                 cfl_stall=0.5, cfl_stall_cut=0.7, cfl_max=1e6, krylov_dims=30, max_restarts=3, rtol=1e-2, fd_eps=1e-7,
-                max_change=0.2, precond="block")
+                max_change=0.2, precond="block", coupling="none")
 
 
 def colouring(mesh):
@@ -128,6 +135,90 @@ def gmres(op, b, m, max_restarts, rtol):
     return x, its, rel
 
 
+def neighbours(mesh):
+    """nb[c, t]: the interior face neighbour of cell c across its face slot t
+    (cell_faces order), -1 for boundary faces and unused slots."""
+    N, maxf = mesh["n_cells"], mesh["max_cell_faces"]
+    cf = np.asarray(mesh["cell_faces"]).reshape(N, maxf)
+    fc = np.asarray(mesh["face_cells"]).reshape(-1, 2)
+    nb = -np.ones((N, maxf), dtype=np.int64)
+    for c in range(N):
+        for t in range(maxf):
+            f = cf[c, t]
+            if f < 0:
+                continue
+            o = fc[f, 1] if fc[f, 0] == c else fc[f, 0]
+            if 0 <= o < N:
+                nb[c, t] = o
+    return nb
+
+
+def colouring2(nb):
+    """Greedy distance-2 colouring (no two cells within two face steps share
+    a colour)."""
+    N = nb.shape[0]
+    colour = -np.ones(N, dtype=np.int64)
+    for c in range(N):
+        used = set()
+        for x in nb[c]:
+            if x < 0:
+                continue
+            used.add(colour[x])
+            for y in nb[x]:
+                if y >= 0:
+                    used.add(colour[y])
+        k = 0
+        while k in used:
+            k += 1
+        colour[c] = k
+    return colour
+
+
+class Dilu:
+    """M = (D + L) D^-1 (D + U) in the colour order; diag [N,5,5] the A1
+    diagonal blocks, off [N,maxf,5,5] the blocks A1_{c, nb[c,t]}."""
+
+    def __init__(self, nb, colour, diag, off):
+        self.nb, self.colour, self.off = nb, colour, off
+        N, maxf = nb.shape
+        valid = nb >= 0
+        cn = np.where(valid, colour[np.maximum(nb, 0)], -1)
+        self.low = valid & (cn < colour[:, None])
+        self.high = valid & (cn > colour[:, None])
+        self.groups = [np.where(colour == q)[0] for q in range(colour.max() + 1)]
+        self.Dinv = np.zeros((N, 5, 5))
+        for g in self.groups:
+            for i in g:
+                D = diag[i].copy()
+                for t in np.where(self.low[i])[0]:
+                    k = nb[i, t]
+                    tk = int(np.where(nb[k] == i)[0][0])
+                    D -= off[i, t] @ self.Dinv[k] @ off[k, tk]
+                self.Dinv[i] = np.linalg.inv(D)
+
+    def apply(self, v):
+        N = self.nb.shape[0]
+        v = np.asarray(v).reshape(N, 5)
+        z = np.zeros((N, 5))
+        for g in self.groups:
+            r = v[g].copy()
+            for t in range(self.nb.shape[1]):
+                m = self.low[g, t]
+                if m.any():
+                    i = g[m]
+                    r[m] -= np.einsum("nij,nj->ni", self.off[i, t], z[self.nb[i, t]])
+            z[g] = np.einsum("nij,nj->ni", self.Dinv[g], r)
+        for g in self.groups[::-1]:
+            r = np.zeros((len(g), 5))
+            for t in range(self.nb.shape[1]):
+                m = self.high[g, t]
+                if m.any():
+                    i = g[m]
+                    r[m] += np.einsum("nij,nj->ni", self.off[i, t], z[self.nb[i, t]])
+            z[g] -= np.einsum("nij,nj->ni", self.Dinv[g], r)
+        return z.reshape(-1)
+
+
 class OracleJfnk:
     def __init__(self, mesh, physics, regions, numerics):
         self.mesh = mesh
@@ -141,6 +232,8 @@ class OracleJfnk:
                                      dict(numerics, reconstruction="First-Order",
                                           limiter="None"))
         self.colour = colouring(mesh)
+        self.nb = neighbours(mesh)
+        self._colour2 = None
         self.evals = 0
 
     def residual(self, U, first=False):
@@ -176,6 +269,35 @@ class OracleJfnk:
                 J[mask, :, k] = d[mask] / hk[:, None]
         return np.eye(5)[None] / dtau[:, None, None] - J
 
+    def offdiag(self, U):
+        """A1_{c, nb[c,t]} = -dR1_c/dU_nb, [N, maxf, 5, 5], by one-sided
+        differences (h = 1e-7 max(|U|, 1)) over a distance-2 colouring."""
+        if self._colour2 is None:
+            self._colour2 = colouring2(self.nb)
+        col2, nb = self._colour2, self.nb
+        N, maxf = nb.shape
+        base = self.residual(U, first=True)
+        off = np.zeros((N, maxf, 5, 5))
+        valid = nb >= 0
+        ncol = np.where(valid, col2[np.maximum(nb, 0)], -1)
+        for q in range(col2.max() + 1):
+            mask = col2 == q
+            sel = valid & (ncol == q)
+            rows, slots = np.where(sel)
+            src = nb[rows, slots]
+            for k in range(5):
+                h = 1e-7 * np.maximum(np.abs(U[mask, k]), 1.0)
+                Up = U.copy()
+                Up[mask, k] += h
+                hk = np.zeros(N)
+                hk[mask] = Up[mask, k] - U[mask, k]
+                d = self.residual(Up, first=True) - base
+                off[rows, slots, :, k] = -d[rows] / hk[src][:, None]
+        return off
+
+    def dilu(self, U, dtau):
+        return Dilu(self.nb, self.colour, self.blocks(U, dtau), self.offdiag(U))
+
     def solve(self, U, verbose=False, **opts):
         o = dict(DEFAULTS, **opts)
         U = np.array(U, dtype=np.float64).reshape(self.N, 5)
@@ -192,11 +314,15 @@ class OracleJfnk:
             if not np.isfinite(norm):
                 raise FloatingPointError("non-finite residual")
             dtau = self.local_dt(U, cfl)
-            if o["precond"] == "block":
-                Minv = np.linalg.inv(self.blocks(U, dtau))
+            if o["coupling"] == "dilu":
+                M = self.dilu(U, dtau).apply
             else:
-                Minv = np.eye(5)[None] * dtau[:, None, None]
-            M = lambda v: np.einsum("nij,nj->ni", Minv, v.reshape(self.N, 5)).reshape(-1)
+                if o["precond"] == "block":
+                    Minv = np.linalg.inv(self.blocks(U, dtau))
+                else:
+                    Minv = np.eye(5)[None] * dtau[:, None, None]
+                M = lambda v, Minv=Minv: np.einsum("nij,nj->ni", Minv,
+                                                   v.reshape(self.N, 5)).reshape(-1)
             fd_num = o["fd_eps"] * (1.0 + np.linalg.norm(U))
             b = R0.reshape(-1)
 
