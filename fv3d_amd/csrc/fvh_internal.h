@@ -47,6 +47,10 @@ int generate_box(RawMesh &m, int nx, int ny, int nz, double x0, double y0,
 int generate_channel(RawMesh &m, int nx, int ny, int nz, double jitter,
                      unsigned long long seed);
 
+// Gmsh 4.1 ASCII mesh file (gmsh_reader.cpp): parse, scale the vertices per
+// axis and preprocess into the reference layout; returns the cell count.
+int read_gmsh(RawMesh &m, const char *path, const double scale[3]);
+
 // Mesh file I/O (mesh_io.cpp): HDF5 (.h5, the reference layout) or the raw
 // container, chosen by extension when writing and by signature when
 // reading.  Throw on error.

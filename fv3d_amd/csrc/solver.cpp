@@ -1780,6 +1780,19 @@ int fvh_mesh_channel(int nx, int ny, int nz, double jitter,
     return FVH_OK;
 }
 
+int fvh_mesh_gmsh(cstring_t path, double sx, double sy, double sz)
+{
+    FVH_TRY({
+        if (!path)
+            throw std::runtime_error("fvh_mesh_gmsh: null path");
+        RawMesh m;
+        const double scale[3] = {sx, sy, sz};
+        read_gmsh(m, path, scale);
+        install_mesh(std::move(m));
+    });
+    return FVH_OK;
+}
+
 int fvh_mesh_write(cstring_t path)
 {
     FVH_TRY({

@@ -10,7 +10,8 @@ t_start, t_end, Explicit/scheme, Implicit/{scheme, max_iter, solver,
 tolerance_lsoe, max_iter_lsoe, max_krylov_dims, max_krylov_restarts}}), as
 in misc/gmsh/naca0012/naca0012_simulation.json.  Values are read from the
 "value" entries; boundary regions hold type and u, v, w, p, T.  The mesh
-file is HDF5 (.h5, the reference layout) or the raw container.
+file is HDF5 (.h5, the reference layout), the raw container, or a Gmsh 4.1
+ASCII file (.msh, preprocessed on load with unit scale).
 
     python -m fv3d_amd.driver case.json
 """
@@ -81,7 +82,8 @@ def config_from_json(params, base_dir="."):
     restart = None
     if _v(params, "Restart", "use_restart", default=False):
         restart = _v(params, "Restart", "restart_file")
-    return dict(mesh=("file", mesh_file), physics=physics, regions=regions, numerics=numerics,
+    mesh = ("gmsh", dict(path=mesh_file)) if mesh_file.endswith(".msh") else ("file", mesh_file)
+    return dict(mesh=mesh, physics=physics, regions=regions, numerics=numerics,
                 title=_v(params, "General", "title", default="untitled")), restart
 
 

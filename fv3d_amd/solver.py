@@ -162,7 +162,7 @@ class Case:
 
     mesh: ("box", dict(n=(nx,ny,nz), origin=..., length=..., tets=bool,
            periodic=mask, tile=int)) or ("channel", dict(n=..., jitter=,
-           seed=)) or ("file", path)
+           seed=)) or ("file", path) or ("gmsh", dict(path=, scale=(sx,sy,sz)))
     physics: cfl_scale, dfl_scale, mu_mix, R_mix, Pr, kappa, flux
            ("AUSM"/"Rusanov"), visc_dt_mode
     regions: {name: (type_string, (u, v, w, p, T))}
@@ -205,6 +205,9 @@ class Case:
             check(L.fvh_mesh_batch(len(p["n_hex"]), nh, p.get("length", 1.0), p.get("tile", 0)))
         elif kind == "file":
             L.init_mesh(mesh[1].encode())
+        elif kind == "gmsh":
+            p = mesh[1]
+            check(L.fvh_mesh_gmsh(p["path"].encode(), *p.get("scale", (1.0, 1.0, 1.0))))
         else:
             raise ValueError(kind)
 
