@@ -51,18 +51,13 @@ struct DiluState
     int N = 0, maxf = 0, ncol = 0;
     std::vector<int> coff;
     int *ccell = nullptr, *nbr = nullptr, *back = nullptr, *flags = nullptr;
-    double *off = nullptr, *dinv = nullptr;
-    float *off32 = nullptr, *dinv32 = nullptr;
+    double *off = nullptr, *dinv = nullptr;     // build layout
+    double *off64 = nullptr, *dinv64 = nullptr; // sweep layout, FP64
+    float *off32 = nullptr, *dinv32 = nullptr;  // sweep layout, FP32
     kry::DiluMesh view() const { return {N, maxf, ncol, ccell, nbr, back, flags}; }
 };
 DiluState g_dilu;
 
-template <class T> const T *dilu_off();
-template <> const float *dilu_off<float>() { return g_dilu.off32; }
-template <> const double *dilu_off<double>() { return g_dilu.off; }
-template <class T> const T *dilu_dinv();
-template <> const float *dilu_dinv<float>() { return g_dilu.dinv32; }
-template <> const double *dilu_dinv<double>() { return g_dilu.dinv; }
 
 template <class X> X *walloc(size_t n)
 {
@@ -127,8 +122,8 @@ void dilu_prepare(const RuntimeView &v, bool fp32)
     if (g_dilu.mesh_cf != m.cf || g_dilu.N != m.N)
     {
         const int N = m.N, F = m.F, maxf = m.maxf;
-        if (maxf > 8)
-            throw std::runtime_error("DILU: cells with more than 8 faces are not supported");
+        if (maxf > 6)
+            throw std::runtime_error("DILU: cells with more than 6 faces are not supported");
         std::vector<int> cf((size_t)N * maxf), ncf(N), fcell((size_t)2 * F);
         FVH_HIP_CHECK(hipMemcpy(cf.data(), m.cf, cf.size() * sizeof(int),
                                 hipMemcpyDeviceToHost));
@@ -182,7 +177,10 @@ void dilu_prepare(const RuntimeView &v, bool fp32)
                 ccell[pos[c]] = c;
             }
         }
-        std::vector<int> nbr((size_t)N * maxf, -1), back((size_t)N * maxf, -1), flags(N, 0);
+        // tiled tables: whole tiles of 64 positions
+        const size_t NT = ((size_t)N + 63) / 64 * 64;
+        auto tiled = [](size_t p, size_t e, size_t E) { return ((p >> 6) * E + e) * 64 + (p & 63); };
+        std::vector<int> nbr(NT * maxf, -1), back((size_t)N * maxf, -1), flags(N, 0);
         for (int p = 0; p < N; ++p)
         {
             const int c = ccell[p];
@@ -198,7 +196,7 @@ void dilu_prepare(const RuntimeView &v, bool fp32)
                         to = u;
                 if (to < 0)
                     throw std::runtime_error("DILU: inconsistent face lists");
-                nbr[(size_t)p * maxf + t] = o;
+                nbr[tiled(p, t, maxf)] = o;
                 back[(size_t)p * maxf + t] = pos[o] * maxf + to;
                 flags[p] |= col[o] < col[c] ? (1 << t) : (1 << (8 + t));
             }
@@ -210,7 +208,7 @@ void dilu_prepare(const RuntimeView &v, bool fp32)
         g_dilu.ncol = ncol;
         g_dilu.coff = coff;
         g_dilu.ccell = walloc<int>(N);
-        g_dilu.nbr = walloc<int>((size_t)N * maxf);
+        g_dilu.nbr = walloc<int>(NT * maxf);
         g_dilu.back = walloc<int>((size_t)N * maxf);
         g_dilu.flags = walloc<int>(N);
         FVH_HIP_CHECK(hipMemcpy(g_dilu.ccell, ccell.data(), (size_t)N * sizeof(int),
@@ -223,15 +221,25 @@ void dilu_prepare(const RuntimeView &v, bool fp32)
                                 hipMemcpyHostToDevice));
         g_dilu.off = walloc<double>((size_t)N * maxf * 25);
         g_dilu.dinv = walloc<double>((size_t)N * 25);
+        // slots of boundary faces and unused slots stay zero (the sweeps
+        // read every slot)
+        FVH_HIP_CHECK(hipMemset(g_dilu.off, 0, (size_t)N * maxf * 25 * sizeof(double)));
     }
+    const size_t NT = ((size_t)g_dilu.N + 63) / 64 * 64; // whole tiles
     if (fp32 && !g_dilu.off32)
     {
-        g_dilu.off32 = walloc<float>((size_t)g_dilu.N * g_dilu.maxf * 25);
-        g_dilu.dinv32 = walloc<float>((size_t)g_dilu.N * 25);
+        g_dilu.off32 = walloc<float>(NT * g_dilu.maxf * 25);
+        g_dilu.dinv32 = walloc<float>(NT * 25);
+    }
+    if (!fp32 && !g_dilu.off64)
+    {
+        g_dilu.off64 = walloc<double>(NT * g_dilu.maxf * 25);
+        g_dilu.dinv64 = walloc<double>(NT * 25);
     }
 }
 
-// The coupled preconditioner at U from the inverse diagonal blocks in Minv.
+// The coupled preconditioner at U from the inverse diagonal blocks in Minv;
+// the sweep copies of the factors in FP32 (fp32) or FP64.
 void dilu_build(const RuntimeView &v, bool fp32)
 {
     dilu_prepare(v, fp32);
@@ -244,7 +252,8 @@ void dilu_build(const RuntimeView &v, bool fp32)
                                     g_dilu.off, g_dilu.dinv, g_dilu.off32, g_dilu.dinv32);
         else
             kry::dilu_factor<double>(v.stream, d, g_dilu.coff[q], g_dilu.coff[q + 1],
-                                     g_ws.Minv, g_dilu.off, g_dilu.dinv, nullptr, nullptr);
+                                     g_ws.Minv, g_dilu.off, g_dilu.dinv, g_dilu.off64,
+                                     g_dilu.dinv64);
     }
 }
 
@@ -291,13 +300,14 @@ GmresResult gmres(const RuntimeView &v, const fvh_jfnk_options_t &o, double fd_n
     // difference quotient takes two FP32-arithmetic residuals (R0j is the
     // FP32 residual of U)
     const bool fp32 = o.precision == FVH_JFNK_FP32;
-    // z = M^-1 in with |z|^2 into dots[kZZ]
+    // z = M^-1 in with |z|^2 into dots[kZZ] (the coupled preconditioner's
+    // factors are stored in FP32 in both precisions)
     const bool coupled = o.coupling == FVH_COUPLING_DILU;
     auto apply_M = [&](const T *in, T *out) {
         if (coupled)
         {
-            kry::dilu_apply<T, T>(s, g_dilu.view(), g_dilu.coff.data(), dilu_off<T>(),
-                                  dilu_dinv<T>(), in, out);
+            kry::dilu_apply<T, float>(s, g_dilu.view(), g_dilu.coff.data(), g_dilu.off32,
+                                      g_dilu.dinv32, in, out);
             kry::multidot<T>(s, n, 1, out, ld, out, g_ws.partials, dots + kZZ);
         }
         else
@@ -491,7 +501,7 @@ int solve(const fvh_jfnk_options_t &o, fvh_jfnk_stats_t &st)
             kry::precond_blocks(s, v.m, v.P, v.flux, v.U, g_ws.dtau,
                                 o.precond == FVH_PRECOND_SCALAR ? 0 : 1, g_ws.Minv);
         if (o.coupling == FVH_COUPLING_DILU)
-            dilu_build(v, o.precision == FVH_JFNK_FP32);
+            dilu_build(v, true);
         FVH_HIP_CHECK(hipStreamSynchronize(s));
         st.precond_seconds +=
             std::chrono::duration<double>(std::chrono::steady_clock::now() - tp).count();
@@ -509,14 +519,8 @@ int solve(const fvh_jfnk_options_t &o, fvh_jfnk_stats_t &st)
         st.gmres_iterations += gr.its;
         st.residual_evals += gr.evals;
         if (o.coupling == FVH_COUPLING_DILU)
-        {
-            if (fp32)
-                kry::dilu_apply<double, float>(s, g_dilu.view(), g_dilu.coff.data(),
-                                               g_dilu.off32, g_dilu.dinv32, g_ws.xacc, g_ws.dU);
-            else
-                kry::dilu_apply<double, double>(s, g_dilu.view(), g_dilu.coff.data(),
-                                                g_dilu.off, g_dilu.dinv, g_ws.xacc, g_ws.dU);
-        }
+            kry::dilu_apply<double, float>(s, g_dilu.view(), g_dilu.coff.data(), g_dilu.off32,
+                                           g_dilu.dinv32, g_ws.xacc, g_ws.dU);
         else
             kry::precond_apply<double>(s, N, g_ws.Minv, g_ws.xacc, g_ws.dU);
         kry::update_ratio(s, N, v.U, g_ws.dU, g_ws.partials, g_ws.dots + kRatio);
@@ -666,8 +670,8 @@ int fvh_jfnk_precond_apply(double cfl, int precond, int coupling, const double *
         if (coupling == FVH_COUPLING_DILU)
         {
             dilu_build(rv, false);
-            kry::dilu_apply<double, double>(s, g_dilu.view(), g_dilu.coff.data(), g_dilu.off,
-                                            g_dilu.dinv, g_ws.xacc, g_ws.dU);
+            kry::dilu_apply<double, double>(s, g_dilu.view(), g_dilu.coff.data(),
+                                            g_dilu.off64, g_dilu.dinv64, g_ws.xacc, g_ws.dU);
         }
         else
             kry::precond_apply<double>(s, N, g_ws.Minv, g_ws.xacc, g_ws.dU);

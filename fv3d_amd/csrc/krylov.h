@@ -46,13 +46,15 @@ void precond_apply_norm(hipStream_t s, int N, const double *Minv, const T *v, T 
 // D_i = A1_ii - sum_{k lower neighbour of i} A1_ik D_k^-1 A1_ki, so that M
 // agrees with A1 on the diagonal blocks and on every face-neighbour block.
 //
-// Blocks are stored by colour-sorted position p (cell ccell[p]) and face slot
+// Blocks are built by colour-sorted position p (cell ccell[p]) and face slot
 // t (the slot order of the cell's face list): off[(p*maxf + t)*25 + r*5 + k]
-// = A1_{i,nbr} row r, column k.  nbr[p*maxf + t] is the neighbour cell or -1
-// (boundary face, or t beyond the cell's face count); back[p*maxf + t] the
-// flat slot index (pk*maxf + t') of the transposed coupling A1_{nbr,i};
-// flags[p] has bit t set for a neighbour of a lower colour and bit 8 + t for
-// one of a higher colour.
+// = A1_{i,nbr} row r, column k.  The sweeps read tiled copies: entry e of E
+// per position at ((p/64)*E + e)*64 + p%64 (offB: E = maxf*25, e = t*25 +
+// r*5 + k; dinvB: E = 25; nbr: E = maxf, e = t).  nbr holds the neighbour
+// cell or -1 (boundary face, or t beyond the cell's face count), allocated
+// for whole tiles; back[p*maxf + t] the flat slot index (pk*maxf + t') of
+// the transposed coupling A1_{nbr,i}; flags[p] has bit t set for a
+// neighbour of a lower colour and bit 8 + t for one of a higher colour.
 struct DiluMesh
 {
     int N, maxf, ncol;
@@ -67,8 +69,8 @@ void dilu_offdiag(hipStream_t s, const dev::DevMesh &m, const dev::PhysDev &P, i
 
 // Factorisation of the positions [p0, p1) of one colour (colours in
 // ascending order): dinv[p] = D_i^-1 with A1_ii = Minv[i]^-1 (Minv by cell,
-// the block-Jacobi or learned inverse blocks).  With B = float the
-// position's blocks are also converted into offB / dinvB.
+// the block-Jacobi or learned inverse blocks); the position's blocks are
+// also written, in precision B, into the component-major offB / dinvB.
 template <typename B>
 void dilu_factor(hipStream_t s, const DiluMesh &d, int p0, int p1, const double *Minv,
                  const double *off, double *dinv, B *offB, B *dinvB);

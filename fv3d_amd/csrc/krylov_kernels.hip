@@ -1,8 +1,10 @@
 // Kernels of the Newton-Krylov (JFNK) path: pseudo time steps, block-Jacobi
 // preconditioner blocks, the finite-difference operator and the
 // deterministic vector reductions of GMRES.
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdint>
 #include <stdexcept>
 
 #include "fv_physics.h"
@@ -18,6 +20,8 @@ using dev::kBlock;
 namespace {
 
 inline int blocks_for(long n) { return (int)((n + kBlock - 1) / kBlock); }
+
+inline bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
 
 __global__ __launch_bounds__(kBlock) void k_local_dt(DevMesh m, PhysDev P,
                                                      const double *__restrict__ U,
@@ -244,23 +248,65 @@ __global__ __launch_bounds__(kBlock) void k_rhs_minus(int n, const double *__res
         w[i] = (T)(b[i] - (double)w[i]);
 }
 
+// 16-byte vector of T for the streaming kernels (4 floats or 2 doubles).
+template <typename T> struct Vec16;
+template <> struct Vec16<float>
+{
+    using type = float4;
+    static constexpr int n = 4;
+    __device__ static void get(const type &v, double *o) { o[0] = v.x, o[1] = v.y, o[2] = v.z, o[3] = v.w; }
+    __device__ static type make(const double *o) { return make_float4((float)o[0], (float)o[1], (float)o[2], (float)o[3]); }
+};
+template <> struct Vec16<double>
+{
+    using type = double2;
+    static constexpr int n = 2;
+    __device__ static void get(const type &v, double *o) { o[0] = v.x, o[1] = v.y; }
+    __device__ static type make(const double *o) { return make_double2(o[0], o[1]); }
+};
+
 // Deterministic multi-dot: a fixed grid of kDotBlocks blocks, grid-stride
-// loop, per-thread FP64 sums, then a per-block column reduction in thread
-// order; the fold adds the block rows in block order.
-// KMAX >= k accumulators per thread (16, 32 or 64: fewer registers, more
-// waves in flight for the short Krylov bases).
-template <typename T, int KMAX>
-__global__ __launch_bounds__(kBlock) void k_multidot(int n, int k, const T *__restrict__ V,
+// loop over 16-byte vectors (VEC: every column and w 16-byte aligned; the
+// elements beyond the last whole vector follow one per thread), per-thread
+// FP64 sums, then a per-block column reduction in thread order; the fold
+// adds the block rows in block order.  K (1..16) columns, a compile-time
+// count so that the loads of all columns are issued together.
+template <typename T, int K, bool VEC>
+__global__ __launch_bounds__(kBlock) void k_multidot(int n, const T *__restrict__ V,
                                                      size_t ld, const T *__restrict__ w,
                                                      double *__restrict__ partials)
 {
+    constexpr int KMAX = 16, k = K;
     __shared__ double sh[kBlock * 17];
     double acc[KMAX];
 #pragma unroll
     for (int j = 0; j < KMAX; ++j)
         acc[j] = 0.0;
     const int stride = gridDim.x * blockDim.x;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    int i0 = 0;
+    if (VEC)
+    {
+        using Vt = typename Vec16<T>::type;
+        constexpr int W = Vec16<T>::n;
+        const int nv = n / W;
+        for (int i = tid; i < nv; i += stride)
+        {
+            double wv[W], vv[W];
+            Vec16<T>::get(reinterpret_cast<const Vt *>(w)[i], wv);
+#pragma unroll
+            for (int j = 0; j < KMAX; ++j)
+                if (j < k)
+                {
+                    Vec16<T>::get(reinterpret_cast<const Vt *>(V + (size_t)j * ld)[i], vv);
+#pragma unroll
+                    for (int e = 0; e < W; ++e)
+                        acc[j] += vv[e] * wv[e];
+                }
+        }
+        i0 = nv * W;
+    }
+    for (int i = i0 + tid; i < n; i += stride)
     {
         const double wi = (double)w[i];
 #pragma unroll
@@ -318,12 +364,44 @@ __global__ __launch_bounds__(kBlock) void k_fold_dots(int nblk, const double *__
         dots[j] = s;
 }
 
-template <typename T>
+// w -= sum_j h[j] V_j, one 16-byte vector per thread (VEC, aligned columns)
+// or one element
+template <typename T, bool VEC>
 __global__ __launch_bounds__(kBlock) void k_multiaxpy(int n, int k, const T *__restrict__ V,
                                                       size_t ld, const double *__restrict__ h,
                                                       T *__restrict__ w)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (VEC)
+    {
+        using Vt = typename Vec16<T>::type;
+        constexpr int W = Vec16<T>::n;
+        const int nv = n / W;
+        if (t < nv)
+        {
+            double a[W], vv[W];
+            Vec16<T>::get(reinterpret_cast<const Vt *>(w)[t], a);
+            for (int j = 0; j < k; ++j)
+            {
+                const double hj = h[j];
+                Vec16<T>::get(reinterpret_cast<const Vt *>(V + (size_t)j * ld)[t], vv);
+#pragma unroll
+                for (int e = 0; e < W; ++e)
+                    a[e] -= hj * vv[e];
+            }
+            reinterpret_cast<Vt *>(w)[t] = Vec16<T>::make(a);
+            return;
+        }
+        const int i = nv * W + (t - nv);
+        if (i >= n)
+            return;
+        double a = (double)w[i];
+        for (int j = 0; j < k; ++j)
+            a -= h[j] * (double)V[(size_t)j * ld + i];
+        w[i] = (T)a;
+        return;
+    }
+    const int i = t;
     if (i >= n)
         return;
     double a = (double)w[i];
@@ -439,6 +517,16 @@ __global__ __launch_bounds__(kBlock) void k_update(int n, double *__restrict__ U
 }
 
 // ---- multicolour block ILU(0) -----------------------------------------------
+
+// Tiled component-major index of entry e (of E per position) of position p:
+// tiles of 64 positions, within a tile entry-major, so that a wavefront's
+// loads of one entry are contiguous and its whole working set is one
+// contiguous run of 64*E values.
+__device__ __forceinline__ size_t tiled(int p, int e, int E)
+{
+    return ((size_t)(p >> 6) * E + e) * 64 + (p & 63);
+}
+
 
 // Off-diagonal blocks A1_{nb,c} = -dR1_nb/dU_c = (s_cf A_f / V_nb) dF_f/dU_c
 // (s_cf = +1 when c is the face's first cell), one thread per position; the
@@ -579,113 +667,93 @@ __global__ __launch_bounds__(kBlock) void k_dilu_factor(DiluMesh d, int p0, int 
     dev::invert5(D, X);
 #pragma unroll
     for (int q = 0; q < 25; ++q)
-        dinv[(size_t)p * 25 + q] = X[q];
-    if (dinvB)
     {
-#pragma unroll
-        for (int q = 0; q < 25; ++q)
-            dinvB[(size_t)p * 25 + q] = (B)X[q];
-        for (int q = 0; q < d.maxf * 25; ++q)
-            offB[(size_t)p * d.maxf * 25 + q] = (B)off[(size_t)p * d.maxf * 25 + q];
+        dinv[(size_t)p * 25 + q] = X[q];
+        dinvB[tiled(p, q, 25)] = (B)X[q];
     }
+    for (int q = 0; q < d.maxf * 25; ++q)
+        offB[tiled(p, q, d.maxf * 25)] = (B)off[(size_t)p * d.maxf * 25 + q];
 }
 
-// forward sweep of one colour: z_i = D_i^-1 (v_i - sum_{k lower} A_ik z_k)
-template <typename T, typename B>
-__global__ __launch_bounds__(kBlock) void k_dilu_forward(DiluMesh d, int p0, int p1,
-                                                         const B *__restrict__ off,
-                                                         const B *__restrict__ dinv,
-                                                         const T *__restrict__ v,
-                                                         T *__restrict__ z)
+// One colour of the sweeps (blocks and neighbour table in the tiled
+// layout).  Every face slot is loaded unconditionally (straight-line code, so
+// that the loads of all slots are in flight together): blocks of boundary
+// and unused slots are zero, a missing neighbour reads cell 0, and the
+// neighbour values of slots outside the sweep's set (the other direction
+// included) are replaced by zero, so those slots contribute exactly zero.
+//   FORWARD: z_i = D_i^-1 (v_i - sum_{k lower} A_ik z_k)
+//   backward: z_i -= D_i^-1 sum_{k higher} A_ik z_k
+template <typename T, typename B, int MAXF, bool FORWARD>
+__global__ __launch_bounds__(kBlock) void k_dilu_sweep(DiluMesh d, int p0, int p1,
+                                                       const B *__restrict__ off,
+                                                       const B *__restrict__ dinv,
+                                                       const T *__restrict__ v,
+                                                       T *__restrict__ z)
 {
     const int p = p0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= p1)
         return;
+    const int mask = FORWARD ? (d.flags[p] & 0xff) : ((d.flags[p] >> 8) & 0xff);
+    if (!FORWARD && !mask)
+        return;
     const int i = d.ccell[p];
-    double r[5];
+    const int E = d.maxf * 25;
+    double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int j = 0; j < 5; ++j)
-        r[j] = (double)v[(size_t)i * 5 + j];
-    const int lo = d.flags[p] & 0xff;
-#pragma unroll 1
-    for (int t = 0; t < d.maxf; ++t)
+    for (int t = 0; t < MAXF; ++t)
     {
-        if (!((lo >> t) & 1))
-            continue;
-        const int k = d.nbr[p * d.maxf + t];
-        const B *A = off + ((size_t)p * d.maxf + t) * 25;
+        const bool on = (mask >> t) & 1;
+        const int kn = d.nbr[tiled(p, t, d.maxf)];
+        const int k = kn < 0 ? 0 : kn;
         double y[5];
 #pragma unroll
         for (int j = 0; j < 5; ++j)
-            y[j] = (double)z[(size_t)k * 5 + j];
-#pragma unroll
-        for (int a = 0; a < 5; ++a)
         {
-            double acc = 0.0;
-#pragma unroll
-            for (int b = 0; b < 5; ++b)
-                acc += (double)A[a * 5 + b] * y[b];
-            r[a] -= acc;
+            const double zk = (double)z[(size_t)k * 5 + j];
+            y[j] = on ? zk : 0.0;
         }
-    }
-    const B *Di = dinv + (size_t)p * 25;
-#pragma unroll
-    for (int a = 0; a < 5; ++a)
-    {
-        double acc = 0.0;
-#pragma unroll
-        for (int b = 0; b < 5; ++b)
-            acc += (double)Di[a * 5 + b] * r[b];
-        z[(size_t)i * 5 + a] = (T)acc;
-    }
-}
-
-// backward sweep of one colour: z_i -= D_i^-1 sum_{k higher} A_ik z_k
-template <typename T, typename B>
-__global__ __launch_bounds__(kBlock) void k_dilu_backward(DiluMesh d, int p0, int p1,
-                                                          const B *__restrict__ off,
-                                                          const B *__restrict__ dinv,
-                                                          T *__restrict__ z)
-{
-    const int p = p0 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= p1)
-        return;
-    const int hi = (d.flags[p] >> 8) & 0xff;
-    if (!hi)
-        return;
-    const int i = d.ccell[p];
-    double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-#pragma unroll 1
-    for (int t = 0; t < d.maxf; ++t)
-    {
-        if (!((hi >> t) & 1))
-            continue;
-        const int k = d.nbr[p * d.maxf + t];
-        const B *A = off + ((size_t)p * d.maxf + t) * 25;
-        double x[5];
-#pragma unroll
-        for (int j = 0; j < 5; ++j)
-            x[j] = (double)z[(size_t)k * 5 + j];
 #pragma unroll
         for (int a = 0; a < 5; ++a)
         {
             double acc = 0.0;
 #pragma unroll
             for (int b = 0; b < 5; ++b)
-                acc += (double)A[a * 5 + b] * x[b];
+                acc += (double)off[tiled(p, t * 25 + a * 5 + b, E)] * y[b];
             r[a] += acc;
         }
     }
-    const B *Di = dinv + (size_t)p * 25;
+    if (FORWARD)
+    {
+#pragma unroll
+        for (int j = 0; j < 5; ++j)
+            r[j] = (double)v[(size_t)i * 5 + j] - r[j];
+    }
+    double out[5];
 #pragma unroll
     for (int a = 0; a < 5; ++a)
     {
         double acc = 0.0;
 #pragma unroll
         for (int b = 0; b < 5; ++b)
-            acc += (double)Di[a * 5 + b] * r[b];
-        z[(size_t)i * 5 + a] = (T)((double)z[(size_t)i * 5 + a] - acc);
+            acc += (double)dinv[tiled(p, a * 5 + b, 25)] * r[b];
+        out[a] = acc;
     }
+#pragma unroll
+    for (int a = 0; a < 5; ++a)
+        z[(size_t)i * 5 + a] =
+            FORWARD ? (T)out[a] : (T)((double)z[(size_t)i * 5 + a] - out[a]);
+}
+
+// k_multidot<T, K, VEC> for a run-time K in 1..16
+template <typename T, bool VEC, int K = 1>
+void multidot_launch(hipStream_t s, int k, int n, const T *V, size_t ld, const T *w,
+                     double *partials)
+{
+    if (k == K)
+        hipLaunchKernelGGL((k_multidot<T, K, VEC>), dim3(kDotBlocks), dim3(kBlock), 0, s, n, V,
+                           ld, w, partials);
+    else if constexpr (K < 16)
+        multidot_launch<T, VEC, K + 1>(s, k, n, V, ld, w, partials);
 }
 
 } // namespace
@@ -772,23 +840,35 @@ void multidot(hipStream_t s, int n, int k, const T *V, size_t ld, const T *w,
 {
     if (k < 1 || k > kMaxDots)
         throw std::runtime_error("multidot: bad vector count");
-    if (k <= 16)
-        hipLaunchKernelGGL((k_multidot<T, 16>), dim3(kDotBlocks), dim3(kBlock), 0, s, n, k, V, ld,
-                           w, partials);
-    else if (k <= 32)
-        hipLaunchKernelGGL((k_multidot<T, 32>), dim3(kDotBlocks), dim3(kBlock), 0, s, n, k, V, ld,
-                           w, partials);
-    else
-        hipLaunchKernelGGL((k_multidot<T, 64>), dim3(kDotBlocks), dim3(kBlock), 0, s, n, k, V, ld,
-                           w, partials);
+    // at most 16 columns per launch (w is read once per launch): 16
+    // accumulators keep the kernel at a high occupancy, which measured
+    // faster than one launch with 32 or 64 accumulators; each column's sum
+    // is the same in either form
+    const bool vec = aligned16(V) && aligned16(w) && (ld * sizeof(T)) % 16 == 0;
+    for (int j0 = 0; j0 < k; j0 += 16)
+    {
+        const int kk = std::min(16, k - j0);
+        if (vec)
+            multidot_launch<T, true>(s, kk, n, V + (size_t)j0 * ld, ld, w, partials + j0);
+        else
+            multidot_launch<T, false>(s, kk, n, V + (size_t)j0 * ld, ld, w, partials + j0);
+    }
     hipLaunchKernelGGL(k_fold_dots, dim3(k), dim3(kBlock), 0, s, kDotBlocks, partials, dots);
 }
 
 template <typename T>
 void multiaxpy(hipStream_t s, int n, int k, const T *V, size_t ld, const double *h, T *w)
 {
-    hipLaunchKernelGGL(k_multiaxpy<T>, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, k, V, ld,
-                       h, w);
+    if (aligned16(V) && aligned16(w) && (ld * sizeof(T)) % 16 == 0)
+    {
+        constexpr int W = 16 / sizeof(T);
+        const int threads = n / W + n % W;
+        hipLaunchKernelGGL((k_multiaxpy<T, true>), dim3(blocks_for(threads)), dim3(kBlock), 0, s,
+                           n, k, V, ld, h, w);
+    }
+    else
+        hipLaunchKernelGGL((k_multiaxpy<T, false>), dim3(blocks_for(n)), dim3(kBlock), 0, s, n,
+                           k, V, ld, h, w);
 }
 
 template <typename T>
@@ -871,20 +951,33 @@ void dilu_factor(hipStream_t s, const DiluMesh &d, int p0, int p1, const double 
                        p1, Minv, off, dinv, offB, dinvB);
 }
 
+template <typename T, typename B, int MAXF>
+void dilu_sweeps(hipStream_t s, const DiluMesh &d, const int *coff, const B *off,
+                 const B *dinv, const T *v, T *z)
+{
+    for (int q = 0; q < d.ncol; ++q)
+        if (coff[q + 1] > coff[q])
+            hipLaunchKernelGGL((k_dilu_sweep<T, B, MAXF, true>),
+                               dim3(blocks_for(coff[q + 1] - coff[q])), dim3(kBlock), 0, s, d,
+                               coff[q], coff[q + 1], off, dinv, v, z);
+    // the highest colour has no higher neighbours
+    for (int q = d.ncol - 2; q >= 0; --q)
+        if (coff[q + 1] > coff[q])
+            hipLaunchKernelGGL((k_dilu_sweep<T, B, MAXF, false>),
+                               dim3(blocks_for(coff[q + 1] - coff[q])), dim3(kBlock), 0, s, d,
+                               coff[q], coff[q + 1], off, dinv, v, z);
+}
+
 template <typename T, typename B>
 void dilu_apply(hipStream_t s, const DiluMesh &d, const int *coff, const B *off,
                 const B *dinv, const T *v, T *z)
 {
-    for (int q = 0; q < d.ncol; ++q)
-        if (coff[q + 1] > coff[q])
-            hipLaunchKernelGGL((k_dilu_forward<T, B>), dim3(blocks_for(coff[q + 1] - coff[q])),
-                               dim3(kBlock), 0, s, d, coff[q], coff[q + 1], off, dinv, v, z);
-    // the highest colour has no higher neighbours
-    for (int q = d.ncol - 2; q >= 0; --q)
-        if (coff[q + 1] > coff[q])
-            hipLaunchKernelGGL((k_dilu_backward<T, B>),
-                               dim3(blocks_for(coff[q + 1] - coff[q])), dim3(kBlock), 0, s, d,
-                               coff[q], coff[q + 1], off, dinv, z);
+    if (d.maxf <= 4)
+        dilu_sweeps<T, B, 4>(s, d, coff, off, dinv, v, z);
+    else if (d.maxf <= 6)
+        dilu_sweeps<T, B, 6>(s, d, coff, off, dinv, v, z);
+    else
+        throw std::runtime_error("DILU: cells with more than 6 faces are not supported");
 }
 
 template void dilu_factor<double>(hipStream_t, const DiluMesh &, int, int, const double *,

@@ -146,7 +146,8 @@ int fvh_device_info(char *name, int name_len, int *cu_count,
                                 neighbour blocks by differences of the shared
                                 face flux, cells ordered by a distance-1
                                 colouring; M agrees with A1 on every diagonal
-                                and face-neighbour block */
+                                and face-neighbour block.  The solve stores
+                                the factors in FP32 (in both precisions) */
 
 typedef struct
 {
@@ -205,10 +206,10 @@ int fvh_jfnk_history(double *rows, int max_rows);
  * minv[N][5][5] (row-major) of M_c = I/dtau_c - dR1_c/dU_c (precond
  * FVH_PRECOND_BLOCK) or of I/dtau_c (FVH_PRECOND_SCALAR). */
 int fvh_jfnk_blocks(double cfl, int precond, double *dtau, double *minv);
-/* z = M^-1 v (host arrays of 5N values, FP64) for the preconditioner that a
- * Newton step at the current state and the given CFL builds with precond /
- * coupling (FVH_PRECOND_LEARNED needs a network; it is applied without the
- * block-error check). */
+/* z = M^-1 v (host arrays of 5N values) for the preconditioner that a Newton
+ * step at the current state and the given CFL builds with precond / coupling
+ * (FVH_PRECOND_LEARNED needs a network; it is applied without the
+ * block-error check), with the factors kept in FP64. */
 int fvh_jfnk_precond_apply(double cfl, int precond, int coupling, const double *v, double *z);
 
 /* ---- batched cases ------------------------------------------------------ */
