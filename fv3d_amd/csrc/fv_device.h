@@ -72,6 +72,7 @@ struct GradOut
 {
     double *vg;   // [N*12] viscous gradient subset
     double *fs;   // [F*16] face states
+    float *fs32;  // [F*16] face states in FP32 (the FP32 kernels write these)
     double *grad; // [N*30] all gradients (tests)
     double *mnmx; // [N*20] neighbourhood min/max (tests)
     int c_begin, c_end; // cells [c_begin, c_end); c_end = 0: all cells
@@ -86,6 +87,7 @@ struct StageParams
     double *R_out;         // may be null
     const double *vg;      // [N*12] when viscous
     const double *fs;      // [F*16] face states (reconstructed path)
+    const float *fs32;     // [F*16] FP32 face states (the FP32 kernels read these)
     const double *gphi;    // [B*10] ghost rows (first-order path)
     double rk_a, rk_g;
     int stage0;            // first stage: k = R
@@ -121,7 +123,7 @@ enum FluxKind
 // faces in fs when fs is not null).  No-op without boundaries.
 void launch_ghost(hipStream_t s, const DevMesh &m, const PhysDev &P,
                   const double *U, double *gphi, double *fs,
-                  const DevScalars *sc);
+                  const DevScalars *sc, float *fs32 = nullptr);
 // linear: limited linear face states; otherwise the cell values.
 // fp32: the arithmetic in FP32 (inputs and outputs stay FP64 arrays), for
 // the FP32 Jacobian-vector products of the Newton-Krylov solve.

@@ -28,6 +28,8 @@
 #include <cmath>
 #include <stdexcept>
 
+#include <type_traits>
+
 #include "fv_physics.h"
 
 // Minimum resident waves per SIMD requested from the register allocator for
@@ -55,7 +57,7 @@ namespace dev {
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void fv_ghost_kernel(
     DevMesh m, PhysDev P, const double *__restrict__ U, double *__restrict__ gphi,
-    double *__restrict__ fs, const DevScalars *sc)
+    double *__restrict__ fs, const DevScalars *sc, float *__restrict__ fs32)
 {
     if (sc && sc->done)
         return;
@@ -77,6 +79,10 @@ __global__ __launch_bounds__(kBlock) void fv_ghost_kernel(
 #pragma unroll
         for (int j = 0; j < 8; ++j)
             fs[(size_t)f * 16 + 8 + j] = g[j];
+    if (fs32)
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            fs32[(size_t)f * 16 + 8 + j] = (float)g[j];
 }
 
 // ---------------------------------------------------------------------------
@@ -210,7 +216,7 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
         }
     }
 
-    if (!out.fs)
+    if (std::is_same<Real, float>::value ? !out.fs32 : !out.fs)
         return;
     // this cell's reconstructed state on each of its faces; the distance
     // vectors of mesh.c:44-56 (dist_cell_1 = x_f - x_c0, dist_cell_2 =
@@ -253,11 +259,20 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
             for (int j = 0; j < 8; ++j)
                 st[j] = pc[j];
         }
-        double2 *d = reinterpret_cast<double2 *>(out.fs + (size_t)f * 16 + (own0 ? 0 : 8));
-        d[0] = make_double2(double(st[0]), double(st[1]));
-        d[1] = make_double2(double(st[2]), double(st[3]));
-        d[2] = make_double2(double(st[4]), double(st[5]));
-        d[3] = make_double2(double(st[6]), double(st[7]));
+        if constexpr (std::is_same<Real, float>::value)
+        {
+            float4 *d = reinterpret_cast<float4 *>(out.fs32 + (size_t)f * 16 + (own0 ? 0 : 8));
+            d[0] = make_float4(st[0], st[1], st[2], st[3]);
+            d[1] = make_float4(st[4], st[5], st[6], st[7]);
+        }
+        else
+        {
+            double2 *d = reinterpret_cast<double2 *>(out.fs + (size_t)f * 16 + (own0 ? 0 : 8));
+            d[0] = make_double2(double(st[0]), double(st[1]));
+            d[1] = make_double2(double(st[2]), double(st[3]));
+            d[2] = make_double2(double(st[4]), double(st[5]));
+            d[3] = make_double2(double(st[6]), double(st[7]));
+        }
     }
 }
 
@@ -309,15 +324,29 @@ __global__ __launch_bounds__(kBlock, FVH_K2_MINW) void fv_residual_group_kernel(
         Real L[10], R[10];
         if (FS)
         {
-            const double2 *p = reinterpret_cast<const double2 *>(sp.fs + (size_t)f * 16);
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
+            if constexpr (std::is_same<Real, float>::value)
             {
-                const double2 a = p[q], b = p[4 + q];
-                L[2 * q] = Real(a.x);
-                L[2 * q + 1] = Real(a.y);
-                R[2 * q] = Real(b.x);
-                R[2 * q + 1] = Real(b.y);
+                const float4 *p = reinterpret_cast<const float4 *>(sp.fs32 + (size_t)f * 16);
+#pragma unroll
+                for (int q = 0; q < 2; ++q)
+                {
+                    const float4 a = p[q], b = p[2 + q];
+                    L[4 * q] = a.x, L[4 * q + 1] = a.y, L[4 * q + 2] = a.z, L[4 * q + 3] = a.w;
+                    R[4 * q] = b.x, R[4 * q + 1] = b.y, R[4 * q + 2] = b.z, R[4 * q + 3] = b.w;
+                }
+            }
+            else
+            {
+                const double2 *p = reinterpret_cast<const double2 *>(sp.fs + (size_t)f * 16);
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                {
+                    const double2 a = p[q], b = p[4 + q];
+                    L[2 * q] = Real(a.x);
+                    L[2 * q + 1] = Real(a.y);
+                    R[2 * q] = Real(b.x);
+                    R[2 * q + 1] = Real(b.y);
+                }
             }
             L[8] = L[9] = R[8] = R[9] = Real(0.0);
         }
@@ -705,12 +734,12 @@ int dt_partials_grid(const DevMesh &m) { return cell_blocks(m.N); }
 
 void launch_ghost(hipStream_t s, const DevMesh &m, const PhysDev &P,
                   const double *U, double *gphi, double *fs,
-                  const DevScalars *sc)
+                  const DevScalars *sc, float *fs32)
 {
     if (m.B == 0)
         return;
     hipLaunchKernelGGL(fv_ghost_kernel, dim3(cell_blocks(m.B)), dim3(kBlock), 0,
-                       s, m, P, U, gphi, fs, sc);
+                       s, m, P, U, gphi, fs, sc, fs32);
 }
 
 void launch_gradient(hipStream_t s, const DevMesh &m, const PhysDev &P,

@@ -156,6 +156,7 @@ struct Device
     double *gphi = nullptr; // [10*B] ghost rows
     double *vg = nullptr;   // [12*N] viscous gradient subset
     double *fs = nullptr;   // [16*F] face states
+    float *fs32 = nullptr;  // [16*F] face states of the FP32 residual
     double *grad = nullptr; // [30*N] all gradients (tests only)
     double *mnmx = nullptr; // [20*N] min/max (tests only)
     double *partials = nullptr;
@@ -330,12 +331,22 @@ void enqueue_residual(const double *U_in, double *U_out, double *R_out,
 {
     const dev::PhysDev P = phys();
     ensure_gradient_buffers();
+    // the FP32 residual keeps its face states in FP32
+    const bool fs32 = fp32 && is_linear();
+    if (fs32 && !g_dev.fs32)
+        g_dev.fs32 = dmalloc<float>(16 * (size_t)g_dev.m.F);
     dev::launch_ghost(g_dev.stream, g_dev.m, P, U_in, g_dev.gphi,
-                      is_linear() ? g_dev.fs : nullptr, sc);
+                      is_linear() && !fs32 ? g_dev.fs : nullptr, sc,
+                      fs32 ? g_dev.fs32 : nullptr);
     if (needs_gradients())
     {
         dev::GradOut go = grad_out();
         go.prims = prims_in;
+        if (fs32)
+        {
+            go.fs = nullptr;
+            go.fs32 = g_dev.fs32;
+        }
         if (tm)
             tm->start();
         dev::launch_gradient(g_dev.stream, g_dev.m, P, U_in, g_dev.gphi, go,
@@ -350,6 +361,7 @@ void enqueue_residual(const double *U_in, double *U_out, double *R_out,
     sp.R_out = R_out;
     sp.vg = g_dev.vg;
     sp.fs = g_dev.fs;
+    sp.fs32 = fs32 ? g_dev.fs32 : nullptr;
     sp.gphi = g_dev.gphi;
     sp.rk_a = update ? g_rk_a[stage] : 0.0;
     sp.rk_g = update ? g_rk_g[stage] : 0.0;
