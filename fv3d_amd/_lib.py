@@ -21,7 +21,8 @@ class JfnkOptions(ctypes.Structure):
                 ("fd_eps", c_double), ("fd_eps32", c_double), ("max_change", c_double),
                 ("precision", c_int),
                 ("precond", c_int), ("verbose", c_int), ("learned_check", c_int),
-                ("learned_tol", c_double), ("coupling", c_int)]
+                ("learned_tol", c_double), ("coupling", c_int),
+                ("reorth", c_double)]
 
 
 class JfnkStats(ctypes.Structure):

@@ -31,13 +31,19 @@ struct Workspace
            *dots = nullptr, *ydev = nullptr, *Mex = nullptr;
     void *V = nullptr, *z = nullptr, *t = nullptr; // T storage
     double *host = nullptr;                        // pinned, kHost doubles
+    int *again = nullptr;                          // device flag of the Arnoldi step
+    hipEvent_t ev[2] = {nullptr, nullptr};         // Arnoldi slot copies done
 };
 
-constexpr int kHost = 4 * kry::kMaxDots + 64;
-// offsets in dots / host
+// offsets in dots (and in each host slot)
 constexpr int kH1 = 0, kH2 = kry::kMaxDots, kWW = 2 * kry::kMaxDots,
               kZZ = kWW + 1, kSums = kWW + 2, kRatio = kWW + 8, kEps = kWW + 9,
-              kUU = kWW + 10;
+              kUU = kWW + 10, kAr = kWW + 12, kWW2 = kWW + 16;
+// pinned host memory: two slots of kCopy doubles for the Arnoldi steps in
+// flight (slot 0 also serves the synchronous fetches), then the GMRES
+// solution y
+constexpr int kCopy = kWW2 + 1, kSlot = kCopy + 15, kY = 2 * kSlot;
+constexpr int kHost = kY + kry::kMaxDots;
 
 Workspace g_ws;
 std::vector<double> g_hist;
@@ -73,6 +79,9 @@ void release()
         (void)hipFree(p);
     if (g_ws.host)
         (void)hipHostFree(g_ws.host);
+    for (hipEvent_t e : g_ws.ev)
+        if (e)
+            (void)hipEventDestroy(e);
     g_ws = Workspace();
     g_dilu = DiluState();
 }
@@ -104,6 +113,9 @@ void prepare(int N, int m, int prec)
     g_ws.t = walloc<char>(g_ws.ld * es);
     FVH_HIP_CHECK(hipHostMalloc((void **)&g_ws.host, kHost * sizeof(double),
                                 hipHostMallocDefault));
+    g_ws.again = walloc<int>(1);
+    for (hipEvent_t &e : g_ws.ev)
+        FVH_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 }
 
 void fetch(hipStream_t s, int off, int count)
@@ -348,40 +360,48 @@ GmresResult gmres(const RuntimeView &v, const fvh_jfnk_options_t &o, double fd_n
         g[0] = beta;
         int k = 0;
         bool done = false;
-        for (int j = 0; j < m; ++j)
-        {
+        const double tiny = 1e-300 * beta0;
+        // One Arnoldi step, enqueued whole: w = A M^-1 V_j; classical
+        // Gram-Schmidt in one pass with the norm by Pythagoras (w is V_{j+1},
+        // so the same multi-dot gives <w, w>); a second pass, run only when
+        // the device finds that the new vector kept at most the fraction
+        // o.reorth of its square norm (0.5: the 1/sqrt(2) criterion of Daniel,
+        // Gragg, Kaufman and Stewart); w scaled by 1/hn on the device; the dots and
+        // hn copied to host slot j % 2.  Step j + 1 is enqueued before the
+        // host reads step j, so the device never waits for the host; when
+        // step j ends the cycle, step j + 1 is wasted work that nothing reads.
+        auto launch_step = [&](int j) {
             T *w = V + (size_t)(j + 1) * ld;
             apply_M(V + (size_t)j * ld, z);
             apply_op(z, w);
-            // classical Gram-Schmidt with one pass and the norm by Pythagoras
-            // (w is V_{j+1}, so the same multi-dot gives <w, w>); a second
-            // pass when the new vector lost more than half its square norm
-            // to cancellation (the 1/sqrt(2) criterion of Daniel, Gragg,
-            // Kaufman and Stewart)
             kry::multidot<T>(s, n, j + 2, V, ld, w, g_ws.partials, dots + kH1);
             kry::multiaxpy<T>(s, n, j + 1, V, ld, dots + kH1, w);
-            fetch(s, 0, j + 2);
+            kry::arnoldi_norm(s, j + 1, dots + kH1, dots + kH1 + j + 1, dots + kAr, g_ws.again,
+                              o.reorth);
+            kry::multidot<T>(s, n, j + 2, V, ld, w, g_ws.partials, dots + kH2, g_ws.again);
+            kry::multiaxpy<T>(s, n, j + 1, V, ld, dots + kH2, w, g_ws.again);
+            kry::multidot<T>(s, n, 1, w, ld, w, g_ws.partials, dots + kWW2, g_ws.again);
+            kry::arnoldi_finish(s, g_ws.again, dots + kWW2, dots + kAr, tiny);
+            kry::scale_dev<T>(s, n, w, dots + kAr + 3);
+            FVH_HIP_CHECK(hipMemcpyAsync(host + (j & 1) * kSlot, dots, kCopy * sizeof(double),
+                                         hipMemcpyDeviceToHost, s));
+            FVH_HIP_CHECK(hipEventRecord(g_ws.ev[j & 1], s));
+        };
+        launch_step(0);
+        for (int j = 0; j < m; ++j)
+        {
+            if (j + 1 < m)
+                launch_step(j + 1);
+            FVH_HIP_CHECK(hipEventSynchronize(g_ws.ev[j & 1]));
+            const double *hb = host + (j & 1) * kSlot;
             ++res.its;
-            const double ww0 = host[kH1 + j + 1];
-            double hh = 0.0;
-            for (int i = 0; i <= j; ++i)
-                hh += host[kH1 + i] * host[kH1 + i];
-            double hn2 = ww0 - hh;
-            for (int i = 0; i <= j; ++i)
-                host[kH2 + i] = 0.0;
-            if (!(hn2 > 0.5 * ww0))
-            {
-                kry::multidot<T>(s, n, j + 2, V, ld, w, g_ws.partials, dots + kH2);
-                kry::multiaxpy<T>(s, n, j + 1, V, ld, dots + kH2, w);
-                kry::multidot<T>(s, n, 1, w, ld, w, g_ws.partials, dots + kWW);
-                fetch(s, kH2, kry::kMaxDots + 1);
-                hn2 = host[kWW];
+            const bool again = hb[kAr + 1] != 0.0;
+            if (again)
                 ++res.reorth;
-            }
             double *Hj = &H[(size_t)j * (m + 1)]; // column j
             for (int i = 0; i <= j; ++i)
-                Hj[i] = host[kH1 + i] + host[kH2 + i];
-            const double hn = std::sqrt(hn2 > 0.0 ? hn2 : 0.0);
+                Hj[i] = hb[kH1 + i] + (again ? hb[kH2 + i] : 0.0);
+            const double hn = hb[kAr + 2];
             Hj[j + 1] = hn;
             for (int i = 0; i < j; ++i)
             {
@@ -401,12 +421,11 @@ GmresResult gmres(const RuntimeView &v, const fvh_jfnk_options_t &o, double fd_n
             res.rel = std::fabs(g[j + 1]) / beta0;
             if (!std::isfinite(res.rel))
                 throw std::runtime_error("GMRES: non-finite Arnoldi coefficients");
-            if (res.rel <= o.rtol || !(hn > 1e-300 * beta0))
+            if (res.rel <= o.rtol || !(hn > tiny))
             {
                 done = true;
                 break;
             }
-            kry::scale<T>(s, n, w, 1.0 / hn, w);
         }
         // y = H^-1 g (upper triangular, k x k)
         for (int i = k - 1; i >= 0; --i)
@@ -417,8 +436,8 @@ GmresResult gmres(const RuntimeView &v, const fvh_jfnk_options_t &o, double fd_n
             y[i] = a / H[(size_t)i * (m + 1) + i];
         }
         for (int i = 0; i < k; ++i)
-            host[kH1 + i] = y[i];
-        FVH_HIP_CHECK(hipMemcpyAsync(g_ws.ydev, host + kH1, k * sizeof(double),
+            host[kY + i] = y[i];
+        FVH_HIP_CHECK(hipMemcpyAsync(g_ws.ydev, host + kY, k * sizeof(double),
                                      hipMemcpyHostToDevice, s));
         kry::combine<T>(s, n, k, V, ld, g_ws.ydev, t);
         kry::accumulate<T>(s, n, t, g_ws.xacc);
@@ -589,6 +608,7 @@ int fvh_jfnk_default_options(fvh_jfnk_options_t *o)
     o->learned_check = 10;
     o->learned_tol = 0.2;
     o->coupling = FVH_COUPLING_NONE;
+    o->reorth = 0.5;
     return FVH_OK;
 }
 

@@ -101,15 +101,29 @@ template <typename T>
 void rhs_minus(hipStream_t s, int n, const double *b, T *w);
 
 // dots[j] = <V_j, w> for j < k, V_j at V + j*ld; FP64 accumulation in a fixed
-// order.  partials holds kDotBlocks*kMaxDots doubles.
+// order.  partials holds kDotBlocks*kMaxDots doubles.  With a device flag
+// `on`, the kernels do nothing when *on == 0 (a pass decided on the device).
 template <typename T>
 void multidot(hipStream_t s, int n, int k, const T *V, size_t ld, const T *w,
-              double *partials, double *dots);
+              double *partials, double *dots, const int *on = nullptr);
 
-// w -= sum_j h[j] V_j  (h in device memory)
+// w -= sum_j h[j] V_j  (h in device memory; `on` as for multidot)
 template <typename T>
 void multiaxpy(hipStream_t s, int n, int k, const T *V, size_t ld,
-               const double *h, T *w);
+               const double *h, T *w, const int *on = nullptr);
+
+// Arnoldi step bookkeeping on the device (one thread each):
+// arnoldi_norm: ar[0] = ww[0] - sum_{i<k} h[i]^2, ar[1] = *again = whether a
+// second Gram-Schmidt pass is needed (ar[0] <= eta ww); arnoldi_finish: after
+// that pass (run or skipped), ar[0] = ww2[0] if it ran, ar[2] = hn =
+// sqrt(max(ar[0], 0)), ar[3] = 1/hn (1 when hn <= tiny).
+void arnoldi_norm(hipStream_t s, int k, const double *h, const double *ww, double *ar,
+                  int *again, double eta);
+void arnoldi_finish(hipStream_t s, const int *again, const double *ww2, double *ar, double tiny);
+
+// w *= sc[0] (sc in device memory)
+template <typename T>
+void scale_dev(hipStream_t s, int n, T *w, const double *sc);
 
 // out = w * scale
 template <typename T>

@@ -248,6 +248,52 @@ __global__ __launch_bounds__(kBlock) void k_rhs_minus(int n, const double *__res
         w[i] = (T)(b[i] - (double)w[i]);
 }
 
+// Arnoldi step bookkeeping on the device (one thread), so that the next
+// step can be enqueued before the host has read this one:
+//   arnoldi_norm: ar[0] = |w|^2 - sum_i h_i^2 (classical Gram-Schmidt with
+//     the norm by Pythagoras; the sum in index order), ar[1] = 1 and
+//     *again = 1 when the new vector kept at most the fraction eta of its
+//     square norm (a second pass is needed), else 0;
+//   arnoldi_finish: after the (skipped or run) second pass, ar[0] = |w|^2 of
+//     that pass when it ran, ar[2] = hn = sqrt(max(ar[0], 0)) and ar[3] the
+//     scale of w: 1/hn, or 1 when hn <= tiny (breakdown; the host stops).
+__global__ void k_arnoldi_norm(int k, const double *__restrict__ h,
+                               const double *__restrict__ ww, double *__restrict__ ar,
+                               int *__restrict__ again, double eta)
+{
+    if (threadIdx.x != 0)
+        return;
+    double hh = 0.0;
+    for (int i = 0; i < k; ++i)
+        hh += h[i] * h[i];
+    const double hn2 = ww[0] - hh;
+    const int a = !(hn2 > eta * ww[0]);
+    ar[0] = hn2;
+    ar[1] = a ? 1.0 : 0.0;
+    *again = a;
+}
+
+__global__ void k_arnoldi_finish(const int *__restrict__ again, const double *__restrict__ ww2,
+                                 double *__restrict__ ar, double tiny)
+{
+    if (threadIdx.x != 0)
+        return;
+    const double hn2 = *again ? ww2[0] : ar[0];
+    const double hn = sqrt(hn2 > 0.0 ? hn2 : 0.0);
+    ar[0] = hn2;
+    ar[2] = hn;
+    ar[3] = hn > tiny ? 1.0 / hn : 1.0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_scale_dev(int n, T *__restrict__ w,
+                                                      const double *__restrict__ sc)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        w[i] = (T)((double)w[i] * sc[0]);
+}
+
 // 16-byte vector of T for the streaming kernels (4 floats or 2 doubles).
 template <typename T> struct Vec16;
 template <> struct Vec16<float>
@@ -274,8 +320,11 @@ template <> struct Vec16<double>
 template <typename T, int K, bool VEC>
 __global__ __launch_bounds__(kBlock) void k_multidot(int n, const T *__restrict__ V,
                                                      size_t ld, const T *__restrict__ w,
-                                                     double *__restrict__ partials)
+                                                     double *__restrict__ partials,
+                                                     const int *__restrict__ on)
 {
+    if (on && !*on)
+        return;
     constexpr int KMAX = 16, k = K;
     __shared__ double sh[kBlock * 17];
     double acc[KMAX];
@@ -352,8 +401,11 @@ __global__ __launch_bounds__(kBlock) void k_multidot(int n, const T *__restrict_
 // a fixed tree over the threads (deterministic, and short: the rows are
 // read in parallel).
 __global__ __launch_bounds__(kBlock) void k_fold_dots(int nblk, const double *__restrict__ partials,
-                                                      double *__restrict__ dots)
+                                                      double *__restrict__ dots,
+                                                      const int *__restrict__ on)
 {
+    if (on && !*on)
+        return;
     __shared__ double sh[kBlock];
     const int j = blockIdx.x;
     double s = 0.0;
@@ -369,8 +421,10 @@ __global__ __launch_bounds__(kBlock) void k_fold_dots(int nblk, const double *__
 template <typename T, bool VEC>
 __global__ __launch_bounds__(kBlock) void k_multiaxpy(int n, int k, const T *__restrict__ V,
                                                       size_t ld, const double *__restrict__ h,
-                                                      T *__restrict__ w)
+                                                      T *__restrict__ w, const int *__restrict__ on)
 {
+    if (on && !*on)
+        return;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (VEC)
     {
@@ -747,13 +801,13 @@ __global__ __launch_bounds__(kBlock) void k_dilu_sweep(DiluMesh d, int p0, int p
 // k_multidot<T, K, VEC> for a run-time K in 1..16
 template <typename T, bool VEC, int K = 1>
 void multidot_launch(hipStream_t s, int k, int n, const T *V, size_t ld, const T *w,
-                     double *partials)
+                     double *partials, const int *on)
 {
     if (k == K)
         hipLaunchKernelGGL((k_multidot<T, K, VEC>), dim3(kDotBlocks), dim3(kBlock), 0, s, n, V,
-                           ld, w, partials);
+                           ld, w, partials, on);
     else if constexpr (K < 16)
-        multidot_launch<T, VEC, K + 1>(s, k, n, V, ld, w, partials);
+        multidot_launch<T, VEC, K + 1>(s, k, n, V, ld, w, partials, on);
 }
 
 } // namespace
@@ -804,7 +858,8 @@ void precond_apply_norm(hipStream_t s, int N, const double *Minv, const T *v, T 
 {
     hipLaunchKernelGGL(k_precond_apply_norm<T>, dim3(kDotBlocks), dim3(kBlock), 0, s, N, Minv,
                        v, z, partials);
-    hipLaunchKernelGGL(k_fold_dots, dim3(1), dim3(kBlock), 0, s, kDotBlocks, partials, zz);
+    hipLaunchKernelGGL(k_fold_dots, dim3(1), dim3(kBlock), 0, s, kDotBlocks, partials, zz,
+                       (const int *)nullptr);
 }
 
 void fd_eps(hipStream_t s, const double *zz, double num, double *eps)
@@ -836,7 +891,7 @@ void rhs_minus(hipStream_t s, int n, const double *b, T *w)
 
 template <typename T>
 void multidot(hipStream_t s, int n, int k, const T *V, size_t ld, const T *w,
-              double *partials, double *dots)
+              double *partials, double *dots, const int *on)
 {
     if (k < 1 || k > kMaxDots)
         throw std::runtime_error("multidot: bad vector count");
@@ -849,26 +904,44 @@ void multidot(hipStream_t s, int n, int k, const T *V, size_t ld, const T *w,
     {
         const int kk = std::min(16, k - j0);
         if (vec)
-            multidot_launch<T, true>(s, kk, n, V + (size_t)j0 * ld, ld, w, partials + j0);
+            multidot_launch<T, true>(s, kk, n, V + (size_t)j0 * ld, ld, w, partials + j0, on);
         else
-            multidot_launch<T, false>(s, kk, n, V + (size_t)j0 * ld, ld, w, partials + j0);
+            multidot_launch<T, false>(s, kk, n, V + (size_t)j0 * ld, ld, w, partials + j0, on);
     }
-    hipLaunchKernelGGL(k_fold_dots, dim3(k), dim3(kBlock), 0, s, kDotBlocks, partials, dots);
+    hipLaunchKernelGGL(k_fold_dots, dim3(k), dim3(kBlock), 0, s, kDotBlocks, partials, dots, on);
 }
 
 template <typename T>
-void multiaxpy(hipStream_t s, int n, int k, const T *V, size_t ld, const double *h, T *w)
+void multiaxpy(hipStream_t s, int n, int k, const T *V, size_t ld, const double *h, T *w,
+               const int *on)
 {
     if (aligned16(V) && aligned16(w) && (ld * sizeof(T)) % 16 == 0)
     {
         constexpr int W = 16 / sizeof(T);
         const int threads = n / W + n % W;
         hipLaunchKernelGGL((k_multiaxpy<T, true>), dim3(blocks_for(threads)), dim3(kBlock), 0, s,
-                           n, k, V, ld, h, w);
+                           n, k, V, ld, h, w, on);
     }
     else
         hipLaunchKernelGGL((k_multiaxpy<T, false>), dim3(blocks_for(n)), dim3(kBlock), 0, s, n,
-                           k, V, ld, h, w);
+                           k, V, ld, h, w, on);
+}
+
+void arnoldi_norm(hipStream_t s, int k, const double *h, const double *ww, double *ar,
+                  int *again, double eta)
+{
+    hipLaunchKernelGGL(k_arnoldi_norm, dim3(1), dim3(64), 0, s, k, h, ww, ar, again, eta);
+}
+
+void arnoldi_finish(hipStream_t s, const int *again, const double *ww2, double *ar, double tiny)
+{
+    hipLaunchKernelGGL(k_arnoldi_finish, dim3(1), dim3(64), 0, s, again, ww2, ar, tiny);
+}
+
+template <typename T>
+void scale_dev(hipStream_t s, int n, T *w, const double *sc)
+{
+    hipLaunchKernelGGL(k_scale_dev<T>, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, w, sc);
 }
 
 template <typename T>
@@ -901,7 +974,8 @@ void residual_sums(hipStream_t s, int N, const double *R, const double *vol,
 {
     hipLaunchKernelGGL(k_residual_sums, dim3(kDotBlocks), dim3(kBlock), 0, s, N, R, vol,
                        partials);
-    hipLaunchKernelGGL(k_fold_dots, dim3(5), dim3(kBlock), 0, s, kDotBlocks, partials, sums);
+    hipLaunchKernelGGL(k_fold_dots, dim3(5), dim3(kBlock), 0, s, kDotBlocks, partials, sums,
+                       (const int *)nullptr);
 }
 
 void update_ratio(hipStream_t s, int N, const double *U, const double *dU, double *partials,
@@ -1003,9 +1077,10 @@ template void dilu_apply<double, float>(hipStream_t, const DiluMesh &, const int
                                const double *, const double *, const double *, T *);       \
     template void rhs_minus<T>(hipStream_t, int, const double *, T *);                     \
     template void multidot<T>(hipStream_t, int, int, const T *, size_t, const T *,         \
-                              double *, double *);                                         \
+                              double *, double *, const int *);                            \
     template void multiaxpy<T>(hipStream_t, int, int, const T *, size_t, const double *,   \
-                               T *);                                                       \
+                               T *, const int *);                                          \
+    template void scale_dev<T>(hipStream_t, int, T *, const double *);                     \
     template void scale<T>(hipStream_t, int, const T *, double, T *);                      \
     template void combine<T>(hipStream_t, int, int, const T *, size_t, const double *,     \
                              T *);                                                         \

@@ -180,6 +180,10 @@ typedef struct
                             the solve falls back to block Jacobi (default 0.2) */
     int coupling;        /* FVH_COUPLING_* (default FVH_COUPLING_NONE); DILU
                             needs precond BLOCK or LEARNED */
+    double reorth;       /* a second Gram-Schmidt pass runs when the new
+                            Krylov vector kept at most this fraction of its
+                            square norm (default 0.5: the 1/sqrt(2) test;
+                            0 never) */
 } fvh_jfnk_options_t;
 
 typedef struct

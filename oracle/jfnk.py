@@ -45,7 +45,7 @@ from . import fvref
 
 DEFAULTS = dict(max_newton=300, tol=1e-8, cfl0=10.0, cfl_growth=1.5, cfl_cut=0.3,
                 cfl_stall=0.5, cfl_stall_cut=0.7, cfl_max=1e6, krylov_dims=30, max_restarts=3, rtol=1e-2, fd_eps=1e-7,
-                max_change=0.2, precond="block", coupling="none")
+                max_change=0.2, precond="block", coupling="none", reorth=0.5)
 
 
 def colouring(mesh):
@@ -67,13 +67,13 @@ def colouring(mesh):
     return colour
 
 
-def gmres(op, b, m, max_restarts, rtol):
+def gmres(op, b, m, max_restarts, rtol, reorth=0.5):
     """Restarted GMRES(m) for op(x) = b from x = 0: classical Gram-Schmidt in
     one pass with the new vector's norm by Pythagoras (|w|^2 - |h|^2), and a
-    second pass when that lost more than half of |w|^2 (the 1/sqrt(2) test
-    of Daniel, Gragg, Kaufman and Stewart); Givens rotations; stops when the
-    estimated residual is at most rtol |b|.  Returns (x, Arnoldi steps,
-    estimated relative residual)."""
+    second pass when that kept at most the fraction reorth of |w|^2 (0.5:
+    the 1/sqrt(2) test of Daniel, Gragg, Kaufman and Stewart); Givens
+    rotations; stops when the estimated residual is at most rtol |b|.
+    Returns (x, Arnoldi steps, estimated relative residual)."""
     x = np.zeros(b.size)
     beta0 = None
     its = 0
@@ -101,7 +101,7 @@ def gmres(op, b, m, max_restarts, rtol):
             w = w - h1 @ Vk[: j + 1]
             hn2 = ww - h1 @ h1
             h2 = np.zeros_like(h1)
-            if not hn2 > 0.5 * ww:
+            if not hn2 > reorth * ww:
                 h2 = Vk[: j + 1] @ w
                 w = w - h2 @ Vk[: j + 1]
                 hn2 = w @ w
@@ -332,7 +332,7 @@ class OracleJfnk:
                 return (z.reshape(self.N, 5) / dtau[:, None] - (Rt - R0) / eps).reshape(-1)
 
             x, its, rel = gmres(lambda v: A(M(v)), b, o["krylov_dims"],
-                                o["max_restarts"], o["rtol"])
+                                o["max_restarts"], o["rtol"], o["reorth"])
             gm_total += its
             dU = M(x).reshape(self.N, 5)
             ratio = np.max(np.abs(dU[:, [0, 4]] / U[:, [0, 4]]))
