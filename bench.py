@@ -9,8 +9,11 @@ HBM.  `value` is cell-stage updates per second of that step loop.
 Beside it, in the same JSON line:
 - `jfnk`: the steady bump channel (160x40x13 hexahedra x 6 tetrahedra =
   499,200 cells, Ma 0.5): wall time to the 1e-8 stop norm of plain FP64
-  block-Jacobi JFNK and of the learned-preconditioner, FP32-Krylov JFNK, the
-  speedup and the relative difference of the two converged solutions;
+  block-Jacobi JFNK and of the variants (FP32 Krylov; learned diagonal
+  blocks; the multicolour block-ILU coupling; their combinations), each
+  with its speedup over plain and the relative difference of its converged
+  solution; `headline` names the learned-preconditioner mixed-precision
+  variant (learned FP32 with the coupling) and its speedup;
 - `batched`: 64 periodic cases of log-uniform size (seeded), one device
   state, graph-replayed steps: aggregate cell updates per second and the
   per-case step time (each case alone) p50/p99;
@@ -226,6 +229,12 @@ def bench_jfnk(n, max_newton=400, only=None):
             name, st["seconds"], st["newton_steps"], st["gmres_iterations"],
             st["precond_fallbacks"]), file=sys.stderr, flush=True)
     case.close()
+    best = out.get("learned_fp32_dilu")
+    if best:
+        out["headline"] = {"variant": "learned_fp32_dilu",
+                           "speedup_vs_plain": best["speedup_vs_plain"],
+                           "seconds": best["seconds"],
+                           "plain_seconds": out["plain_fp64_block"]["seconds"]}
     return out
 
 

@@ -38,7 +38,7 @@ struct Workspace
 // offsets in dots (and in each host slot)
 constexpr int kH1 = 0, kH2 = kry::kMaxDots, kWW = 2 * kry::kMaxDots,
               kZZ = kWW + 1, kSums = kWW + 2, kRatio = kWW + 8, kEps = kWW + 9,
-              kUU = kWW + 10, kAr = kWW + 12, kWW2 = kWW + 16;
+              kUU = kWW + 10, kAr = kWW + 12, kWW2 = kWW + 17;
 // pinned host memory: two slots of kCopy doubles for the Arnoldi steps in
 // flight (slot 0 also serves the synchronous fetches), then the GMRES
 // solution y
@@ -59,7 +59,8 @@ struct DiluState
     int *ccell = nullptr, *nbr = nullptr, *back = nullptr, *flags = nullptr;
     double *off = nullptr, *dinv = nullptr;     // build layout
     double *off64 = nullptr, *dinv64 = nullptr; // sweep layout, FP64
-    float *off32 = nullptr, *dinv32 = nullptr;  // sweep layout, FP32
+    kry::bf16_t *off16 = nullptr;               // sweep layout, bfloat16 blocks
+    float *dinv32 = nullptr;                    //   with FP32 diagonal inverses
     kry::DiluMesh view() const { return {N, maxf, ncol, ccell, nbr, back, flags}; }
 };
 DiluState g_dilu;
@@ -238,9 +239,9 @@ void dilu_prepare(const RuntimeView &v, bool fp32)
         FVH_HIP_CHECK(hipMemset(g_dilu.off, 0, (size_t)N * maxf * 25 * sizeof(double)));
     }
     const size_t NT = ((size_t)g_dilu.N + 63) / 64 * 64; // whole tiles
-    if (fp32 && !g_dilu.off32)
+    if (fp32 && !g_dilu.off16)
     {
-        g_dilu.off32 = walloc<float>(NT * g_dilu.maxf * 25);
+        g_dilu.off16 = walloc<kry::bf16_t>(NT * g_dilu.maxf * 25);
         g_dilu.dinv32 = walloc<float>(NT * 25);
     }
     if (!fp32 && !g_dilu.off64)
@@ -251,7 +252,8 @@ void dilu_prepare(const RuntimeView &v, bool fp32)
 }
 
 // The coupled preconditioner at U from the inverse diagonal blocks in Minv;
-// the sweep copies of the factors in FP32 (fp32) or FP64.
+// the sweep copies of the factors with bfloat16 off-diagonal blocks and FP32
+// diagonal inverses (fp32), or all in FP64.
 void dilu_build(const RuntimeView &v, bool fp32)
 {
     dilu_prepare(v, fp32);
@@ -260,12 +262,13 @@ void dilu_build(const RuntimeView &v, bool fp32)
     for (int q = 0; q < g_dilu.ncol; ++q)
     {
         if (fp32)
-            kry::dilu_factor<float>(v.stream, d, g_dilu.coff[q], g_dilu.coff[q + 1], g_ws.Minv,
-                                    g_dilu.off, g_dilu.dinv, g_dilu.off32, g_dilu.dinv32);
+            kry::dilu_factor<float, kry::bf16_t>(v.stream, d, g_dilu.coff[q],
+                                                 g_dilu.coff[q + 1], g_ws.Minv, g_dilu.off,
+                                                 g_dilu.dinv, g_dilu.off16, g_dilu.dinv32);
         else
-            kry::dilu_factor<double>(v.stream, d, g_dilu.coff[q], g_dilu.coff[q + 1],
-                                     g_ws.Minv, g_dilu.off, g_dilu.dinv, g_dilu.off64,
-                                     g_dilu.dinv64);
+            kry::dilu_factor<double, double>(v.stream, d, g_dilu.coff[q], g_dilu.coff[q + 1],
+                                             g_ws.Minv, g_dilu.off, g_dilu.dinv, g_dilu.off64,
+                                             g_dilu.dinv64);
     }
 }
 
@@ -313,21 +316,20 @@ GmresResult gmres(const RuntimeView &v, const fvh_jfnk_options_t &o, double fd_n
     // FP32 residual of U)
     const bool fp32 = o.precision == FVH_JFNK_FP32;
     // z = M^-1 in with |z|^2 into dots[kZZ] (the coupled preconditioner's
-    // factors are stored in FP32 in both precisions)
+    // factors are stored in reduced precision in both precisions)
     const bool coupled = o.coupling == FVH_COUPLING_DILU;
     auto apply_M = [&](const T *in, T *out) {
         if (coupled)
         {
-            kry::dilu_apply<T, float>(s, g_dilu.view(), g_dilu.coff.data(), g_dilu.off32,
-                                      g_dilu.dinv32, in, out);
+            kry::dilu_apply<T, float, kry::bf16_t>(s, g_dilu.view(), g_dilu.coff.data(),
+                                                   g_dilu.off16, g_dilu.dinv32, in, out);
             kry::multidot<T>(s, n, 1, out, ld, out, g_ws.partials, dots + kZZ);
         }
         else
             kry::precond_apply_norm<T>(s, N, g_ws.Minv, in, out, g_ws.partials, dots + kZZ);
     };
     auto apply_op = [&](const T *zin, T *out) {
-        kry::fd_eps(s, dots + kZZ, fd_num, dots + kEps);
-        kry::perturb<T>(s, n, v.U, dots + kEps, zin, g_ws.Ut);
+        kry::perturb<T>(s, n, v.U, dots + kZZ, fd_num, zin, g_ws.Ut, dots + kEps);
         eval_residual(g_ws.Ut, g_ws.Rt, fp32);
         kry::op_finish<T>(s, N, zin, g_ws.dtau, g_ws.Rt, g_ws.R0j, dots + kEps, out);
         ++res.evals;
@@ -378,11 +380,13 @@ GmresResult gmres(const RuntimeView &v, const fvh_jfnk_options_t &o, double fd_n
             kry::multiaxpy<T>(s, n, j + 1, V, ld, dots + kH1, w);
             kry::arnoldi_norm(s, j + 1, dots + kH1, dots + kH1 + j + 1, dots + kAr, g_ws.again,
                               o.reorth);
-            kry::multidot<T>(s, n, j + 2, V, ld, w, g_ws.partials, dots + kH2, g_ws.again);
-            kry::multiaxpy<T>(s, n, j + 1, V, ld, dots + kH2, w, g_ws.again);
-            kry::multidot<T>(s, n, 1, w, ld, w, g_ws.partials, dots + kWW2, g_ws.again);
-            kry::arnoldi_finish(s, g_ws.again, dots + kWW2, dots + kAr, tiny);
-            kry::scale_dev<T>(s, n, w, dots + kAr + 3);
+            if (o.reorth > 0.0)
+            {
+                kry::multidot<T>(s, n, j + 2, V, ld, w, g_ws.partials, dots + kH2, g_ws.again);
+                kry::multiaxpy<T>(s, n, j + 1, V, ld, dots + kH2, w, g_ws.again);
+                kry::multidot<T>(s, n, 1, w, ld, w, g_ws.partials, dots + kWW2, g_ws.again);
+            }
+            kry::arnoldi_finish<T>(s, n, w, g_ws.again, dots + kWW2, dots + kAr, tiny);
             FVH_HIP_CHECK(hipMemcpyAsync(host + (j & 1) * kSlot, dots, kCopy * sizeof(double),
                                          hipMemcpyDeviceToHost, s));
             FVH_HIP_CHECK(hipEventRecord(g_ws.ev[j & 1], s));
@@ -538,8 +542,9 @@ int solve(const fvh_jfnk_options_t &o, fvh_jfnk_stats_t &st)
         st.gmres_iterations += gr.its;
         st.residual_evals += gr.evals;
         if (o.coupling == FVH_COUPLING_DILU)
-            kry::dilu_apply<double, float>(s, g_dilu.view(), g_dilu.coff.data(), g_dilu.off32,
-                                           g_dilu.dinv32, g_ws.xacc, g_ws.dU);
+            kry::dilu_apply<double, float, kry::bf16_t>(s, g_dilu.view(), g_dilu.coff.data(),
+                                                        g_dilu.off16, g_dilu.dinv32, g_ws.xacc,
+                                                        g_ws.dU);
         else
             kry::precond_apply<double>(s, N, g_ws.Minv, g_ws.xacc, g_ws.dU);
         kry::update_ratio(s, N, v.U, g_ws.dU, g_ws.partials, g_ws.dots + kRatio);
@@ -608,7 +613,7 @@ int fvh_jfnk_default_options(fvh_jfnk_options_t *o)
     o->learned_check = 10;
     o->learned_tol = 0.2;
     o->coupling = FVH_COUPLING_NONE;
-    o->reorth = 0.5;
+    o->reorth = 0.1;
     return FVH_OK;
 }
 
@@ -690,8 +695,9 @@ int fvh_jfnk_precond_apply(double cfl, int precond, int coupling, const double *
         if (coupling == FVH_COUPLING_DILU)
         {
             dilu_build(rv, false);
-            kry::dilu_apply<double, double>(s, g_dilu.view(), g_dilu.coff.data(),
-                                            g_dilu.off64, g_dilu.dinv64, g_ws.xacc, g_ws.dU);
+            kry::dilu_apply<double, double, double>(s, g_dilu.view(), g_dilu.coff.data(),
+                                                    g_dilu.off64, g_dilu.dinv64, g_ws.xacc,
+                                                    g_ws.dU);
         }
         else
             kry::precond_apply<double>(s, N, g_ws.Minv, g_ws.xacc, g_ws.dU);

@@ -55,6 +55,12 @@ void precond_apply_norm(hipStream_t s, int N, const double *Minv, const T *v, T 
 // for whole tiles; back[p*maxf + t] the flat slot index (pk*maxf + t') of
 // the transposed coupling A1_{nbr,i}; flags[p] has bit t set for a
 // neighbour of a lower colour and bit 8 + t for one of a higher colour.
+// bfloat16 storage (the upper 16 bits of an FP32 pattern)
+struct bf16_t
+{
+    unsigned short v;
+};
+
 struct DiluMesh
 {
     int N, maxf, ncol;
@@ -70,25 +76,24 @@ void dilu_offdiag(hipStream_t s, const dev::DevMesh &m, const dev::PhysDev &P, i
 // Factorisation of the positions [p0, p1) of one colour (colours in
 // ascending order): dinv[p] = D_i^-1 with A1_ii = Minv[i]^-1 (Minv by cell,
 // the block-Jacobi or learned inverse blocks); the position's blocks are
-// also written, in precision B, into the component-major offB / dinvB.
-template <typename B>
+// also written into the tiled sweep copies: dinvB in precision B, offB in O
+// (double, float or bfloat16).
+template <typename B, typename O>
 void dilu_factor(hipStream_t s, const DiluMesh &d, int p0, int p1, const double *Minv,
-                 const double *off, double *dinv, B *offB, B *dinvB);
+                 const double *off, double *dinv, O *offB, B *dinvB);
 
 // z = M^-1 v: the forward sweep over the colours in ascending order, then
 // the backward sweep in descending order (z in place); coff[q] is the first
 // position of colour q (host array of ncol + 1 entries).
-template <typename T, typename B>
-void dilu_apply(hipStream_t s, const DiluMesh &d, const int *coff, const B *off,
+template <typename T, typename B, typename O>
+void dilu_apply(hipStream_t s, const DiluMesh &d, const int *coff, const O *off,
                 const B *dinv, const T *v, T *z);
 
-// eps[0] = num / sqrt(zz[0]) (num / 1e-300 for zz = 0)
-void fd_eps(hipStream_t s, const double *zz, double num, double *eps);
-
-// Ut = U + eps[0] * z  (eps in device memory)
+// Ut = U + eps * z with eps = num / sqrt(zz[0]) (num / 1e-300 for zz = 0),
+// zz in device memory; eps also stored to eps[0]
 template <typename T>
-void perturb(hipStream_t s, int n, const double *U, const double *eps,
-             const T *z, double *Ut);
+void perturb(hipStream_t s, int n, const double *U, const double *zz, double num,
+             const T *z, double *Ut, double *eps);
 
 // w = z / dtau - (Rt - R0) / eps[0]   (A z with A = I/dtau - J, J z by
 // finite difference)
@@ -115,15 +120,14 @@ void multiaxpy(hipStream_t s, int n, int k, const T *V, size_t ld,
 // Arnoldi step bookkeeping on the device (one thread each):
 // arnoldi_norm: ar[0] = ww[0] - sum_{i<k} h[i]^2, ar[1] = *again = whether a
 // second Gram-Schmidt pass is needed (ar[0] <= eta ww); arnoldi_finish: after
-// that pass (run or skipped), ar[0] = ww2[0] if it ran, ar[2] = hn =
-// sqrt(max(ar[0], 0)), ar[3] = 1/hn (1 when hn <= tiny).
+// that pass (run or skipped), with hn2 = ww2[0] if it ran, else ar[0]:
+// ar[2] = hn = sqrt(max(hn2, 0)), ar[3] = 1/hn (1 when hn <= tiny), ar[4] =
+// hn2, and w *= ar[3].
 void arnoldi_norm(hipStream_t s, int k, const double *h, const double *ww, double *ar,
                   int *again, double eta);
-void arnoldi_finish(hipStream_t s, const int *again, const double *ww2, double *ar, double tiny);
-
-// w *= sc[0] (sc in device memory)
 template <typename T>
-void scale_dev(hipStream_t s, int n, T *w, const double *sc);
+void arnoldi_finish(hipStream_t s, int n, T *w, const int *again, const double *ww2, double *ar,
+                    double tiny);
 
 // out = w * scale
 template <typename T>

@@ -169,15 +169,6 @@ __global__ __launch_bounds__(kBlock) void k_precond_apply(int N,
     }
 }
 
-__global__ void k_fd_eps(const double *__restrict__ zz, double num, double *__restrict__ eps)
-{
-    if (threadIdx.x == 0)
-    {
-        const double nz = sqrt(zz[0]);
-        eps[0] = num / (nz > 1e-300 ? nz : 1e-300);
-    }
-}
-
 // z = Minv v and the per-block partial sums of |z|^2 (grid-stride over
 // cells on the fixed dot grid; z^2 of the stored T values)
 template <typename T>
@@ -215,13 +206,19 @@ __global__ __launch_bounds__(kBlock) void k_precond_apply_norm(int N,
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_perturb(int n, const double *__restrict__ U,
-                                                    const double *__restrict__ eps,
+                                                    const double *__restrict__ zz, double num,
                                                     const T *__restrict__ z,
-                                                    double *__restrict__ Ut)
+                                                    double *__restrict__ Ut,
+                                                    double *__restrict__ eps_out)
 {
+    // eps = num / |z| (num / 1e-300 for z = 0)
+    const double nz = sqrt(zz[0]);
+    const double eps = num / (nz > 1e-300 ? nz : 1e-300);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0)
+        eps_out[0] = eps;
     if (i < n)
-        Ut[i] = U[i] + eps[0] * (double)z[i];
+        Ut[i] = U[i] + eps * (double)z[i];
 }
 
 template <typename T>
@@ -254,9 +251,10 @@ __global__ __launch_bounds__(kBlock) void k_rhs_minus(int n, const double *__res
 //     the norm by Pythagoras; the sum in index order), ar[1] = 1 and
 //     *again = 1 when the new vector kept at most the fraction eta of its
 //     square norm (a second pass is needed), else 0;
-//   arnoldi_finish: after the (skipped or run) second pass, ar[0] = |w|^2 of
-//     that pass when it ran, ar[2] = hn = sqrt(max(ar[0], 0)) and ar[3] the
-//     scale of w: 1/hn, or 1 when hn <= tiny (breakdown; the host stops).
+//   arnoldi_finish: after the (skipped or run) second pass, with hn2 = |w|^2
+//     of that pass when it ran, else ar[0]: ar[2] = hn = sqrt(max(hn2, 0)),
+//     ar[3] the scale of w (1/hn, or 1 when hn <= tiny: breakdown, the host
+//     stops), ar[4] = hn2; and w scaled in place.
 __global__ void k_arnoldi_norm(int k, const double *__restrict__ h,
                                const double *__restrict__ ww, double *__restrict__ ar,
                                int *__restrict__ again, double eta)
@@ -267,31 +265,30 @@ __global__ void k_arnoldi_norm(int k, const double *__restrict__ h,
     for (int i = 0; i < k; ++i)
         hh += h[i] * h[i];
     const double hn2 = ww[0] - hh;
-    const int a = !(hn2 > eta * ww[0]);
+    const int a = eta > 0.0 && !(hn2 > eta * ww[0]);
     ar[0] = hn2;
     ar[1] = a ? 1.0 : 0.0;
     *again = a;
 }
 
-__global__ void k_arnoldi_finish(const int *__restrict__ again, const double *__restrict__ ww2,
-                                 double *__restrict__ ar, double tiny)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_arnoldi_finish(int n, T *__restrict__ w,
+                                                           const int *__restrict__ again,
+                                                           const double *__restrict__ ww2,
+                                                           double *__restrict__ ar, double tiny)
 {
-    if (threadIdx.x != 0)
-        return;
     const double hn2 = *again ? ww2[0] : ar[0];
     const double hn = sqrt(hn2 > 0.0 ? hn2 : 0.0);
-    ar[0] = hn2;
-    ar[2] = hn;
-    ar[3] = hn > tiny ? 1.0 / hn : 1.0;
-}
-
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_scale_dev(int n, T *__restrict__ w,
-                                                      const double *__restrict__ sc)
-{
+    const double sc = hn > tiny ? 1.0 / hn : 1.0;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0)
+    {
+        ar[2] = hn;
+        ar[3] = sc;
+        ar[4] = hn2;
+    }
     if (i < n)
-        w[i] = (T)((double)w[i] * sc[0]);
+        w[i] = (T)((double)w[i] * sc);
 }
 
 // 16-byte vector of T for the streaming kernels (4 floats or 2 doubles).
@@ -581,6 +578,21 @@ __device__ __forceinline__ size_t tiled(int p, int e, int E)
     return ((size_t)(p >> 6) * E + e) * 64 + (p & 63);
 }
 
+// storage conversions of the sweep copies (bfloat16: the upper half of the
+// FP32 pattern, rounded to nearest even)
+__device__ __forceinline__ double to_d(float x) { return x; }
+__device__ __forceinline__ double to_d(double x) { return x; }
+__device__ __forceinline__ double to_d(bf16_t x) { return __uint_as_float((unsigned)x.v << 16); }
+template <class O> __device__ __forceinline__ O from_d(double x);
+template <> __device__ __forceinline__ float from_d<float>(double x) { return (float)x; }
+template <> __device__ __forceinline__ double from_d<double>(double x) { return x; }
+template <> __device__ __forceinline__ bf16_t from_d<bf16_t>(double x)
+{
+    unsigned u = __float_as_uint((float)x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return bf16_t{(unsigned short)(u >> 16)};
+}
+
 
 // Off-diagonal blocks A1_{nb,c} = -dR1_nb/dU_c = (s_cf A_f / V_nb) dF_f/dU_c
 // (s_cf = +1 when c is the face's first cell), one thread per position; the
@@ -677,12 +689,12 @@ __device__ __forceinline__ void mm5(const double *A, const double *B, double *C)
         }
 }
 
-template <typename B>
+template <typename B, typename O>
 __global__ __launch_bounds__(kBlock) void k_dilu_factor(DiluMesh d, int p0, int p1,
                                                         const double *__restrict__ Minv,
                                                         const double *__restrict__ off,
                                                         double *__restrict__ dinv,
-                                                        B *__restrict__ offB,
+                                                        O *__restrict__ offB,
                                                         B *__restrict__ dinvB)
 {
     const int p = p0 + blockIdx.x * blockDim.x + threadIdx.x;
@@ -726,7 +738,7 @@ __global__ __launch_bounds__(kBlock) void k_dilu_factor(DiluMesh d, int p0, int 
         dinvB[tiled(p, q, 25)] = (B)X[q];
     }
     for (int q = 0; q < d.maxf * 25; ++q)
-        offB[tiled(p, q, d.maxf * 25)] = (B)off[(size_t)p * d.maxf * 25 + q];
+        offB[tiled(p, q, d.maxf * 25)] = from_d<O>(off[(size_t)p * d.maxf * 25 + q]);
 }
 
 // One colour of the sweeps (blocks and neighbour table in the tiled
@@ -737,9 +749,9 @@ __global__ __launch_bounds__(kBlock) void k_dilu_factor(DiluMesh d, int p0, int 
 // included) are replaced by zero, so those slots contribute exactly zero.
 //   FORWARD: z_i = D_i^-1 (v_i - sum_{k lower} A_ik z_k)
 //   backward: z_i -= D_i^-1 sum_{k higher} A_ik z_k
-template <typename T, typename B, int MAXF, bool FORWARD>
+template <typename T, typename B, typename O, int MAXF, bool FORWARD>
 __global__ __launch_bounds__(kBlock) void k_dilu_sweep(DiluMesh d, int p0, int p1,
-                                                       const B *__restrict__ off,
+                                                       const O *__restrict__ off,
                                                        const B *__restrict__ dinv,
                                                        const T *__restrict__ v,
                                                        T *__restrict__ z)
@@ -772,7 +784,7 @@ __global__ __launch_bounds__(kBlock) void k_dilu_sweep(DiluMesh d, int p0, int p
             double acc = 0.0;
 #pragma unroll
             for (int b = 0; b < 5; ++b)
-                acc += (double)off[tiled(p, t * 25 + a * 5 + b, E)] * y[b];
+                acc += to_d(off[tiled(p, t * 25 + a * 5 + b, E)]) * y[b];
             r[a] += acc;
         }
     }
@@ -862,17 +874,12 @@ void precond_apply_norm(hipStream_t s, int N, const double *Minv, const T *v, T 
                        (const int *)nullptr);
 }
 
-void fd_eps(hipStream_t s, const double *zz, double num, double *eps)
-{
-    hipLaunchKernelGGL(k_fd_eps, dim3(1), dim3(64), 0, s, zz, num, eps);
-}
-
 template <typename T>
-void perturb(hipStream_t s, int n, const double *U, const double *eps, const T *z,
-             double *Ut)
+void perturb(hipStream_t s, int n, const double *U, const double *zz, double num,
+             const T *z, double *Ut, double *eps)
 {
-    hipLaunchKernelGGL(k_perturb<T>, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, U, eps,
-                       z, Ut);
+    hipLaunchKernelGGL(k_perturb<T>, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, U, zz, num,
+                       z, Ut, eps);
 }
 
 template <typename T>
@@ -933,15 +940,12 @@ void arnoldi_norm(hipStream_t s, int k, const double *h, const double *ww, doubl
     hipLaunchKernelGGL(k_arnoldi_norm, dim3(1), dim3(64), 0, s, k, h, ww, ar, again, eta);
 }
 
-void arnoldi_finish(hipStream_t s, const int *again, const double *ww2, double *ar, double tiny)
-{
-    hipLaunchKernelGGL(k_arnoldi_finish, dim3(1), dim3(64), 0, s, again, ww2, ar, tiny);
-}
-
 template <typename T>
-void scale_dev(hipStream_t s, int n, T *w, const double *sc)
+void arnoldi_finish(hipStream_t s, int n, T *w, const int *again, const double *ww2, double *ar,
+                    double tiny)
 {
-    hipLaunchKernelGGL(k_scale_dev<T>, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, w, sc);
+    hipLaunchKernelGGL(k_arnoldi_finish<T>, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, w,
+                       again, ww2, ar, tiny);
 }
 
 template <typename T>
@@ -1015,64 +1019,68 @@ void dilu_offdiag(hipStream_t s, const DevMesh &m, const PhysDev &P, int flux,
     }
 }
 
-template <typename B>
+template <typename B, typename O>
 void dilu_factor(hipStream_t s, const DiluMesh &d, int p0, int p1, const double *Minv,
-                 const double *off, double *dinv, B *offB, B *dinvB)
+                 const double *off, double *dinv, O *offB, B *dinvB)
 {
     if (p1 <= p0)
         return;
-    hipLaunchKernelGGL(k_dilu_factor<B>, dim3(blocks_for(p1 - p0)), dim3(kBlock), 0, s, d, p0,
-                       p1, Minv, off, dinv, offB, dinvB);
+    hipLaunchKernelGGL((k_dilu_factor<B, O>), dim3(blocks_for(p1 - p0)), dim3(kBlock), 0, s, d,
+                       p0, p1, Minv, off, dinv, offB, dinvB);
 }
 
-template <typename T, typename B, int MAXF>
-void dilu_sweeps(hipStream_t s, const DiluMesh &d, const int *coff, const B *off,
+template <typename T, typename B, typename O, int MAXF>
+void dilu_sweeps(hipStream_t s, const DiluMesh &d, const int *coff, const O *off,
                  const B *dinv, const T *v, T *z)
 {
     for (int q = 0; q < d.ncol; ++q)
         if (coff[q + 1] > coff[q])
-            hipLaunchKernelGGL((k_dilu_sweep<T, B, MAXF, true>),
+            hipLaunchKernelGGL((k_dilu_sweep<T, B, O, MAXF, true>),
                                dim3(blocks_for(coff[q + 1] - coff[q])), dim3(kBlock), 0, s, d,
                                coff[q], coff[q + 1], off, dinv, v, z);
     // the highest colour has no higher neighbours
     for (int q = d.ncol - 2; q >= 0; --q)
         if (coff[q + 1] > coff[q])
-            hipLaunchKernelGGL((k_dilu_sweep<T, B, MAXF, false>),
+            hipLaunchKernelGGL((k_dilu_sweep<T, B, O, MAXF, false>),
                                dim3(blocks_for(coff[q + 1] - coff[q])), dim3(kBlock), 0, s, d,
                                coff[q], coff[q + 1], off, dinv, v, z);
 }
 
-template <typename T, typename B>
-void dilu_apply(hipStream_t s, const DiluMesh &d, const int *coff, const B *off,
+template <typename T, typename B, typename O>
+void dilu_apply(hipStream_t s, const DiluMesh &d, const int *coff, const O *off,
                 const B *dinv, const T *v, T *z)
 {
     if (d.maxf <= 4)
-        dilu_sweeps<T, B, 4>(s, d, coff, off, dinv, v, z);
+        dilu_sweeps<T, B, O, 4>(s, d, coff, off, dinv, v, z);
     else if (d.maxf <= 6)
-        dilu_sweeps<T, B, 6>(s, d, coff, off, dinv, v, z);
+        dilu_sweeps<T, B, O, 6>(s, d, coff, off, dinv, v, z);
     else
         throw std::runtime_error("DILU: cells with more than 6 faces are not supported");
 }
 
-template void dilu_factor<double>(hipStream_t, const DiluMesh &, int, int, const double *,
-                                  const double *, double *, double *, double *);
-template void dilu_factor<float>(hipStream_t, const DiluMesh &, int, int, const double *,
-                                 const double *, double *, float *, float *);
-template void dilu_apply<double, double>(hipStream_t, const DiluMesh &, const int *,
-                                         const double *, const double *, const double *,
-                                         double *);
-template void dilu_apply<float, float>(hipStream_t, const DiluMesh &, const int *,
-                                       const float *, const float *, const float *, float *);
-template void dilu_apply<double, float>(hipStream_t, const DiluMesh &, const int *,
-                                        const float *, const float *, const double *,
-                                        double *);
+template void dilu_factor<double, double>(hipStream_t, const DiluMesh &, int, int,
+                                          const double *, const double *, double *, double *,
+                                          double *);
+template void dilu_factor<float, float>(hipStream_t, const DiluMesh &, int, int, const double *,
+                                        const double *, double *, float *, float *);
+template void dilu_factor<float, bf16_t>(hipStream_t, const DiluMesh &, int, int,
+                                         const double *, const double *, double *, bf16_t *,
+                                         float *);
+#define FVH_DILU_APPLY(T, B, O)                                                            \
+    template void dilu_apply<T, B, O>(hipStream_t, const DiluMesh &, const int *, const O *, \
+                                      const B *, const T *, T *);
+FVH_DILU_APPLY(double, double, double)
+FVH_DILU_APPLY(float, float, float)
+FVH_DILU_APPLY(double, float, float)
+FVH_DILU_APPLY(float, float, bf16_t)
+FVH_DILU_APPLY(double, float, bf16_t)
 
 #define FVH_KRY_INST(T)                                                                    \
     template void precond_apply<T>(hipStream_t, int, const double *, const T *, T *);      \
     template void precond_apply_norm<T>(hipStream_t, int, const double *, const T *, T *,  \
                                         double *, double *);                               \
-    template void perturb<T>(hipStream_t, int, const double *, const double *, const T *,  \
-                             double *);                                                    \
+    template void perturb<T>(hipStream_t, int, const double *, const double *, double,     \
+                             const T *, double *, double *);                               \
     template void op_finish<T>(hipStream_t, int, const T *, const double *,                \
                                const double *, const double *, const double *, T *);       \
     template void rhs_minus<T>(hipStream_t, int, const double *, T *);                     \
@@ -1080,7 +1088,8 @@ template void dilu_apply<double, float>(hipStream_t, const DiluMesh &, const int
                               double *, double *, const int *);                            \
     template void multiaxpy<T>(hipStream_t, int, int, const T *, size_t, const double *,   \
                                T *, const int *);                                          \
-    template void scale_dev<T>(hipStream_t, int, T *, const double *);                     \
+    template void arnoldi_finish<T>(hipStream_t, int, T *, const int *, const double *,    \
+                                    double *, double);                                     \
     template void scale<T>(hipStream_t, int, const T *, double, T *);                      \
     template void combine<T>(hipStream_t, int, int, const T *, size_t, const double *,     \
                              T *);                                                         \

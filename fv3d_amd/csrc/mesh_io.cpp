@@ -1,12 +1,16 @@
-// Raw little-endian container (.fvm mesh, .fvs state) holding the same
-// groups and datasets as the reference's HDF5 files: the mesh layout of
-// tools/pyMeshFV3D/pymeshfv3d/write.py:19-202 read by
-// src/Solver/src/mesh/mesh.c:121-630, and the state layout of
-// src/Solver/src/output/output.c:20-191.  Used where HDF5 is not available;
-// dataset names are "GROUP/name", attributes are one-element datasets.
+// Mesh and state files: the writer/reader choice (HDF5 for ".h5" names and
+// HDF5 signatures, hdf5_io.cpp; otherwise the raw container), the mesh
+// layout of tools/pyMeshFV3D/pymeshfv3d/write.py:19-202 read by
+// src/Solver/src/mesh/mesh.c:121-630 (write_mesh_file / read_mesh_file),
+// and the raw container itself.
 //
-// Format: "FV3DRAW1", then records {u32 name_len, name, u8 type ('i' int32,
-// 'd' float64, 's' string), u32 ndim, u64 dims[ndim], payload}.
+// The raw little-endian container (.fvm mesh, .fvs state) holds the same
+// groups and datasets as the reference's HDF5 files (the mesh layout above
+// and the state layout of src/Solver/src/output/output.c:20-191) where
+// libhdf5 is not available; dataset names are "GROUP/name", attributes are
+// one-element datasets.  Format: "FV3DRAW1", then records {u32 name_len,
+// name, u8 type ('i' int32, 'd' float64, 's' string), u32 ndim, u64
+// dims[ndim], payload}.
 #include <cstdint>
 #include <cstdio>
 #include <cstring>

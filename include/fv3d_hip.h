@@ -182,8 +182,9 @@ typedef struct
                             needs precond BLOCK or LEARNED */
     double reorth;       /* a second Gram-Schmidt pass runs when the new
                             Krylov vector kept at most this fraction of its
-                            square norm (default 0.5: the 1/sqrt(2) test;
-                            0 never) */
+                            square norm (default 0.1; 0.5 is the 1/sqrt(2)
+                            test of Daniel, Gragg, Kaufman and Stewart; 0
+                            never) */
 } fvh_jfnk_options_t;
 
 typedef struct

@@ -45,7 +45,7 @@ from . import fvref
 
 DEFAULTS = dict(max_newton=300, tol=1e-8, cfl0=10.0, cfl_growth=1.5, cfl_cut=0.3,
                 cfl_stall=0.5, cfl_stall_cut=0.7, cfl_max=1e6, krylov_dims=30, max_restarts=3, rtol=1e-2, fd_eps=1e-7,
-                max_change=0.2, precond="block", coupling="none", reorth=0.5)
+                max_change=0.2, precond="block", coupling="none", reorth=0.1)
 
 
 def colouring(mesh):
