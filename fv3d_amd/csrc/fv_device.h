@@ -48,6 +48,13 @@ struct DevMesh
     const int *__restrict__ bnd_face;
     const int *__restrict__ region_type;
     const double *__restrict__ region_phi;
+    // FP32 face records of the FP32 residual (built on first use):
+    // fgA32 [F][12] = n, t1, t2, area, 2 pad; fgB32 [F][12] = n, area,
+    // lambda, r0 = x_f - x_c0, r1 = x_f - x_c1 (through the periodic image),
+    // 1 pad; each value the FP32 rounding of the FP64 one (r from the FP64
+    // difference)
+    const float *__restrict__ fgA32;
+    const float *__restrict__ fgB32;
 };
 
 struct PhysDev
@@ -77,6 +84,7 @@ struct GradOut
     double *mnmx; // [N*20] neighbourhood min/max (tests)
     int c_begin, c_end; // cells [c_begin, c_end); c_end = 0: all cells
     const double *prims; // input: primitives [N][5] of U (null: computed)
+    const float *U32;    // input of the FP32 kernel: the state in FP32 (null: U)
 };
 
 struct StageParams
@@ -101,6 +109,7 @@ struct StageParams
     const int *cell_case;
     const double *prims_in; // primitives [N][5] of U_in (null: computed)
     double *prims_out;      // primitives of U_out, written by the stage update
+    const float *U32;       // the FP32 kernel's input state in FP32 (null: U_in)
 };
 
 constexpr int kBlock = 256;
@@ -123,7 +132,8 @@ enum FluxKind
 // faces in fs when fs is not null).  No-op without boundaries.
 void launch_ghost(hipStream_t s, const DevMesh &m, const PhysDev &P,
                   const double *U, double *gphi, double *fs,
-                  const DevScalars *sc, float *fs32 = nullptr);
+                  const DevScalars *sc, float *fs32 = nullptr,
+                  const float *U32 = nullptr);
 // linear: limited linear face states; otherwise the cell values.
 // fp32: the arithmetic in FP32 (inputs and outputs stay FP64 arrays), for
 // the FP32 Jacobian-vector products of the Newton-Krylov solve.

@@ -57,7 +57,8 @@ namespace dev {
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void fv_ghost_kernel(
     DevMesh m, PhysDev P, const double *__restrict__ U, double *__restrict__ gphi,
-    double *__restrict__ fs, const DevScalars *sc, float *__restrict__ fs32)
+    double *__restrict__ fs, const DevScalars *sc, float *__restrict__ fs32,
+    const float *__restrict__ U32)
 {
     if (sc && sc->done)
         return;
@@ -67,7 +68,10 @@ __global__ __launch_bounds__(kBlock) void fv_ghost_kernel(
     const int f = m.bnd_face[b];
     const int c = m.fcell[2 * f];
     double pc[10], g[10];
-    load_cons(U, c, pc);
+    if (U32)
+        load_cons(U32, c, pc);
+    else
+        load_cons(U, c, pc);
     con_to_prim(P, pc);
     Frame fr;
     load_frame(m, f, fr);
@@ -105,8 +109,15 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
     if (c >= out.c_end)
         return;
 
+    // the FP32 kernel may read an FP32 copy of the state (the same values:
+    // its first operation is the rounding to FP32)
+    constexpr bool f32 = std::is_same<Real, float>::value;
+    const float *U32 = f32 ? out.U32 : nullptr;
     Real pc[10];
-    load_phi(P, U, out.prims, c, pc);
+    if (f32 && U32)
+        load_phi(P, U32, out.prims, c, pc);
+    else
+        load_phi(P, U, out.prims, c, pc);
 
     Real gx[10], gy[10], gz[10], mn[10], mx[10];
 #pragma unroll
@@ -130,15 +141,28 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
         const int2 fc = reinterpret_cast<const int2 *>(m.fcell)[f];
         const bool own0 = fc.x == c;
         const int nb = own0 ? fc.y : fc.x;
-        const double2 *g = reinterpret_cast<const double2 *>(m.fgB + (size_t)f * kFgB);
-        const double2 g0 = g[0], g1 = g[1], g2 = g[2];
-        const Real n0 = Real(g0.x), n1 = Real(g0.y), n2 = Real(g1.x);
-        const Real area = Real(g1.y), lam = Real(g2.x);
+        Real n0, n1, n2, area, lam;
+        if (f32 && m.fgB32)
+        {
+            const float4 *q = reinterpret_cast<const float4 *>(m.fgB32 + (size_t)f * 12);
+            const float4 q0 = q[0], q1 = q[1];
+            n0 = q0.x, n1 = q0.y, n2 = q0.z, area = q0.w, lam = q1.x;
+        }
+        else
+        {
+            const double2 *g = reinterpret_cast<const double2 *>(m.fgB + (size_t)f * kFgB);
+            const double2 g0 = g[0], g1 = g[1], g2 = g[2];
+            n0 = Real(g0.x), n1 = Real(g0.y), n2 = Real(g1.x);
+            area = Real(g1.y), lam = Real(g2.x);
+        }
 
         Real pn[10];
         if (nb < N)
         {
-            load_phi(P, U, out.prims, nb, pn);
+            if (f32 && U32)
+                load_phi(P, U32, out.prims, nb, pn);
+            else
+                load_phi(P, U, out.prims, nb, pn);
         }
         else
         {
@@ -233,25 +257,40 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
         Real st[8];
         if (linear)
         {
-            const double2 *g = reinterpret_cast<const double2 *>(m.fgB + (size_t)f * kFgB);
-            const double2 g2 = g[2], g3 = g[3];
             Real r[3];
-            if (own0)
+            if (f32 && m.fgB32)
             {
-                r[0] = Real(g2.y - xc0);
-                r[1] = Real(g3.x - xc1);
-                r[2] = Real(g3.y - xc2);
+                const float4 *q = reinterpret_cast<const float4 *>(m.fgB32 + (size_t)f * 12);
+                const float4 q1 = q[1], q2 = q[2];
+                if (own0)
+                    r[0] = q1.y, r[1] = q1.z, r[2] = q1.w;
+                else
+                    r[0] = q2.x, r[1] = q2.y, r[2] = q2.z;
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    st[j] = limited(pc[j], gx[j], gy[j], gz[j], r, mn[j], mx[j], P.bj);
             }
             else
             {
-                const double *sh = m.shtab + (size_t)m.fsh[f] * 3;
-                r[0] = Real(g2.y - (xc0 + sh[0]));
-                r[1] = Real(g3.x - (xc1 + sh[1]));
-                r[2] = Real(g3.y - (xc2 + sh[2]));
-            }
+                const double2 *g = reinterpret_cast<const double2 *>(m.fgB + (size_t)f * kFgB);
+                const double2 g2 = g[2], g3 = g[3];
+                if (own0)
+                {
+                    r[0] = Real(g2.y - xc0);
+                    r[1] = Real(g3.x - xc1);
+                    r[2] = Real(g3.y - xc2);
+                }
+                else
+                {
+                    const double *sh = m.shtab + (size_t)m.fsh[f] * 3;
+                    r[0] = Real(g2.y - (xc0 + sh[0]));
+                    r[1] = Real(g3.x - (xc1 + sh[1]));
+                    r[2] = Real(g3.y - (xc2 + sh[2]));
+                }
 #pragma unroll
-            for (int j = 0; j < 8; ++j)
-                st[j] = limited(pc[j], gx[j], gy[j], gz[j], r, mn[j], mx[j], P.bj);
+                for (int j = 0; j < 8; ++j)
+                    st[j] = limited(pc[j], gx[j], gy[j], gz[j], r, mn[j], mx[j], P.bj);
+            }
         }
         else
         {
@@ -352,14 +391,22 @@ __global__ __launch_bounds__(kBlock, FVH_K2_MINW) void fv_residual_group_kernel(
         }
         else
         {
+            constexpr bool f32 = std::is_same<Real, float>::value;
             Real pc[10], pn[10];
-            load_phi(P, sp.U_in, sp.prims_in, c, pc);
+            if (f32 && sp.U32)
+                load_phi(P, sp.U32, sp.prims_in, c, pc);
+            else
+                load_phi(P, sp.U_in, sp.prims_in, c, pc);
             if (ghost)
             {
                 const double *gp = sp.gphi + (size_t)(nb - N) * 10;
 #pragma unroll
                 for (int j = 0; j < 10; ++j)
                     pn[j] = Real(gp[j]);
+            }
+            else if (f32 && sp.U32)
+            {
+                load_phi(P, sp.U32, sp.prims_in, nb, pn);
             }
             else
             {
@@ -734,12 +781,12 @@ int dt_partials_grid(const DevMesh &m) { return cell_blocks(m.N); }
 
 void launch_ghost(hipStream_t s, const DevMesh &m, const PhysDev &P,
                   const double *U, double *gphi, double *fs,
-                  const DevScalars *sc, float *fs32)
+                  const DevScalars *sc, float *fs32, const float *U32)
 {
     if (m.B == 0)
         return;
     hipLaunchKernelGGL(fv_ghost_kernel, dim3(cell_blocks(m.B)), dim3(kBlock), 0,
-                       s, m, P, U, gphi, fs, sc, fs32);
+                       s, m, P, U, gphi, fs, sc, fs32, U32);
 }
 
 void launch_gradient(hipStream_t s, const DevMesh &m, const PhysDev &P,

@@ -10,6 +10,8 @@
 // converted on load.
 #pragma once
 
+#include <type_traits>
+
 #include <cfloat>
 #include <cmath>
 
@@ -97,6 +99,19 @@ using Frame = FrameT<double>;
 template <class Real>
 __device__ __forceinline__ void load_frame(const DevMesh &m, int f, FrameT<Real> &fr)
 {
+    if constexpr (std::is_same<Real, float>::value)
+    {
+        if (m.fgA32)
+        {
+            const float4 *q = reinterpret_cast<const float4 *>(m.fgA32 + (size_t)f * 12);
+            const float4 a = q[0], b = q[1], c = q[2];
+            fr.n[0] = a.x, fr.n[1] = a.y, fr.n[2] = a.z;
+            fr.t1[0] = a.w, fr.t1[1] = b.x, fr.t1[2] = b.y;
+            fr.t2[0] = b.z, fr.t2[1] = b.w, fr.t2[2] = c.x;
+            fr.area = c.y;
+            return;
+        }
+    }
     const double2 *g = reinterpret_cast<const double2 *>(m.fgA + (size_t)f * kFgA);
     const double2 a = g[0], b = g[1], c = g[2], d = g[3], e = g[4];
     fr.n[0] = Real(a.x), fr.n[1] = Real(a.y), fr.n[2] = Real(b.x);
@@ -192,11 +207,20 @@ __device__ __forceinline__ void load_cons(const double *__restrict__ U, int c, R
         phi[j] = Real(p[j]);
 }
 
+template <class Real>
+__device__ __forceinline__ void load_cons(const float *__restrict__ U, int c, Real *phi)
+{
+    const float *p = U + (size_t)c * 5;
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+        phi[j] = Real(p[j]);
+}
+
 // Conservative state and its primitives: read from the primitive rows
 // [N][5] (u, v, w, p, T) when they are kept beside the state (the values
 // con_to_prim produced from the same state, so bit-identical), else computed.
-template <class Real>
-__device__ __forceinline__ void load_phi(const PhysDev &P, const double *__restrict__ U,
+template <class Real, class UT>
+__device__ __forceinline__ void load_phi(const PhysDev &P, const UT *__restrict__ U,
                                          const double *__restrict__ prims, int c, Real *phi)
 {
     load_cons(U, c, phi);

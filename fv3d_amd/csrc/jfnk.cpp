@@ -30,6 +30,7 @@ struct Workspace
            *Minv = nullptr, *xacc = nullptr, *dU = nullptr, *partials = nullptr,
            *dots = nullptr, *ydev = nullptr, *Mex = nullptr;
     void *V = nullptr, *z = nullptr, *t = nullptr; // T storage
+    float *U32 = nullptr, *Ut32 = nullptr;         // FP32 state copies (FP32 mode)
     double *host = nullptr;                        // pinned, kHost doubles
     int *again = nullptr;                          // device flag of the Arnoldi step
     hipEvent_t ev[2] = {nullptr, nullptr};         // Arnoldi slot copies done
@@ -100,6 +101,11 @@ void prepare(int N, int m, int prec)
     const size_t es = prec == FVH_JFNK_FP32 ? sizeof(float) : sizeof(double);
     g_ws.R0 = walloc<double>(n);
     g_ws.R0j = prec == FVH_JFNK_FP32 ? walloc<double>(n) : g_ws.R0;
+    if (prec == FVH_JFNK_FP32)
+    {
+        g_ws.U32 = walloc<float>(n);
+        g_ws.Ut32 = walloc<float>(n);
+    }
     g_ws.Rt = walloc<double>(n);
     g_ws.Ut = walloc<double>(n);
     g_ws.dtau = walloc<double>(N);
@@ -312,8 +318,9 @@ GmresResult gmres(const RuntimeView &v, const fvh_jfnk_options_t &o, double fd_n
     double beta0 = 0.0;
 
     // w = A z into out, with |z|^2 already in dots[kZZ]; in FP32 mode the
-    // difference quotient takes two FP32-arithmetic residuals (R0j is the
-    // FP32 residual of U)
+    // difference quotient takes two FP32-arithmetic residuals of FP32 states
+    // (R0j is the FP32 residual of U32, the FP32 rounding of U; the
+    // perturbed state is rounded once, after the FP64 sum)
     const bool fp32 = o.precision == FVH_JFNK_FP32;
     // z = M^-1 in with |z|^2 into dots[kZZ] (the coupled preconditioner's
     // factors are stored in reduced precision in both precisions)
@@ -329,8 +336,16 @@ GmresResult gmres(const RuntimeView &v, const fvh_jfnk_options_t &o, double fd_n
             kry::precond_apply_norm<T>(s, N, g_ws.Minv, in, out, g_ws.partials, dots + kZZ);
     };
     auto apply_op = [&](const T *zin, T *out) {
-        kry::perturb<T>(s, n, v.U, dots + kZZ, fd_num, zin, g_ws.Ut, dots + kEps);
-        eval_residual(g_ws.Ut, g_ws.Rt, fp32);
+        if (fp32)
+        {
+            kry::perturb<T, float>(s, n, v.U, dots + kZZ, fd_num, zin, g_ws.Ut32, dots + kEps);
+            eval_residual(g_ws.Ut, g_ws.Rt, true, g_ws.Ut32);
+        }
+        else
+        {
+            kry::perturb<T, double>(s, n, v.U, dots + kZZ, fd_num, zin, g_ws.Ut, dots + kEps);
+            eval_residual(g_ws.Ut, g_ws.Rt, false);
+        }
         kry::op_finish<T>(s, N, zin, g_ws.dtau, g_ws.Rt, g_ws.R0j, dots + kEps, out);
         ++res.evals;
     };
@@ -534,7 +549,8 @@ int solve(const fvh_jfnk_options_t &o, fvh_jfnk_stats_t &st)
         const double fd_num = (fp32 ? o.fd_eps32 : o.fd_eps) * (1.0 + std::sqrt(g_ws.host[kUU]));
         if (fp32)
         {
-            eval_residual(v.U, g_ws.R0j, true);
+            kry::convert<float>(s, n, v.U, g_ws.U32);
+            eval_residual(v.U, g_ws.R0j, true, g_ws.U32);
             ++st.residual_evals;
         }
         GmresResult gr = o.precision == FVH_JFNK_FP32 ? gmres<float>(v, o, fd_num)

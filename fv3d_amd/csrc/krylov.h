@@ -89,11 +89,12 @@ template <typename T, typename B, typename O>
 void dilu_apply(hipStream_t s, const DiluMesh &d, const int *coff, const O *off,
                 const B *dinv, const T *v, T *z);
 
-// Ut = U + eps * z with eps = num / sqrt(zz[0]) (num / 1e-300 for zz = 0),
-// zz in device memory; eps also stored to eps[0]
-template <typename T>
+// Ut = U + eps * z (computed in FP64, stored as O) with eps = num /
+// sqrt(zz[0]) (num / 1e-300 for zz = 0), zz in device memory; eps also
+// stored to eps[0]
+template <typename T, typename O>
 void perturb(hipStream_t s, int n, const double *U, const double *zz, double num,
-             const T *z, double *Ut, double *eps);
+             const T *z, O *Ut, double *eps);
 
 // w = z / dtau - (Rt - R0) / eps[0]   (A z with A = I/dtau - J, J z by
 // finite difference)

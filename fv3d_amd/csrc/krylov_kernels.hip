@@ -204,11 +204,11 @@ __global__ __launch_bounds__(kBlock) void k_precond_apply_norm(int N,
         partials[(size_t)blockIdx.x * kMaxDots] = acc;
 }
 
-template <typename T>
+template <typename T, typename O>
 __global__ __launch_bounds__(kBlock) void k_perturb(int n, const double *__restrict__ U,
                                                     const double *__restrict__ zz, double num,
                                                     const T *__restrict__ z,
-                                                    double *__restrict__ Ut,
+                                                    O *__restrict__ Ut,
                                                     double *__restrict__ eps_out)
 {
     // eps = num / |z| (num / 1e-300 for z = 0)
@@ -218,7 +218,7 @@ __global__ __launch_bounds__(kBlock) void k_perturb(int n, const double *__restr
     if (i == 0)
         eps_out[0] = eps;
     if (i < n)
-        Ut[i] = U[i] + eps * (double)z[i];
+        Ut[i] = (O)(U[i] + eps * (double)z[i]);
 }
 
 template <typename T>
@@ -874,13 +874,22 @@ void precond_apply_norm(hipStream_t s, int N, const double *Minv, const T *v, T 
                        (const int *)nullptr);
 }
 
-template <typename T>
+template <typename T, typename O>
 void perturb(hipStream_t s, int n, const double *U, const double *zz, double num,
-             const T *z, double *Ut, double *eps)
+             const T *z, O *Ut, double *eps)
 {
-    hipLaunchKernelGGL(k_perturb<T>, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, U, zz, num,
-                       z, Ut, eps);
+    hipLaunchKernelGGL((k_perturb<T, O>), dim3(blocks_for(n)), dim3(kBlock), 0, s, n, U, zz,
+                       num, z, Ut, eps);
 }
+
+template void perturb<double, double>(hipStream_t, int, const double *, const double *, double,
+                                      const double *, double *, double *);
+template void perturb<float, double>(hipStream_t, int, const double *, const double *, double,
+                                     const float *, double *, double *);
+template void perturb<float, float>(hipStream_t, int, const double *, const double *, double,
+                                    const float *, float *, double *);
+template void perturb<double, float>(hipStream_t, int, const double *, const double *, double,
+                                     const double *, float *, double *);
 
 template <typename T>
 void op_finish(hipStream_t s, int N, const T *z, const double *dtau, const double *Rt,
@@ -1079,8 +1088,6 @@ FVH_DILU_APPLY(double, float, bf16_t)
     template void precond_apply<T>(hipStream_t, int, const double *, const T *, T *);      \
     template void precond_apply_norm<T>(hipStream_t, int, const double *, const T *, T *,  \
                                         double *, double *);                               \
-    template void perturb<T>(hipStream_t, int, const double *, const double *, double,     \
-                             const T *, double *, double *);                               \
     template void op_finish<T>(hipStream_t, int, const T *, const double *,                \
                                const double *, const double *, const double *, T *);       \
     template void rhs_minus<T>(hipStream_t, int, const double *, T *);                     \

@@ -56,8 +56,10 @@ StageBuffers stage_buffers();
 
 // Enqueues R = dU/dt(U) (finite_volume_time_derivative semantics, the
 // reference's ghost rows recomputed from U) on the runtime stream.  fp32:
-// the gradient and flux arithmetic in FP32 (for Jacobian-vector products).
-void eval_residual(const double *U, double *R, bool fp32 = false);
+// the gradient and flux arithmetic in FP32 (for Jacobian-vector products),
+// with the state read from U32 when given (an FP32 copy; U is then unused
+// on the linear-reconstruction path).
+void eval_residual(const double *U, double *R, bool fp32 = false, const float *U32 = nullptr);
 
 // The current state was changed outside the time loop: the cached residual
 // and host mirror are stale.

@@ -157,6 +157,7 @@ struct Device
     double *vg = nullptr;   // [12*N] viscous gradient subset
     double *fs = nullptr;   // [16*F] face states
     float *fs32 = nullptr;  // [16*F] face states of the FP32 residual
+    float *fgA32 = nullptr, *fgB32 = nullptr; // FP32 face records (FP32 residual)
     double *grad = nullptr; // [30*N] all gradients (tests only)
     double *mnmx = nullptr; // [20*N] min/max (tests only)
     double *partials = nullptr;
@@ -278,6 +279,49 @@ void require_ready()
             "(calc_ns_flux, ns_update, ns_update_gradients)");
 }
 
+// FP32 face records of the FP32 residual (layouts in dev::DevMesh), built
+// on first use from the host mesh: each value the FP32 rounding of the FP64
+// one, the cell-to-face vectors from the FP64 differences (through the
+// periodic image on side 1), as the FP32 kernels compute them from the FP64
+// records.
+void ensure_fp32_geometry()
+{
+    if (g_dev.m.fgA32)
+        return;
+    const Mesh_t *mm = solver_mesh;
+    const Faces_t *faces = mm->faces;
+    const Cells_t *cells = mm->cells;
+    const int F = g_dev.m.F, N = g_dev.m.N;
+    std::vector<float> a((size_t)F * 12, 0.0f), b((size_t)F * 12, 0.0f);
+    for (int f = 0; f < F; ++f)
+    {
+        float *ra = &a[(size_t)f * 12];
+        float *rb = &b[(size_t)f * 12];
+        for (int d = 0; d < 3; ++d)
+        {
+            ra[d] = (float)faces->n[3 * f + d];
+            ra[3 + d] = (float)faces->t1[3 * f + d];
+            ra[6 + d] = (float)faces->t2[3 * f + d];
+            rb[d] = (float)faces->n[3 * f + d];
+        }
+        ra[9] = (float)faces->area[f];
+        rb[3] = (float)faces->area[f];
+        rb[4] = (float)faces->lambda[f];
+        const int c0 = faces->cells[2 * f], c1 = faces->cells[2 * f + 1];
+        const double *xf = &faces->x[3 * f];
+        const double *sh = &g_face_shift[(size_t)3 * f];
+        for (int d = 0; d < 3; ++d)
+        {
+            rb[5 + d] = (float)(xf[d] - cells->x[3 * c0 + d]);
+            rb[8 + d] = c1 < N ? (float)(xf[d] - (cells->x[3 * c1 + d] + sh[d])) : 0.0f;
+        }
+    }
+    g_dev.fgA32 = dupload(a.data(), a.size());
+    g_dev.fgB32 = dupload(b.data(), b.size());
+    g_dev.m.fgA32 = g_dev.fgA32;
+    g_dev.m.fgB32 = g_dev.fgB32;
+}
+
 void ensure_gradient_buffers()
 {
     size_t N = (size_t)g_dev.m.N, F = (size_t)g_dev.m.F;
@@ -327,21 +371,28 @@ void ensure_prims()
 void enqueue_residual(const double *U_in, double *U_out, double *R_out,
                       int stage, bool update, bool reduce, dev::DevScalars *sc,
                       const double *prims_in = nullptr, double *prims_out = nullptr,
-                      KernelTimer *tm = nullptr, bool fp32 = false)
+                      KernelTimer *tm = nullptr, bool fp32 = false,
+                      const float *U32 = nullptr)
 {
     const dev::PhysDev P = phys();
     ensure_gradient_buffers();
-    // the FP32 residual keeps its face states in FP32
+    // the FP32 residual keeps its face states and face records in FP32 and
+    // may take its state in FP32
     const bool fs32 = fp32 && is_linear();
     if (fs32 && !g_dev.fs32)
         g_dev.fs32 = dmalloc<float>(16 * (size_t)g_dev.m.F);
+    if (fp32)
+        ensure_fp32_geometry();
+    else
+        U32 = nullptr;
     dev::launch_ghost(g_dev.stream, g_dev.m, P, U_in, g_dev.gphi,
                       is_linear() && !fs32 ? g_dev.fs : nullptr, sc,
-                      fs32 ? g_dev.fs32 : nullptr);
+                      fs32 ? g_dev.fs32 : nullptr, U32);
     if (needs_gradients())
     {
         dev::GradOut go = grad_out();
         go.prims = prims_in;
+        go.U32 = U32;
         if (fs32)
         {
             go.fs = nullptr;
@@ -362,6 +413,7 @@ void enqueue_residual(const double *U_in, double *U_out, double *R_out,
     sp.vg = g_dev.vg;
     sp.fs = g_dev.fs;
     sp.fs32 = fs32 ? g_dev.fs32 : nullptr;
+    sp.U32 = U32;
     sp.gphi = g_dev.gphi;
     sp.rk_a = update ? g_rk_a[stage] : 0.0;
     sp.rk_g = update ? g_rk_g[stage] : 0.0;
@@ -1017,10 +1069,10 @@ RuntimeView runtime_view()
     return v;
 }
 
-void eval_residual(const double *U, double *R, bool fp32)
+void eval_residual(const double *U, double *R, bool fp32, const float *U32)
 {
     enqueue_residual(U, nullptr, R, 0, false, false, nullptr, nullptr, nullptr, nullptr,
-                     fp32);
+                     fp32, U32);
 }
 
 void state_changed() { g_dev.r_valid = false; }
