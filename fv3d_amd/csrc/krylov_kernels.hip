@@ -749,7 +749,9 @@ __global__ __launch_bounds__(kBlock) void k_dilu_factor(DiluMesh d, int p0, int 
 // included) are replaced by zero, so those slots contribute exactly zero.
 //   FORWARD: z_i = D_i^-1 (v_i - sum_{k lower} A_ik z_k)
 //   backward: z_i -= D_i^-1 sum_{k higher} A_ik z_k
-template <typename T, typename B, typename O, int MAXF, bool FORWARD>
+// NOFF: the forward sweep of the lowest colour, whose cells have no lower
+// neighbours (z_i = D_i^-1 v_i, no block or neighbour loads).
+template <typename T, typename B, typename O, int MAXF, bool FORWARD, bool NOFF = false>
 __global__ __launch_bounds__(kBlock) void k_dilu_sweep(DiluMesh d, int p0, int p1,
                                                        const O *__restrict__ off,
                                                        const B *__restrict__ dinv,
@@ -766,7 +768,7 @@ __global__ __launch_bounds__(kBlock) void k_dilu_sweep(DiluMesh d, int p0, int p
     const int E = d.maxf * 25;
     double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int t = 0; t < MAXF; ++t)
+    for (int t = 0; t < (NOFF ? 0 : MAXF); ++t)
     {
         const bool on = (mask >> t) & 1;
         const int kn = d.nbr[tiled(p, t, d.maxf)];
@@ -1042,7 +1044,13 @@ template <typename T, typename B, typename O, int MAXF>
 void dilu_sweeps(hipStream_t s, const DiluMesh &d, const int *coff, const O *off,
                  const B *dinv, const T *v, T *z)
 {
-    for (int q = 0; q < d.ncol; ++q)
+    // colour 0 has no lower neighbours (a greedy colouring gives every cell
+    // of colour q > 0 a neighbour of each lower colour, and none to colour 0)
+    if (d.ncol > 0 && coff[1] > coff[0])
+        hipLaunchKernelGGL((k_dilu_sweep<T, B, O, MAXF, true, true>),
+                           dim3(blocks_for(coff[1] - coff[0])), dim3(kBlock), 0, s, d, coff[0],
+                           coff[1], off, dinv, v, z);
+    for (int q = 1; q < d.ncol; ++q)
         if (coff[q + 1] > coff[q])
             hipLaunchKernelGGL((k_dilu_sweep<T, B, O, MAXF, true>),
                                dim3(blocks_for(coff[q + 1] - coff[q])), dim3(kBlock), 0, s, d,
