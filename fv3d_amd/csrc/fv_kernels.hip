@@ -327,8 +327,13 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
 // FS: face states from K1 (reconstructed path); otherwise first order, the
 // face states are the cell values of U.
 // ---------------------------------------------------------------------------
+// The FP64 viscous kernel on face states is held to 4 waves per SIMD (128
+// VGPRs, a 12-byte spill), which measured 2% faster than 3 waves; the other
+// instantiations keep the compiler's choice (more spill at 4 waves).
 template <int FLUX, bool FS, bool VISC, int LANES, class Real>
-__global__ __launch_bounds__(kBlock, FVH_K2_MINW) void fv_residual_group_kernel(
+__global__ __launch_bounds__(kBlock, (FS && VISC && std::is_same<Real, double>::value)
+                                         ? 4
+                                         : FVH_K2_MINW) void fv_residual_group_kernel(
     DevMesh m, PhysDev P, StageParams sp)
 {
     __shared__ double sh[kBlock];
