@@ -36,6 +36,10 @@ struct DevMesh
     const int *__restrict__ cf;     // [N*maxf]
     const int *__restrict__ ncf;    // [N]
     const int *__restrict__ fcell;  // [F*2]
+    // per cell face slot: {face, v} with v = neighbour when the cell is the
+    // face's first cell, else -(neighbour + 1); slots past the cell's face
+    // count repeat slot 0 [N*maxf]
+    const int2 *__restrict__ cfn;
     const double *__restrict__ fgA; // [F*kFgA]
     const double *__restrict__ fgB; // [F*kFgB]
     const int *__restrict__ fsh;    // [F]

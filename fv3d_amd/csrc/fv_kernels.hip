@@ -33,12 +33,23 @@
 #include "fv_physics.h"
 
 // Minimum resident waves per SIMD requested from the register allocator for
-// the two stencil kernels (occupancy against spilling).
+// the two stencil kernels (occupancy against spilling).  The gradient kernel
+// keeps a batch of neighbour rows and face records in registers (about 247
+// VGPRs in FP64 with batches of 4 faces): 2 waves per SIMD without spills.
+// Batches of 4 at 3 waves spill and ran at half the speed; batches of 1 at 3
+// waves ran at the same speed with 44% more HBM reads.
 #ifndef FVH_K1_MINW
-#define FVH_K1_MINW 1
+#define FVH_K1_MINW 2
+#endif
+// Faces whose neighbour rows and records the gradient kernel loads together.
+#ifndef FVH_K1_FB
+#define FVH_K1_FB 4
 #endif
 #ifndef FVH_K2_MINW
 #define FVH_K2_MINW 1
+#endif
+#ifndef FVH_K2_VISC_W
+#define FVH_K2_VISC_W 3
 #endif
 // Dynamic LDS reserved per gradient-kernel workgroup; a non-zero value caps
 // the resident workgroups per CU (occupancy experiments).
@@ -96,7 +107,9 @@ __global__ __launch_bounds__(kBlock) void fv_ghost_kernel(
 // (reconstruction.c:168-255 / 142-163).  Writes the face states by face id
 // and side, and the gradient subset the viscous flux needs.
 // ---------------------------------------------------------------------------
-template <int MAXF, class Real>
+// ALL: also the gradient of p and the bounds of p and T, which only the
+// test outputs grad/mnmx read (fewer live registers without them).
+template <int MAXF, class Real, bool ALL>
 __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
     DevMesh m, PhysDev P, const double *__restrict__ U,
     const double *__restrict__ gphi, GradOut out, int linear,
@@ -109,13 +122,13 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
     if (c >= out.c_end)
         return;
 
-    // the FP32 kernel may read an FP32 copy of the state (the same values:
-    // its first operation is the rounding to FP32)
+    // the FP32 kernel reads the FP32 copy of the state (the same values: its
+    // first operation is the rounding to FP32) and the FP32 face records
     constexpr bool f32 = std::is_same<Real, float>::value;
-    const float *U32 = f32 ? out.U32 : nullptr;
+    const float *U32 = out.U32;
     Real pc[10];
-    if (f32 && U32)
-        load_phi(P, U32, out.prims, c, pc);
+    if (f32)
+        load_phi(P, U32, nullptr, c, pc);
     else
         load_phi(P, U, out.prims, c, pc);
 
@@ -130,72 +143,118 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
         mx[j] = pc[j];
     }
 
+    // Loads in rounds, each for several faces at once (the cell's face and
+    // neighbour list; neighbour rows and face records, in batches of up to 4
+    // faces), so that a wave waits for a few memory latencies rather than
+    // three per face.  Slots s >= nf repeat slot 0 and are skipped in the
+    // sums.
+    constexpr int FB = MAXF % FVH_K1_FB == 0 ? FVH_K1_FB : (MAXF % 3 == 0 ? 3 : MAXF);
     const int nf = m.ncf[c];
-    const int *cfc = m.cf + (size_t)c * m.maxf;
+    const int2 *cfn = m.cfn + (size_t)c * m.maxf;
+    int fid[MAXF];
+    bool own[MAXF];
+    int nbr[MAXF], shi[MAXF];
 #pragma unroll
     for (int s = 0; s < MAXF; ++s)
     {
-        if (s >= nf)
-            break;
-        const int f = cfc[s];
-        const int2 fc = reinterpret_cast<const int2 *>(m.fcell)[f];
-        const bool own0 = fc.x == c;
-        const int nb = own0 ? fc.y : fc.x;
-        Real n0, n1, n2, area, lam;
-        if (f32 && m.fgB32)
+        const int2 e = cfn[s];
+        fid[s] = e.x;
+        own[s] = e.y >= 0;
+        nbr[s] = own[s] ? e.y : -(e.y + 1);
+    }
+#pragma unroll
+    for (int s = 0; s < MAXF; ++s)
+        shi[s] = (!f32 && linear) ? m.fsh[fid[s]] : 0;
+
+#pragma unroll
+    for (int s0 = 0; s0 < MAXF; s0 += FB)
+    {
+        // neighbour rows (boundary faces: the cell's own row as a
+        // placeholder, replaced by the ghost row written by fv_ghost_kernel)
+        // and the face normals, areas and weights of this batch
+        Real pn[FB][10];
+        Real gn[FB][5];
+#pragma unroll
+        for (int i = 0; i < FB; ++i)
         {
-            const float4 *q = reinterpret_cast<const float4 *>(m.fgB32 + (size_t)f * 12);
-            const float4 q0 = q[0], q1 = q[1];
-            n0 = q0.x, n1 = q0.y, n2 = q0.z, area = q0.w, lam = q1.x;
-        }
-        else
-        {
-            const double2 *g = reinterpret_cast<const double2 *>(m.fgB + (size_t)f * kFgB);
-            const double2 g0 = g[0], g1 = g[1], g2 = g[2];
-            n0 = Real(g0.x), n1 = Real(g0.y), n2 = Real(g1.x);
-            area = Real(g1.y), lam = Real(g2.x);
+            const int s = s0 + i;
+            const int nb = nbr[s] < N ? nbr[s] : c;
+            if (f32)
+                load_cons(U32, nb, pn[i]);
+            else
+                load_cons(U, nb, pn[i]);
+            if (!f32 && out.prims)
+            {
+                const double *q = out.prims + (size_t)nb * 5;
+#pragma unroll
+                for (int j = 0; j < 5; ++j)
+                    pn[i][5 + j] = Real(q[j]);
+            }
+            if (f32)
+            {
+                const float4 *q = reinterpret_cast<const float4 *>(m.fgB32 + (size_t)fid[s] * 12);
+                const float4 q0 = q[0], q1 = q[1];
+                gn[i][0] = q0.x, gn[i][1] = q0.y, gn[i][2] = q0.z, gn[i][3] = q0.w;
+                gn[i][4] = q1.x;
+            }
+            else
+            {
+                const double2 *g =
+                    reinterpret_cast<const double2 *>(m.fgB + (size_t)fid[s] * kFgB);
+                const double2 g0 = g[0], g1 = g[1], g2 = g[2];
+                gn[i][0] = Real(g0.x), gn[i][1] = Real(g0.y), gn[i][2] = Real(g1.x);
+                gn[i][3] = Real(g1.y), gn[i][4] = Real(g2.x);
+            }
         }
 
-        Real pn[10];
-        if (nb < N)
+#pragma unroll
+        for (int i = 0; i < FB; ++i)
         {
-            if (f32 && U32)
-                load_phi(P, U32, out.prims, nb, pn);
-            else
-                load_phi(P, U, out.prims, nb, pn);
-        }
-        else
-        {
-            // boundary face: the ghost row written by fv_ghost_kernel
-            const double *gp = gphi + (size_t)(nb - N) * 10;
+            const int s = s0 + i;
+            if (s >= nf)
+                break;
+            if (nbr[s] >= N)
+            {
+                const double *gp = gphi + (size_t)(nbr[s] - N) * 10;
+#pragma unroll
+                for (int j = 0; j < 10; ++j)
+                    pn[i][j] = Real(gp[j]);
+            }
+            else if (f32 || !out.prims)
+                con_to_prim(P, pn[i]);
+            const bool own0 = own[s];
+            const Real n0 = gn[i][0], n1 = gn[i][1], n2 = gn[i][2], area = gn[i][3],
+                       lam = gn[i][4];
 #pragma unroll
             for (int j = 0; j < 10; ++j)
-                pn[j] = Real(gp[j]);
-        }
-
-#pragma unroll
-        for (int j = 0; j < 10; ++j)
-        {
-            const Real p0 = own0 ? pc[j] : pn[j];
-            const Real p1 = own0 ? pn[j] : pc[j];
-            const Real pm = p0 + lam * (p1 - p0);
-            const Real ax = pm * n0 * area;
-            const Real ay = pm * n1 * area;
-            const Real az = pm * n2 * area;
-            if (own0)
             {
-                gx[j] += ax;
-                gy[j] += ay;
-                gz[j] += az;
+                const Real p0 = own0 ? pc[j] : pn[i][j];
+                const Real p1 = own0 ? pn[i][j] : pc[j];
+                const Real pm = p0 + lam * (p1 - p0);
+                const Real ax = pm * n0 * area;
+                const Real ay = pm * n1 * area;
+                const Real az = pm * n2 * area;
+                if (ALL || j != 8)
+                {
+                    if (own0)
+                    {
+                        gx[j] += ax;
+                        gy[j] += ay;
+                        gz[j] += az;
+                    }
+                    else
+                    {
+                        gx[j] -= ax;
+                        gy[j] -= ay;
+                        gz[j] -= az;
+                    }
+                }
+                if (ALL || j < 8)
+                {
+                    mn[j] = rmin(mn[j], pn[i][j]);
+                    mx[j] = rmax(mx[j], pn[i][j]);
+                }
             }
-            else
-            {
-                gx[j] -= ax;
-                gy[j] -= ay;
-                gz[j] -= az;
-            }
-            mn[j] = rmin(mn[j], pn[j]);
-            mx[j] = rmax(mx[j], pn[j]);
         }
     }
 
@@ -218,7 +277,7 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
         d[4] = make_double2(double(gz[1]), double(gz[2]));
         d[5] = make_double2(double(gz[3]), double(gz[9]));
     }
-    if (out.grad)
+    if (ALL && out.grad)
     {
         double *d = out.grad + (size_t)c * 30;
 #pragma unroll
@@ -229,7 +288,7 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
             d[20 + j] = gz[j];
         }
     }
-    if (out.mnmx)
+    if (ALL && out.mnmx)
     {
         double *d = out.mnmx + (size_t)c * 20;
 #pragma unroll
@@ -247,50 +306,54 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
     // x_f - x_c1 through the periodic image) from the face and cell centres
     const double xc0 = m.cx[(size_t)c * 3], xc1 = m.cx[(size_t)c * 3 + 1],
                  xc2 = m.cx[(size_t)c * 3 + 2];
+    // distance vectors of all faces first (one round of loads), then the
+    // limited states
+    Real rv[MAXF][3];
+    if (linear)
+    {
+#pragma unroll
+        for (int s = 0; s < MAXF; ++s)
+        {
+            const int f = fid[s];
+            if (f32)
+            {
+                const float4 *q = reinterpret_cast<const float4 *>(m.fgB32 + (size_t)f * 12);
+                const float4 q1 = q[1], q2 = q[2];
+                rv[s][0] = own[s] ? q1.y : q2.x;
+                rv[s][1] = own[s] ? q1.z : q2.y;
+                rv[s][2] = own[s] ? q1.w : q2.z;
+            }
+            else
+            {
+                // side 1 through the periodic image (shift entry 0 is zero);
+                // side 0 keeps the unshifted difference.  The shift is loaded
+                // for both sides so that the loads are not conditional.
+                const double *g = m.fgB + (size_t)f * kFgB;
+                const double xf0 = g[5], xf1 = g[6], xf2 = g[7];
+                const double *sh = m.shtab + (size_t)shi[s] * 3;
+                const double s0 = sh[0], s1 = sh[1], s2 = sh[2];
+                const double c0 = own[s] ? xc0 : xc0 + s0;
+                const double c1 = own[s] ? xc1 : xc1 + s1;
+                const double c2 = own[s] ? xc2 : xc2 + s2;
+                rv[s][0] = Real(xf0 - c0);
+                rv[s][1] = Real(xf1 - c1);
+                rv[s][2] = Real(xf2 - c2);
+            }
+        }
+    }
 #pragma unroll
     for (int s = 0; s < MAXF; ++s)
     {
         if (s >= nf)
             break;
-        const int f = cfc[s];
-        const bool own0 = m.fcell[2 * f] == c;
+        const int f = fid[s];
+        const bool own0 = own[s];
         Real st[8];
         if (linear)
         {
-            Real r[3];
-            if (f32 && m.fgB32)
-            {
-                const float4 *q = reinterpret_cast<const float4 *>(m.fgB32 + (size_t)f * 12);
-                const float4 q1 = q[1], q2 = q[2];
-                if (own0)
-                    r[0] = q1.y, r[1] = q1.z, r[2] = q1.w;
-                else
-                    r[0] = q2.x, r[1] = q2.y, r[2] = q2.z;
 #pragma unroll
-                for (int j = 0; j < 8; ++j)
-                    st[j] = limited(pc[j], gx[j], gy[j], gz[j], r, mn[j], mx[j], P.bj);
-            }
-            else
-            {
-                const double2 *g = reinterpret_cast<const double2 *>(m.fgB + (size_t)f * kFgB);
-                const double2 g2 = g[2], g3 = g[3];
-                if (own0)
-                {
-                    r[0] = Real(g2.y - xc0);
-                    r[1] = Real(g3.x - xc1);
-                    r[2] = Real(g3.y - xc2);
-                }
-                else
-                {
-                    const double *sh = m.shtab + (size_t)m.fsh[f] * 3;
-                    r[0] = Real(g2.y - (xc0 + sh[0]));
-                    r[1] = Real(g3.x - (xc1 + sh[1]));
-                    r[2] = Real(g3.y - (xc2 + sh[2]));
-                }
-#pragma unroll
-                for (int j = 0; j < 8; ++j)
-                    st[j] = limited(pc[j], gx[j], gy[j], gz[j], r, mn[j], mx[j], P.bj);
-            }
+            for (int j = 0; j < 8; ++j)
+                st[j] = limited(pc[j], gx[j], gy[j], gz[j], rv[s], mn[j], mx[j], P.bj);
         }
         else
         {
@@ -327,12 +390,13 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
 // FS: face states from K1 (reconstructed path); otherwise first order, the
 // face states are the cell values of U.
 // ---------------------------------------------------------------------------
-// The FP64 viscous kernel on face states is held to 4 waves per SIMD (128
-// VGPRs, a 12-byte spill), which measured 2% faster than 3 waves; the other
-// instantiations keep the compiler's choice (more spill at 4 waves).
+// The FP64 viscous kernel on face states is held to 3 waves per SIMD
+// (FVH_K2_VISC_W): with the epilogue operands loaded up front it needs about
+// 146 VGPRs, and at 4 waves its spills made it 1.6x slower; the other
+// instantiations keep the compiler's choice.
 template <int FLUX, bool FS, bool VISC, int LANES, class Real>
 __global__ __launch_bounds__(kBlock, (FS && VISC && std::is_same<Real, double>::value)
-                                         ? 4
+                                         ? FVH_K2_VISC_W
                                          : FVH_K2_MINW) void fv_residual_group_kernel(
     DevMesh m, PhysDev P, StageParams sp)
 {
@@ -354,13 +418,30 @@ __global__ __launch_bounds__(kBlock, (FS && VISC && std::is_same<Real, double>::
     }
 
     const int nf = cell_ok ? m.ncf[c] : 0;
+    // the epilogue's operands (the cell's state and RK register, element j in
+    // lane j % LANES of the group, and its volume), loaded before the face
+    // work so that their latency overlaps it
+    constexpr int NQ = (5 + LANES - 1) / LANES;
+    const int cc = cell_ok ? c : sp.c_begin;
+    double pu[NQ], pk[NQ];
+    if (sp.do_update)
+    {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+        {
+            const int j = q * LANES + s < 5 ? q * LANES + s : 0;
+            pu[q] = sp.U_in[(size_t)cc * 5 + j];
+            pk[q] = sp.stage0 ? 0.0 : sp.k[(size_t)cc * 5 + j];
+        }
+    }
+    const double vol = m.vol[cc];
     Real contrib[5] = {Real(0), Real(0), Real(0), Real(0), Real(0)};
     if (s < nf)
     {
-        const int f = m.cf[(size_t)c * m.maxf + s];
-        const int2 fc = reinterpret_cast<const int2 *>(m.fcell)[f];
-        const bool own0 = fc.x == c;
-        const int nb = own0 ? fc.y : fc.x;
+        const int2 e = m.cfn[(size_t)c * m.maxf + s];
+        const int f = e.x;
+        const bool own0 = e.y >= 0;
+        const int nb = own0 ? e.y : -(e.y + 1);
         const bool ghost = nb >= N;
         FrameT<Real> fr;
         load_frame(m, f, fr);
@@ -476,10 +557,20 @@ __global__ __launch_bounds__(kBlock, (FS && VISC && std::is_same<Real, double>::
         acc[j] = a;
     }
 
+    double u5[5], k5[5];
+    if (sp.do_update)
+    {
+#pragma unroll
+        for (int j = 0; j < 5; ++j)
+        {
+            u5[j] = __shfl(pu[j / LANES], j % LANES, LANES);
+            k5[j] = __shfl(pk[j / LANES], j % LANES, LANES);
+        }
+    }
+
     double part[7] = {0.0, 0.0, 0.0, 0.0, 0.0, DBL_MAX, DBL_MAX};
     if (cell_ok && s == 0)
     {
-        const double vol = m.vol[c];
         double R5[5];
 #pragma unroll
         for (int j = 0; j < 5; ++j)
@@ -493,7 +584,9 @@ __global__ __launch_bounds__(kBlock, (FS && VISC && std::is_same<Real, double>::
         if (sp.do_update)
         {
             double pc[10];
-            load_cons(sp.U_in, c, pc);
+#pragma unroll
+            for (int j = 0; j < 5; ++j)
+                pc[j] = u5[j];
             const double dt = sp.case_dt ? sp.case_dt[sp.cell_case[c]] : sp.sc->dt;
             double unew[5];
 #pragma unroll
@@ -501,7 +594,7 @@ __global__ __launch_bounds__(kBlock, (FS && VISC && std::is_same<Real, double>::
             {
                 double kj = R5[j];
                 if (!sp.stage0)
-                    kj = R5[j] + sp.k[(size_t)c * 5 + j] * sp.rk_a;
+                    kj = R5[j] + k5[j] * sp.rk_a;
                 sp.k[(size_t)c * 5 + j] = kj;
                 unew[j] = pc[j] + kj * dt * sp.rk_g;
                 sp.U_out[(size_t)c * 5 + j] = unew[j];
@@ -810,20 +903,36 @@ void launch_gradient(hipStream_t s, const DevMesh &m, const PhysDev &P,
     const int grid = cell_blocks(out.c_end - out.c_begin);
     if (fp32)
     {
+        if (out.grad || out.mnmx)
+            throw std::runtime_error("launch_gradient: the test outputs are FP64 only");
+        if (!out.U32 || !m.fgB32)
+            throw std::runtime_error("launch_gradient: FP32 needs the FP32 state and face records");
         if (m.maxf <= 4)
-            hipLaunchKernelGGL((fv_gradient_kernel<4, float>), dim3(grid), dim3(kBlock),
+            hipLaunchKernelGGL((fv_gradient_kernel<4, float, false>), dim3(grid), dim3(kBlock),
                                FVH_K1_LDS, s, m, P, U, gphi, out, linear, sc);
         else
-            hipLaunchKernelGGL((fv_gradient_kernel<6, float>), dim3(grid), dim3(kBlock),
+            hipLaunchKernelGGL((fv_gradient_kernel<6, float, false>), dim3(grid), dim3(kBlock),
                                FVH_K1_LDS, s, m, P, U, gphi, out, linear, sc);
         return;
     }
+    if (out.grad || out.mnmx)
+    {
+        if (m.maxf <= 4)
+            hipLaunchKernelGGL((fv_gradient_kernel<4, double, true>), dim3(grid),
+                               dim3(kBlock), FVH_K1_LDS, s, m, P, U, gphi, out, linear,
+                               sc);
+        else
+            hipLaunchKernelGGL((fv_gradient_kernel<6, double, true>), dim3(grid),
+                               dim3(kBlock), FVH_K1_LDS, s, m, P, U, gphi, out, linear,
+                               sc);
+        return;
+    }
     if (m.maxf <= 4)
-        hipLaunchKernelGGL((fv_gradient_kernel<4, double>), dim3(grid),
+        hipLaunchKernelGGL((fv_gradient_kernel<4, double, false>), dim3(grid),
                            dim3(kBlock), FVH_K1_LDS, s, m, P, U, gphi, out, linear,
                            sc);
     else
-        hipLaunchKernelGGL((fv_gradient_kernel<6, double>), dim3(grid),
+        hipLaunchKernelGGL((fv_gradient_kernel<6, double, false>), dim3(grid),
                            dim3(kBlock), FVH_K1_LDS, s, m, P, U, gphi, out, linear,
                            sc);
 }

@@ -1275,6 +1275,21 @@ void init_finite_volume(void)
         m.ncf = dupload(cells->n_faces, N);
         m.fcell = dupload(faces->cells, (size_t)F * 2);
         {
+            // the stencil kernels' face list: face and neighbour per slot
+            std::vector<int2> cfn((size_t)N * maxf);
+            for (int c = 0; c < N; ++c)
+            {
+                const int nf = cells->n_faces[c];
+                for (int s = 0; s < maxf; ++s)
+                {
+                    const int f = cells->faces[(size_t)c * maxf + (s < nf ? s : 0)];
+                    const int c0 = faces->cells[2 * f], c1 = faces->cells[2 * f + 1];
+                    cfn[(size_t)c * maxf + s] = c0 == c ? make_int2(f, c1) : make_int2(f, -(c0 + 1));
+                }
+            }
+            m.cfn = dupload(cfn.data(), cfn.size());
+        }
+        {
             // face records; periodic shifts as indices into a table of the
             // distinct exact shift vectors (entry 0 = no shift)
             std::vector<double> a((size_t)F * dev::kFgA), b((size_t)F * dev::kFgB);
