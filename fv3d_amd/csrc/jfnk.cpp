@@ -183,16 +183,32 @@ void dilu_prepare(const RuntimeView &v, bool fp32)
             col[c] = q;
             ncol = std::max(ncol, q + 1);
         }
+        // positions: by colour, and inside a colour by the number of lower
+        // neighbours (the order inside a colour does not change the factors),
+        // so that the 64 positions of a tile mostly use the same sweep slots
+        std::vector<int> nlow(N, 0);
+        for (int c = 0; c < N; ++c)
+            for (int t = 0; t < maxf; ++t)
+            {
+                const int o = nb[(size_t)c * maxf + t];
+                if (o >= 0 && col[o] < col[c])
+                    ++nlow[c];
+            }
         std::vector<int> coff(ncol + 1, 0), ccell(N), pos(N);
         for (int c = 0; c < N; ++c)
             ++coff[col[c] + 1];
         for (int q = 0; q < ncol; ++q)
             coff[q + 1] += coff[q];
         {
-            std::vector<int> fill(coff.begin(), coff.end() - 1);
+            const int nk = maxf + 1;
+            std::vector<int> boff((size_t)ncol * nk + 1, 0);
+            for (int c = 0; c < N; ++c)
+                ++boff[(size_t)col[c] * nk + nlow[c] + 1];
+            for (size_t q = 0; q + 1 < boff.size(); ++q)
+                boff[q + 1] += boff[q];
             for (int c = 0; c < N; ++c)
             {
-                pos[c] = fill[col[c]]++;
+                pos[c] = boff[(size_t)col[c] * nk + nlow[c]]++;
                 ccell[pos[c]] = c;
             }
         }
@@ -215,10 +231,16 @@ void dilu_prepare(const RuntimeView &v, bool fp32)
                         to = u;
                 if (to < 0)
                     throw std::runtime_error("DILU: inconsistent face lists");
-                nbr[tiled(p, t, maxf)] = o;
                 back[(size_t)p * maxf + t] = pos[o] * maxf + to;
                 flags[p] |= col[o] < col[c] ? (1 << t) : (1 << (8 + t));
             }
+            // sweep slots: the lower neighbours in face order, then the
+            // higher ones
+            int t2 = 0;
+            for (int pass = 0; pass < 2; ++pass)
+                for (int t = 0; t < maxf; ++t)
+                    if ((flags[p] >> (8 * pass + t)) & 1)
+                        nbr[tiled(p, t2++, maxf)] = nb[(size_t)c * maxf + t];
         }
         g_dilu = DiluState();
         g_dilu.mesh_cf = m.cf;
@@ -244,15 +266,20 @@ void dilu_prepare(const RuntimeView &v, bool fp32)
         // read every slot)
         FVH_HIP_CHECK(hipMemset(g_dilu.off, 0, (size_t)N * maxf * 25 * sizeof(double)));
     }
-    const size_t NT = ((size_t)g_dilu.N + 63) / 64 * 64; // whole tiles
+    // whole tiles, and a last tile of zero blocks for the slots the sweeps
+    // skip
+    const size_t NT = ((size_t)g_dilu.N + 63) / 64 * 64;
+    const size_t E = (size_t)g_dilu.maxf * 25;
     if (fp32 && !g_dilu.off16)
     {
-        g_dilu.off16 = walloc<kry::bf16_t>(NT * g_dilu.maxf * 25);
+        g_dilu.off16 = walloc<kry::bf16_t>((NT + 64) * E);
+        FVH_HIP_CHECK(hipMemset(g_dilu.off16 + NT * E, 0, 64 * E * sizeof(kry::bf16_t)));
         g_dilu.dinv32 = walloc<float>(NT * 25);
     }
     if (!fp32 && !g_dilu.off64)
     {
-        g_dilu.off64 = walloc<double>(NT * g_dilu.maxf * 25);
+        g_dilu.off64 = walloc<double>((NT + 64) * E);
+        FVH_HIP_CHECK(hipMemset(g_dilu.off64 + NT * E, 0, 64 * E * sizeof(double)));
         g_dilu.dinv64 = walloc<double>(NT * 25);
     }
 }

@@ -737,16 +737,37 @@ __global__ __launch_bounds__(kBlock) void k_dilu_factor(DiluMesh d, int p0, int 
         dinv[(size_t)p * 25 + q] = X[q];
         dinvB[tiled(p, q, 25)] = (B)X[q];
     }
-    for (int q = 0; q < d.maxf * 25; ++q)
-        offB[tiled(p, q, d.maxf * 25)] = from_d<O>(off[(size_t)p * d.maxf * 25 + q]);
+    // sweep slots: the lower blocks in face order, then the higher ones,
+    // then zeros
+    const int E = d.maxf * 25;
+    const int hi = (d.flags[p] >> 8) & 0xff;
+    int t2 = 0;
+#pragma unroll 1
+    for (int pass = 0; pass < 2; ++pass)
+#pragma unroll 1
+        for (int t = 0; t < d.maxf; ++t)
+            if (((pass ? hi : lo) >> t) & 1)
+            {
+                for (int q = 0; q < 25; ++q)
+                    offB[tiled(p, t2 * 25 + q, E)] =
+                        from_d<O>(off[((size_t)p * d.maxf + t) * 25 + q]);
+                ++t2;
+            }
+#pragma unroll 1
+    for (; t2 < d.maxf; ++t2)
+        for (int q = 0; q < 25; ++q)
+            offB[tiled(p, t2 * 25 + q, E)] = from_d<O>(0.0);
 }
 
-// One colour of the sweeps (blocks and neighbour table in the tiled
-// layout).  Every face slot is loaded unconditionally (straight-line code, so
-// that the loads of all slots are in flight together): blocks of boundary
-// and unused slots are zero, a missing neighbour reads cell 0, and the
-// neighbour values of slots outside the sweep's set (the other direction
-// included) are replaced by zero, so those slots contribute exactly zero.
+// One colour of the sweeps (blocks and neighbour table in the tiled layout,
+// each position's lower blocks first, then its higher ones).  Every slot is
+// loaded unconditionally (straight-line code, so that the loads of all slots
+// are in flight together); a slot outside the sweep's set reads the zero
+// blocks of the tables' last tile and the values of cell 0, replaced by
+// zero, so it contributes exactly zero and, when no position of a wavefront
+// uses it, its blocks cost no HBM traffic: positions are ordered by their
+// number of lower neighbours inside a colour.  The neighbour table is read
+// for every slot, so that its load does not wait for the flags.
 //   FORWARD: z_i = D_i^-1 (v_i - sum_{k lower} A_ik z_k)
 //   backward: z_i -= D_i^-1 sum_{k higher} A_ik z_k
 // NOFF: the forward sweep of the lowest colour, whose cells have no lower
@@ -761,18 +782,21 @@ __global__ __launch_bounds__(kBlock) void k_dilu_sweep(DiluMesh d, int p0, int p
     const int p = p0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= p1)
         return;
-    const int mask = FORWARD ? (d.flags[p] & 0xff) : ((d.flags[p] >> 8) & 0xff);
-    if (!FORWARD && !mask)
+    const int fl = d.flags[p];
+    const int nlo = __popc(fl & 0xff), nhi = __popc((fl >> 8) & 0xff);
+    if (!FORWARD && !nhi)
         return;
+    const int t0 = FORWARD ? 0 : nlo, t1 = FORWARD ? nlo : nlo + nhi;
     const int i = d.ccell[p];
     const int E = d.maxf * 25;
+    const int ptail = (d.N + 63) & ~63; // first position of the last tile
     double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int t = 0; t < (NOFF ? 0 : MAXF); ++t)
     {
-        const bool on = (mask >> t) & 1;
+        const bool on = t >= t0 && t < t1;
         const int kn = d.nbr[tiled(p, t, d.maxf)];
-        const int k = kn < 0 ? 0 : kn;
+        const int k = on && kn >= 0 ? kn : 0;
         double y[5];
 #pragma unroll
         for (int j = 0; j < 5; ++j)
@@ -780,13 +804,14 @@ __global__ __launch_bounds__(kBlock) void k_dilu_sweep(DiluMesh d, int p0, int p
             const double zk = (double)z[(size_t)k * 5 + j];
             y[j] = on ? zk : 0.0;
         }
+        const O *blk = off + tiled(on ? p : ptail, t * 25, E);
 #pragma unroll
         for (int a = 0; a < 5; ++a)
         {
             double acc = 0.0;
 #pragma unroll
             for (int b = 0; b < 5; ++b)
-                acc += to_d(off[tiled(p, t * 25 + a * 5 + b, E)]) * y[b];
+                acc += to_d(blk[(a * 5 + b) * 64]) * y[b];
             r[a] += acc;
         }
     }
