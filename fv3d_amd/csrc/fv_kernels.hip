@@ -509,8 +509,12 @@ __global__ __launch_bounds__(kBlock, (FS && VISC && std::is_same<Real, double>::
         ViscGradT<Real> gL, gR;
         if (VISC)
         {
+            // the neighbour's row is loaded unconditionally (the cell's own
+            // row stands in on boundary faces), so that its load is issued
+            // with the face's other loads
             ViscGradT<Real> go, gn;
             load_vg(sp.vg, c, go);
+            load_vg(sp.vg, ghost ? c : nb, gn);
             if (ghost)
             {
                 const int type = m.region_type[m.bnd_region[nb - N]];
@@ -518,10 +522,6 @@ __global__ __launch_bounds__(kBlock, (FS && VISC && std::is_same<Real, double>::
                 for (int q = 0; q < 4; ++q)
                     ghost_grad(type, fr, go.v[q], go.v[4 + q], go.v[8 + q],
                                gn.v[q], gn.v[4 + q], gn.v[8 + q]);
-            }
-            else
-            {
-                load_vg(sp.vg, nb, gn);
             }
 #pragma unroll
             for (int q = 0; q < 12; ++q)
