@@ -50,8 +50,9 @@ B_CELL_STAGE_FO = 388
 # HBM bytes per launch of the residual (K2) and gradient (K1) kernels on the
 # default 96^3 x 6 mesh, from rocprofv3 FETCH_SIZE and WRITE_SIZE per
 # dispatch scaled by a 1 GiB device copy's counters (read = 2.0 x FETCH_SIZE,
-# written = 1.0 x WRITE_SIZE): profiles/explicit_tgv96_rk3_kernels.md
-TRAFFIC_N96 = {"residual": 3.656e9 + 0.498e9, "gradient": 5.137e9 + 1.932e9}
+# which the 128-B read-request counts confirm; written = 1.0 x WRITE_SIZE):
+# profiles/explicit_tgv96_rk3_kernels.md
+TRAFFIC_N96 = {"residual": 3.631e9 + 0.497e9, "gradient": 1.663e9 + 2.096e9}
 
 
 def main():
