@@ -794,8 +794,8 @@ __global__ __launch_bounds__(kBlock) void k_dilu_sweep(DiluMesh d, int p0, int p
 #pragma unroll
     for (int t = 0; t < (NOFF ? 0 : MAXF); ++t)
     {
-        const bool on = t >= t0 && t < t1;
-        const int kn = d.nbr[tiled(p, t, d.maxf)];
+        const bool on = t >= t0 && t < t1; // t1 <= d.maxf
+        const int kn = d.nbr[tiled(p, t < d.maxf ? t : 0, d.maxf)];
         const int k = on && kn >= 0 ? kn : 0;
         double y[5];
 #pragma unroll
@@ -804,7 +804,7 @@ __global__ __launch_bounds__(kBlock) void k_dilu_sweep(DiluMesh d, int p0, int p
             const double zk = (double)z[(size_t)k * 5 + j];
             y[j] = on ? zk : 0.0;
         }
-        const O *blk = off + tiled(on ? p : ptail, t * 25, E);
+        const O *blk = off + tiled(on ? p : ptail, (t < d.maxf ? t : 0) * 25, E);
 #pragma unroll
         for (int a = 0; a < 5; ++a)
         {

@@ -228,7 +228,9 @@ class OracleJfnk:
         self.dx = np.asarray(mesh["cell_dx"], dtype=np.float64).reshape(self.N, 3)
         self.kappa = physics.get("kappa", 1.4)
         self.full = fvref.from_case(mesh, physics, regions, numerics)
-        self.first = fvref.from_case(mesh, physics, regions,
+        # the preconditioners' residual R1: first order and inviscid, whatever
+        # the case's viscosity
+        self.first = fvref.from_case(mesh, dict(physics, mu_mix=0.0), regions,
                                      dict(numerics, reconstruction="First-Order",
                                           limiter="None"))
         self.colour = colouring(mesh)

@@ -1,6 +1,7 @@
 """Writers of small Gmsh 4.1 ASCII meshes for the reader tests: structured
 boxes of hexahedra or Kuhn tetrahedra in the vertex and cell order of
-fvh_mesh_box (tile 0), and a mixed mesh of hexahedra, prisms and pyramids.
+fvh_mesh_box (tile 0), boxes of prisms, and a mixed mesh of hexahedra, prisms
+and pyramids.
 Boundary elements are the unmatched local faces, one physical surface per box
 side ("xmin", "xmax", "ymin", "ymax", "zmin", "zmax"), written in the order of
 the faces' first appearance; the volume's physical name is "flow"."""
@@ -41,6 +42,20 @@ def box_cells(n, length=(1.0, 1.0, 1.0), tets=False):
                     cells += [(TETRAEDER, [h[q] for q in t]) for t in KUHN]
                 else:
                     cells.append((HEXAEDER, h))
+    return X, cells
+
+
+def prism_cells(n, length=(1.0, 1.0, 1.0)):
+    """Every hexahedron split into two prisms along its (0, 2) diagonal (cells
+    of 5 faces only)."""
+    X, vid = _lattice(n, length)
+    cells = []
+    for k in range(n[2]):
+        for j in range(n[1]):
+            for i in range(n[0]):
+                h = _hex_nodes(vid, i, j, k)
+                cells.append((PRISM, [h[0], h[1], h[2], h[4], h[5], h[6]]))
+                cells.append((PRISM, [h[0], h[2], h[3], h[4], h[6], h[7]]))
     return X, cells
 
 
