@@ -111,18 +111,26 @@ def main():
         ach_stage = B_STAGE * N / k_stage_s / 1e9
         ach_grad = B_GRAD * N / k_grad_s / 1e9
         step_gbs = B_CELL_STAGE * N * stages * args.steps / sec / 1e9
-        roofline = {"bound": "hbm",
-                    "kernel": "fv_residual_group_kernel<AUSM,face states,viscous,4>",
-                    "achieved": round(ach_stage, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                    "frac": round(ach_stage / PEAK_HBM_GBS, 4),
-                    "traffic": TRAFFIC_N96["residual"] if args.n == 96 else None,
-                    "bytes_per_cell": B_STAGE, "avg_us": round(k_us[1], 2)}
-        extra = {"gradient_kernel": {"achieved": round(ach_grad, 1), "bytes_per_cell": B_GRAD,
-                                     "traffic": TRAFFIC_N96["gradient"] if args.n == 96 else None,
-                                     "avg_us": round(k_us[0], 2),
-                                     "frac": round(ach_grad / PEAK_HBM_GBS, 4)},
-                 "step": {"achieved": round(step_gbs, 1), "bytes_per_cell_stage": B_CELL_STAGE,
-                          "frac": round(step_gbs / PEAK_HBM_GBS, 4)}}
+        kernels = {
+            "residual_kernel": {
+                "bound": "hbm", "kernel": "fv_residual_group_kernel<AUSM,face states,viscous,4>",
+                "achieved": round(ach_stage, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(ach_stage / PEAK_HBM_GBS, 4),
+                "traffic": TRAFFIC_N96["residual"] if args.n == 96 else None,
+                "bytes_per_cell": B_STAGE, "avg_us": round(k_us[1], 2)},
+            "gradient_kernel": {
+                "bound": "hbm", "kernel": "fv_gradient_kernel<4,double>",
+                "achieved": round(ach_grad, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(ach_grad / PEAK_HBM_GBS, 4),
+                "traffic": TRAFFIC_N96["gradient"] if args.n == 96 else None,
+                "bytes_per_cell": B_GRAD, "avg_us": round(k_us[0], 2)}}
+        # the dominant kernel (the longer launch) is the roofline; the other
+        # one and the whole step are in roofline_detail
+        dominant = max(kernels, key=lambda k: kernels[k]["avg_us"])
+        roofline = kernels.pop(dominant)
+        extra = dict(kernels)
+        extra["step"] = {"achieved": round(step_gbs, 1), "bytes_per_cell_stage": B_CELL_STAGE,
+                         "frac": round(step_gbs / PEAK_HBM_GBS, 4)}
         print("explicit: %.3f ms/step, %.3e cell-stage/s" % (ms_per_step, value),
               file=sys.stderr, flush=True)
         case.close()
