@@ -1,7 +1,7 @@
 """GPU parity of the residual and explicit time-stepping path against the CPU
 oracle, through the C-ABI (libfv3d_hip.so via ctypes).
 
-Bar: the north star's 1e-12 relative per component, |a-b| <= 1e-12 *
+Bar: 1e-12 relative per component, |a-b| <= 1e-12 *
 max(|b|, scale_j) with scale_j = max_i |b_ij|, for dU/dt and for the state
 after N steps.  With -ffp-contract=off on both sides most values are
 bitwise equal; the count of differing bits is printed.
