@@ -40,6 +40,10 @@ struct DevMesh
     // face's first cell, else -(neighbour + 1); slots past the cell's face
     // count repeat slot 0 [N*maxf]
     const int2 *__restrict__ cfn;
+    // per cell face slot: the distance vector from the cell centre to the
+    // face centre (x_f - x_c; on side 1 to the periodic image of the cell,
+    // x_f - (x_c + shift)), as the reconstruction uses it [N*maxf][3]
+    const double *__restrict__ crv;
     const double *__restrict__ fgA; // [F*kFgA]
     const double *__restrict__ fgB; // [F*kFgB]
     const int *__restrict__ fsh;    // [F]

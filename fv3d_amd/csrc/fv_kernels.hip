@@ -153,7 +153,7 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
     const int2 *cfn = m.cfn + (size_t)c * m.maxf;
     int fid[MAXF];
     bool own[MAXF];
-    int nbr[MAXF], shi[MAXF];
+    int nbr[MAXF];
 #pragma unroll
     for (int s = 0; s < MAXF; ++s)
     {
@@ -162,9 +162,6 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
         own[s] = e.y >= 0;
         nbr[s] = own[s] ? e.y : -(e.y + 1);
     }
-#pragma unroll
-    for (int s = 0; s < MAXF; ++s)
-        shi[s] = (!f32 && linear) ? m.fsh[fid[s]] : 0;
 
 #pragma unroll
     for (int s0 = 0; s0 < MAXF; s0 += FB)
@@ -303,21 +300,18 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
         return;
     // this cell's reconstructed state on each of its faces; the distance
     // vectors of mesh.c:44-56 (dist_cell_1 = x_f - x_c0, dist_cell_2 =
-    // x_f - x_c1 through the periodic image) from the face and cell centres
-    const double xc0 = m.cx[(size_t)c * 3], xc1 = m.cx[(size_t)c * 3 + 1],
-                 xc2 = m.cx[(size_t)c * 3 + 2];
-    // distance vectors of all faces first (one round of loads), then the
-    // limited states
+    // x_f - x_c1 through the periodic image), all faces' first (one round of
+    // loads: the cell's own row of the slot table in FP64, the face records
+    // in FP32), then the limited states
     Real rv[MAXF][3];
     if (linear)
     {
 #pragma unroll
         for (int s = 0; s < MAXF; ++s)
         {
-            const int f = fid[s];
             if (f32)
             {
-                const float4 *q = reinterpret_cast<const float4 *>(m.fgB32 + (size_t)f * 12);
+                const float4 *q = reinterpret_cast<const float4 *>(m.fgB32 + (size_t)fid[s] * 12);
                 const float4 q1 = q[1], q2 = q[2];
                 rv[s][0] = own[s] ? q1.y : q2.x;
                 rv[s][1] = own[s] ? q1.z : q2.y;
@@ -325,19 +319,10 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
             }
             else
             {
-                // side 1 through the periodic image (shift entry 0 is zero);
-                // side 0 keeps the unshifted difference.  The shift is loaded
-                // for both sides so that the loads are not conditional.
-                const double *g = m.fgB + (size_t)f * kFgB;
-                const double xf0 = g[5], xf1 = g[6], xf2 = g[7];
-                const double *sh = m.shtab + (size_t)shi[s] * 3;
-                const double s0 = sh[0], s1 = sh[1], s2 = sh[2];
-                const double c0 = own[s] ? xc0 : xc0 + s0;
-                const double c1 = own[s] ? xc1 : xc1 + s1;
-                const double c2 = own[s] ? xc2 : xc2 + s2;
-                rv[s][0] = Real(xf0 - c0);
-                rv[s][1] = Real(xf1 - c1);
-                rv[s][2] = Real(xf2 - c2);
+                const double *r = m.crv + ((size_t)c * m.maxf + s) * 3;
+                rv[s][0] = Real(r[0]);
+                rv[s][1] = Real(r[1]);
+                rv[s][2] = Real(r[2]);
             }
         }
     }

@@ -1329,6 +1329,27 @@ void init_finite_volume(void)
             m.shtab = dupload(shtab.data(), shtab.size());
         }
         m.cx = dupload(cells->x, (size_t)N * 3);
+        {
+            // the gradient kernel's distance vectors per face slot, the
+            // subtractions of mesh.c:44-56 (side 1 through the periodic
+            // image of the cell)
+            std::vector<double> crv((size_t)N * maxf * 3);
+            for (int c = 0; c < N; ++c)
+            {
+                const int nf = cells->n_faces[c];
+                for (int s = 0; s < maxf; ++s)
+                {
+                    const int f = cells->faces[(size_t)c * maxf + (s < nf ? s : 0)];
+                    const bool own0 = faces->cells[2 * f] == c;
+                    const double *sh = &g_face_shift[(size_t)3 * f];
+                    double *r = &crv[((size_t)c * maxf + s) * 3];
+                    for (int d = 0; d < 3; ++d)
+                        r[d] = own0 ? faces->x[3 * f + d] - cells->x[3 * c + d]
+                                    : faces->x[3 * f + d] - (cells->x[3 * c + d] + sh[d]);
+                }
+            }
+            m.crv = dupload(crv.data(), crv.size());
+        }
         m.vol = dupload(cells->volume, N);
         {
             std::vector<double> sv(N);
