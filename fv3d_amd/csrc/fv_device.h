@@ -13,8 +13,8 @@
 //   face records      : fs[F][16] (reconstructed state of variables 0..7
 //                       on side 0 = cells[0], then side 1 = cells[1] or the
 //                       ghost), fgA[F][10] (n, t1, t2, area: flux kernel),
-//                       fgB[F][8] (n, area, lambda, face centre x_f:
-//                       gradient kernel), fsh[F] (index into shtab[][3],
+//                       fgB[F][6] (n, area, lambda, pad: gradient
+//                       kernel), fsh[F] (index into shtab[][3],
 //                       the exact periodic shifts; 0 = none), fcell[F][2]
 //                       (cells[1] >= N is the ghost row N + boundary)
 //   boundary records  : gphi[B][10] (ghost rows), bnd_region[B],
@@ -28,7 +28,7 @@ namespace fvh {
 namespace dev {
 
 constexpr int kFgA = 10; // doubles per flux-kernel face record
-constexpr int kFgB = 8;  // doubles per gradient-kernel face record
+constexpr int kFgB = 6;  // doubles per gradient-kernel face record
 
 struct DevMesh
 {

@@ -1305,7 +1305,6 @@ void init_finite_volume(void)
                     ra[3 + d] = faces->t1[3 * f + d];
                     ra[6 + d] = faces->t2[3 * f + d];
                     rb[d] = faces->n[3 * f + d];
-                    rb[5 + d] = faces->x[3 * f + d];
                 }
                 ra[9] = faces->area[f];
                 rb[3] = faces->area[f];
