@@ -57,12 +57,12 @@ struct DevMesh
     const int *__restrict__ region_type;
     const double *__restrict__ region_phi;
     // FP32 face records of the FP32 residual (built on first use):
-    // fgA32 [F][12] = n, t1, t2, area, 2 pad; fgB32 [F][12] = n, area,
-    // lambda, r0 = x_f - x_c0, r1 = x_f - x_c1 (through the periodic image),
-    // 1 pad; each value the FP32 rounding of the FP64 one (r from the FP64
-    // difference)
+    // fgA32 [F][12] = n, t1, t2, area, 2 pad; fgB32 [F][8] = n, area,
+    // lambda, 3 pad; crv32 [N*maxf][4] = the slot distance vectors, 1 pad;
+    // each value the FP32 rounding of the FP64 one
     const float *__restrict__ fgA32;
     const float *__restrict__ fgB32;
+    const float *__restrict__ crv32;
 };
 
 struct PhysDev

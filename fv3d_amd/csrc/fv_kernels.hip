@@ -189,7 +189,7 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
             }
             if (f32)
             {
-                const float4 *q = reinterpret_cast<const float4 *>(m.fgB32 + (size_t)fid[s] * 12);
+                const float4 *q = reinterpret_cast<const float4 *>(m.fgB32 + (size_t)fid[s] * 8);
                 const float4 q0 = q[0], q1 = q[1];
                 gn[i][0] = q0.x, gn[i][1] = q0.y, gn[i][2] = q0.z, gn[i][3] = q0.w;
                 gn[i][4] = q1.x;
@@ -301,8 +301,7 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
     // this cell's reconstructed state on each of its faces; the distance
     // vectors of mesh.c:44-56 (dist_cell_1 = x_f - x_c0, dist_cell_2 =
     // x_f - x_c1 through the periodic image), all faces' first (one round of
-    // loads: the cell's own row of the slot table in FP64, the face records
-    // in FP32), then the limited states
+    // loads: the cell's own row of the slot table), then the limited states
     Real rv[MAXF][3];
     if (linear)
     {
@@ -311,11 +310,10 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
         {
             if (f32)
             {
-                const float4 *q = reinterpret_cast<const float4 *>(m.fgB32 + (size_t)fid[s] * 12);
-                const float4 q1 = q[1], q2 = q[2];
-                rv[s][0] = own[s] ? q1.y : q2.x;
-                rv[s][1] = own[s] ? q1.z : q2.y;
-                rv[s][2] = own[s] ? q1.w : q2.z;
+                const float4 q = reinterpret_cast<const float4 *>(m.crv32)[(size_t)c * m.maxf + s];
+                rv[s][0] = q.x;
+                rv[s][1] = q.y;
+                rv[s][2] = q.z;
             }
             else
             {
@@ -890,7 +888,7 @@ void launch_gradient(hipStream_t s, const DevMesh &m, const PhysDev &P,
     {
         if (out.grad || out.mnmx)
             throw std::runtime_error("launch_gradient: the test outputs are FP64 only");
-        if (!out.U32 || !m.fgB32)
+        if (!out.U32 || !m.fgB32 || !m.crv32)
             throw std::runtime_error("launch_gradient: FP32 needs the FP32 state and face records");
         if (m.maxf <= 4)
             hipLaunchKernelGGL((fv_gradient_kernel<4, float, false>), dim3(grid), dim3(kBlock),

@@ -158,6 +158,7 @@ struct Device
     double *fs = nullptr;   // [16*F] face states
     float *fs32 = nullptr;  // [16*F] face states of the FP32 residual
     float *fgA32 = nullptr, *fgB32 = nullptr; // FP32 face records (FP32 residual)
+    float *crv32 = nullptr;                   // FP32 distance vectors by slot
     double *grad = nullptr; // [30*N] all gradients (tests only)
     double *mnmx = nullptr; // [20*N] min/max (tests only)
     double *partials = nullptr;
@@ -279,6 +280,33 @@ void require_ready()
             "(calc_ns_flux, ns_update, ns_update_gradients)");
 }
 
+// The reconstruction's distance vectors of the mesh by cell face slot
+// (x_f - x_c, on side 1 through the periodic image of the cell: the
+// subtractions of mesh.c:44-56), slots past a cell's face count repeating
+// slot 0; r[(c * maxf + s) * 3 + d].
+std::vector<double> slot_distances(int maxf)
+{
+    const int N = g_dev.m.N;
+    const Cells_t *cells = solver_mesh->cells;
+    const Faces_t *faces = solver_mesh->faces;
+    std::vector<double> crv((size_t)N * maxf * 3);
+    for (int c = 0; c < N; ++c)
+    {
+        const int nf = cells->n_faces[c];
+        for (int s = 0; s < maxf; ++s)
+        {
+            const int f = cells->faces[(size_t)c * maxf + (s < nf ? s : 0)];
+            const bool own0 = faces->cells[2 * f] == c;
+            const double *sh = &g_face_shift[(size_t)3 * f];
+            double *r = &crv[((size_t)c * maxf + s) * 3];
+            for (int d = 0; d < 3; ++d)
+                r[d] = own0 ? faces->x[3 * f + d] - cells->x[3 * c + d]
+                            : faces->x[3 * f + d] - (cells->x[3 * c + d] + sh[d]);
+        }
+    }
+    return crv;
+}
+
 // FP32 face records of the FP32 residual (layouts in dev::DevMesh), built
 // on first use from the host mesh: each value the FP32 rounding of the FP64
 // one, the cell-to-face vectors from the FP64 differences (through the
@@ -292,11 +320,11 @@ void ensure_fp32_geometry()
     const Faces_t *faces = mm->faces;
     const Cells_t *cells = mm->cells;
     const int F = g_dev.m.F, N = g_dev.m.N;
-    std::vector<float> a((size_t)F * 12, 0.0f), b((size_t)F * 12, 0.0f);
+    std::vector<float> a((size_t)F * 12, 0.0f), b((size_t)F * 8, 0.0f);
     for (int f = 0; f < F; ++f)
     {
         float *ra = &a[(size_t)f * 12];
-        float *rb = &b[(size_t)f * 12];
+        float *rb = &b[(size_t)f * 8];
         for (int d = 0; d < 3; ++d)
         {
             ra[d] = (float)faces->n[3 * f + d];
@@ -307,19 +335,21 @@ void ensure_fp32_geometry()
         ra[9] = (float)faces->area[f];
         rb[3] = (float)faces->area[f];
         rb[4] = (float)faces->lambda[f];
-        const int c0 = faces->cells[2 * f], c1 = faces->cells[2 * f + 1];
-        const double *xf = &faces->x[3 * f];
-        const double *sh = &g_face_shift[(size_t)3 * f];
-        for (int d = 0; d < 3; ++d)
-        {
-            rb[5 + d] = (float)(xf[d] - cells->x[3 * c0 + d]);
-            rb[8 + d] = c1 < N ? (float)(xf[d] - (cells->x[3 * c1 + d] + sh[d])) : 0.0f;
-        }
     }
     g_dev.fgA32 = dupload(a.data(), a.size());
     g_dev.fgB32 = dupload(b.data(), b.size());
     g_dev.m.fgA32 = g_dev.fgA32;
     g_dev.m.fgB32 = g_dev.fgB32;
+    // the distance vectors by slot, FP32 roundings of the FP64 differences,
+    // 4 floats per slot
+    const int maxf = g_dev.m.maxf;
+    const std::vector<double> crv = slot_distances(maxf);
+    std::vector<float> r32((size_t)N * maxf * 4, 0.0f);
+    for (size_t q = 0; q < (size_t)N * maxf; ++q)
+        for (int d = 0; d < 3; ++d)
+            r32[q * 4 + d] = (float)crv[q * 3 + d];
+    g_dev.crv32 = dupload(r32.data(), r32.size());
+    g_dev.m.crv32 = g_dev.crv32;
 }
 
 void ensure_gradient_buffers()
@@ -1329,24 +1359,8 @@ void init_finite_volume(void)
         }
         m.cx = dupload(cells->x, (size_t)N * 3);
         {
-            // the gradient kernel's distance vectors per face slot, the
-            // subtractions of mesh.c:44-56 (side 1 through the periodic
-            // image of the cell)
-            std::vector<double> crv((size_t)N * maxf * 3);
-            for (int c = 0; c < N; ++c)
-            {
-                const int nf = cells->n_faces[c];
-                for (int s = 0; s < maxf; ++s)
-                {
-                    const int f = cells->faces[(size_t)c * maxf + (s < nf ? s : 0)];
-                    const bool own0 = faces->cells[2 * f] == c;
-                    const double *sh = &g_face_shift[(size_t)3 * f];
-                    double *r = &crv[((size_t)c * maxf + s) * 3];
-                    for (int d = 0; d < 3; ++d)
-                        r[d] = own0 ? faces->x[3 * f + d] - cells->x[3 * c + d]
-                                    : faces->x[3 * f + d] - (cells->x[3 * c + d] + sh[d]);
-                }
-            }
+            // the gradient kernel's distance vectors per face slot
+            const std::vector<double> crv = slot_distances(maxf);
             m.crv = dupload(crv.data(), crv.size());
         }
         m.vol = dupload(cells->volume, N);
