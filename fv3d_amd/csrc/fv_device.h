@@ -40,6 +40,9 @@ struct DevMesh
     // face's first cell, else -(neighbour + 1); slots past the cell's face
     // count repeat slot 0 [N*maxf]
     const int2 *__restrict__ cfn;
+    // per cell face slot: the slot that the same face has in the neighbour's
+    // list (0 on boundary faces and past the cell's face count) [N*maxf]
+    const unsigned char *__restrict__ nsl;
     // per cell face slot: the distance vector from the cell centre to the
     // face centre (x_f - x_c; on side 1 to the periodic image of the cell,
     // x_f - (x_c + shift)), as the reconstruction uses it [N*maxf][3]

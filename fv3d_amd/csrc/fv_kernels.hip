@@ -90,14 +90,15 @@ __global__ __launch_bounds__(kBlock) void fv_ghost_kernel(
 #pragma unroll
     for (int j = 0; j < 10; ++j)
         gphi[(size_t)b * 10 + j] = g[j];
+    const size_t row = (size_t)m.N * m.maxf + b;
     if (fs)
 #pragma unroll
         for (int j = 0; j < 8; ++j)
-            fs[(size_t)f * 16 + 8 + j] = g[j];
+            fs[row * 8 + j] = g[j];
     if (fs32)
 #pragma unroll
         for (int j = 0; j < 8; ++j)
-            fs32[(size_t)f * 16 + 8 + j] = (float)g[j];
+            fs32[row * 8 + j] = (float)g[j];
 }
 
 // ---------------------------------------------------------------------------
@@ -298,7 +299,8 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
 
     if (std::is_same<Real, float>::value ? !out.fs32 : !out.fs)
         return;
-    // this cell's reconstructed state on each of its faces; the distance
+    // this cell's reconstructed state on each of its faces, one 8-value row
+    // per face slot (a cell's rows are contiguous); the distance
     // vectors of mesh.c:44-56 (dist_cell_1 = x_f - x_c0, dist_cell_2 =
     // x_f - x_c1 through the periodic image), all faces' first (one round of
     // loads: the cell's own row of the slot table), then the limited states
@@ -329,8 +331,6 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
     {
         if (s >= nf)
             break;
-        const int f = fid[s];
-        const bool own0 = own[s];
         Real st[8];
         if (linear)
         {
@@ -346,13 +346,13 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
         }
         if constexpr (std::is_same<Real, float>::value)
         {
-            float4 *d = reinterpret_cast<float4 *>(out.fs32 + (size_t)f * 16 + (own0 ? 0 : 8));
+            float4 *d = reinterpret_cast<float4 *>(out.fs32 + ((size_t)c * m.maxf + s) * 8);
             d[0] = make_float4(st[0], st[1], st[2], st[3]);
             d[1] = make_float4(st[4], st[5], st[6], st[7]);
         }
         else
         {
-            double2 *d = reinterpret_cast<double2 *>(out.fs + (size_t)f * 16 + (own0 ? 0 : 8));
+            double2 *d = reinterpret_cast<double2 *>(out.fs + ((size_t)c * m.maxf + s) * 8);
             d[0] = make_double2(double(st[0]), double(st[1]));
             d[1] = make_double2(double(st[2]), double(st[3]));
             d[2] = make_double2(double(st[4]), double(st[5]));
@@ -432,24 +432,32 @@ __global__ __launch_bounds__(kBlock, (FS && VISC && std::is_same<Real, double>::
         Real L[10], R[10];
         if (FS)
         {
+            // the face's two states: this cell's row of the slot, and the
+            // neighbour's row of the same face (or the ghost row)
+            const size_t ro = (size_t)c * m.maxf + s;
+            const size_t rn = ghost ? (size_t)N * m.maxf + (nb - N)
+                                    : (size_t)nb * m.maxf + m.nsl[ro];
+            const size_t rl = own0 ? ro : rn, rr = own0 ? rn : ro;
             if constexpr (std::is_same<Real, float>::value)
             {
-                const float4 *p = reinterpret_cast<const float4 *>(sp.fs32 + (size_t)f * 16);
+                const float4 *pl = reinterpret_cast<const float4 *>(sp.fs32 + rl * 8);
+                const float4 *pr = reinterpret_cast<const float4 *>(sp.fs32 + rr * 8);
 #pragma unroll
                 for (int q = 0; q < 2; ++q)
                 {
-                    const float4 a = p[q], b = p[2 + q];
+                    const float4 a = pl[q], b = pr[q];
                     L[4 * q] = a.x, L[4 * q + 1] = a.y, L[4 * q + 2] = a.z, L[4 * q + 3] = a.w;
                     R[4 * q] = b.x, R[4 * q + 1] = b.y, R[4 * q + 2] = b.z, R[4 * q + 3] = b.w;
                 }
             }
             else
             {
-                const double2 *p = reinterpret_cast<const double2 *>(sp.fs + (size_t)f * 16);
+                const double2 *pl = reinterpret_cast<const double2 *>(sp.fs + rl * 8);
+                const double2 *pr = reinterpret_cast<const double2 *>(sp.fs + rr * 8);
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
                 {
-                    const double2 a = p[q], b = p[4 + q];
+                    const double2 a = pl[q], b = pr[q];
                     L[2 * q] = Real(a.x);
                     L[2 * q + 1] = Real(a.y);
                     R[2 * q] = Real(b.x);

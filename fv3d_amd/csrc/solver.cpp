@@ -155,8 +155,8 @@ struct Device
     double *R = nullptr;
     double *gphi = nullptr; // [10*B] ghost rows
     double *vg = nullptr;   // [12*N] viscous gradient subset
-    double *fs = nullptr;   // [16*F] face states
-    float *fs32 = nullptr;  // [16*F] face states of the FP32 residual
+    double *fs = nullptr;   // [N*maxf+B][8] face states by cell slot, ghosts
+    float *fs32 = nullptr;  // the same rows for the FP32 residual
     float *fgA32 = nullptr, *fgB32 = nullptr; // FP32 face records (FP32 residual)
     float *crv32 = nullptr;                   // FP32 distance vectors by slot
     double *grad = nullptr; // [30*N] all gradients (tests only)
@@ -352,13 +352,20 @@ void ensure_fp32_geometry()
     g_dev.m.crv32 = g_dev.crv32;
 }
 
+// face-state rows: one per cell face slot (the cell's reconstructed state on
+// that face), then one per boundary face (its ghost state)
+size_t face_state_rows()
+{
+    return (size_t)g_dev.m.N * g_dev.m.maxf + (size_t)g_dev.m.B;
+}
+
 void ensure_gradient_buffers()
 {
     size_t N = (size_t)g_dev.m.N, F = (size_t)g_dev.m.F;
     if (is_viscous() && !g_dev.vg)
         g_dev.vg = dmalloc<double>(12 * N);
     if (is_linear() && !g_dev.fs)
-        g_dev.fs = dmalloc<double>(16 * F);
+        g_dev.fs = dmalloc<double>(8 * face_state_rows());
 }
 
 dev::GradOut grad_out()
@@ -410,7 +417,7 @@ void enqueue_residual(const double *U_in, double *U_out, double *R_out,
     // may take its state in FP32
     const bool fs32 = fp32 && is_linear();
     if (fs32 && !g_dev.fs32)
-        g_dev.fs32 = dmalloc<float>(16 * (size_t)g_dev.m.F);
+        g_dev.fs32 = dmalloc<float>(8 * face_state_rows());
     if (fp32)
         ensure_fp32_geometry();
     else
@@ -1318,6 +1325,30 @@ void init_finite_volume(void)
                 }
             }
             m.cfn = dupload(cfn.data(), cfn.size());
+            // the neighbour's slot of each slot's face (a cell that is its
+            // own periodic neighbour takes the face's other slot)
+            std::vector<unsigned char> nsl((size_t)N * maxf, 0);
+            for (int c = 0; c < N; ++c)
+            {
+                const int nf = cells->n_faces[c];
+                for (int s = 0; s < nf; ++s)
+                {
+                    const int f = cells->faces[(size_t)c * maxf + s];
+                    const int c0 = faces->cells[2 * f], c1 = faces->cells[2 * f + 1];
+                    const int nb = c0 == c ? c1 : c0;
+                    if (nb < 0 || nb >= N)
+                        continue;
+                    int t = 0;
+                    while (t < cells->n_faces[nb] &&
+                           (cells->faces[(size_t)nb * maxf + t] != f || (nb == c && t == s)))
+                        ++t;
+                    if (t == cells->n_faces[nb])
+                        fatal("face " + std::to_string(f) + " is missing from the face list of cell " +
+                              std::to_string(nb));
+                    nsl[(size_t)c * maxf + s] = (unsigned char)t;
+                }
+            }
+            m.nsl = dupload(nsl.data(), nsl.size());
         }
         {
             // face records; periodic shifts as indices into a table of the
