@@ -52,7 +52,7 @@ B_CELL_STAGE_FO = 388
 # dispatch scaled by a 1 GiB device copy's counters (read = 2.0 x FETCH_SIZE,
 # which the 128-B read-request counts confirm; written = 1.0 x WRITE_SIZE):
 # profiles/explicit_tgv96_rk3_kernels.md
-TRAFFIC_N96 = {"residual": 3.631e9 + 0.497e9, "gradient": 1.627e9 + 2.092e9}
+TRAFFIC_N96 = {"residual": 3.711e9 + 0.497e9, "gradient": 1.623e9 + 1.922e9}
 
 
 def main():
