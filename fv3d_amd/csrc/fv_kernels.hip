@@ -35,11 +35,12 @@
 // Minimum resident waves per SIMD requested from the register allocator for
 // the two stencil kernels (occupancy against spilling).  The gradient kernel
 // keeps a batch of neighbour rows and face records in registers (about 247
-// VGPRs in FP64 with batches of 4 faces): 2 waves per SIMD without spills.
+// VGPRs in FP64 with batches of 4 faces) without spills; with the face states
+// stored by cell slot, a minimum of 1 wave per SIMD ran 2% faster than 2.
 // Batches of 4 at 3 waves spill and ran at half the speed; batches of 1 at 3
 // waves ran at the same speed with 44% more HBM reads.
 #ifndef FVH_K1_MINW
-#define FVH_K1_MINW 2
+#define FVH_K1_MINW 1
 #endif
 // Faces whose neighbour rows and records the gradient kernel loads together.
 #ifndef FVH_K1_FB
