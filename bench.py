@@ -17,9 +17,14 @@ Beside it, in the same JSON line:
 - `batched`: 64 periodic cases of log-uniform size (seeded), one device
   state, graph-replayed steps: aggregate cell updates per second and the
   per-case step time (each case alone) p50/p99;
-- `roofline` of the dominant kernel (algorithmic bytes per launch over its
-  HIP-event-timed average duration) and `cpu_baseline` (the oracle, one
-  core, on a bounded sample of the same workload).
+- `roofline` of the explicit stage (the gradient and residual kernels that
+  make one RK stage): 1,276 algorithmic bytes per cell-stage times the cells
+  over the two kernels' HIP-event-timed average durations, with the measured
+  HBM bytes of the two launches when a counter pass of the current kernel
+  sources is recorded (profiles/explicit_tgv96_traffic.json); per-kernel
+  figures with their share of the byte budget in `roofline_detail`; and
+  `cpu_baseline` (the oracle, one core, on a bounded sample of the same
+  workload).
 """
 import argparse
 import json
@@ -33,26 +38,52 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0  # MI355X published HBM3E peak
 
-# Algorithmic bytes per cell and kernel launch (FP64, int32 indices, each
-# array element once per launch), tetrahedra (2 faces per cell, 4 faces in a
-# cell's list), second order, viscous.
-#   gradient kernel: U read 40, face n/area/lambda/cells 48 x 2 faces,
-#     cell->face list 4*4+4, 1/V 8, gradients write 240, min/max write 160
-#   residual/stage kernel: U read 40 + write 40, RK register read + write 80,
-#     face n/t1/t2/area/cells/x_f 112 x 2 faces, centroid 24, V 8, dx 24,
-#     cell->face list 20, gradients read 240, min/max read 160
-B_GRAD = 40 + 96 + 20 + 8 + 240 + 160
-B_STAGE = 80 + 80 + 224 + 24 + 8 + 24 + 20 + 240 + 160
-# Whole-stage compulsory traffic (each array element once per stage).
+# Algorithmic bytes per cell-stage (FP64, int32 indices, each array element
+# once per stage, tetrahedra: 2 faces per cell, 4 faces in a cell's list),
+# second order, viscous: U read + write 80, RK register read + write 80,
+# face n, t1, t2, area, lambda, x_f, cells 2 x 120, V + dx 32, centroid 24,
+# cell->face list 20, gradients write + read 480, min/max write + read 320.
 B_CELL_STAGE = 1276
+# The same terms split between the two kernels of a stage, each term counted
+# in the first kernel that touches it:
+#   gradient kernel: U read 40, face n/area/lambda/cells 2 x 48, cell->face
+#     list 20, 1/V 8, gradients write 240, min/max write 160
+#   residual/stage kernel: U write 40, RK register read + write 80, face
+#     t1/t2/x_f 2 x 72, centroid 24, dx 24, gradients read 240, min/max
+#     read 160
+B_GRAD = 40 + 96 + 20 + 8 + 240 + 160
+B_STAGE = 40 + 80 + 144 + 24 + 24 + 240 + 160
+assert B_GRAD + B_STAGE == B_CELL_STAGE
 # Tetrahedra, first order, inviscid (batched Euler cases)
 B_CELL_STAGE_FO = 388
-# HBM bytes per launch of the residual (K2) and gradient (K1) kernels on the
-# default 96^3 x 6 mesh, from rocprofv3 FETCH_SIZE and WRITE_SIZE per
-# dispatch scaled by a 1 GiB device copy's counters (read = 2.0 x FETCH_SIZE,
-# which the 128-B read-request counts confirm; written = 1.0 x WRITE_SIZE):
-# profiles/explicit_tgv96_rk3_kernels.md
-TRAFFIC_N96 = {"residual": 3.711e9 + 0.497e9, "gradient": 1.623e9 + 1.922e9}
+# Measured HBM bytes per launch of the two kernels (rocprofv3 FETCH_SIZE and
+# WRITE_SIZE scaled by a device copy's counters), valid only for the kernel
+# sources they were measured on (their SHA-256 is recorded with them).
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "explicit_tgv96_traffic.json")
+KERNEL_SOURCES = ["fv3d_amd/csrc/fv_kernels.hip", "fv3d_amd/csrc/fv_physics.h",
+                  "fv3d_amd/csrc/fv_device.h"]
+
+
+def kernel_source_hash():
+    import hashlib
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()
+
+
+def measured_traffic(n):
+    """Measured HBM bytes per launch {gradient, residual} for the current
+    kernel sources on the n^3 x 6 mesh, or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("sources_sha256") != kernel_source_hash() or t.get("n") != n:
+        return None
+    return t["bytes_per_launch"]
 
 
 def main():
@@ -108,29 +139,42 @@ def main():
         ms_per_step = sec * 1e3 / args.steps
 
         k_grad_s, k_stage_s = k_us[0] * 1e-6, k_us[1] * 1e-6
+        traffic = measured_traffic(args.n)
+        # roofline: the whole stage (gradient + residual kernel launches,
+        # HIP-event timed on the library's stream) on the survey's bytes per
+        # cell-stage
+        stage_s = k_grad_s + k_stage_s
+        ach = B_CELL_STAGE * N / stage_s / 1e9
+        roofline = {
+            "bound": "hbm",
+            "kernel": "explicit stage = fv_gradient_tile_kernel<4> + "
+                      "fv_residual_group_kernel<AUSM,face states,viscous,4>",
+            "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": round(ach / PEAK_HBM_GBS, 4),
+            "traffic": (traffic["gradient"] + traffic["residual"]) if traffic else None,
+            "bytes_per_cell_stage": B_CELL_STAGE, "cells": N,
+            "stage_us": round(stage_s * 1e6, 2)}
         ach_stage = B_STAGE * N / k_stage_s / 1e9
         ach_grad = B_GRAD * N / k_grad_s / 1e9
         step_gbs = B_CELL_STAGE * N * stages * args.steps / sec / 1e9
-        kernels = {
+        extra = {
             "residual_kernel": {
-                "bound": "hbm", "kernel": "fv_residual_group_kernel<AUSM,face states,viscous,4>",
-                "achieved": round(ach_stage, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "kernel": "fv_residual_group_kernel<AUSM,face states,viscous,4>",
+                "budget_bytes_per_cell": B_STAGE,
+                "achieved": round(ach_stage, 1), "unit": "GB/s",
                 "frac": round(ach_stage / PEAK_HBM_GBS, 4),
-                "traffic": TRAFFIC_N96["residual"] if args.n == 96 else None,
-                "bytes_per_cell": B_STAGE, "avg_us": round(k_us[1], 2)},
+                "traffic": traffic["residual"] if traffic else None,
+                "avg_us": round(k_us[1], 2)},
             "gradient_kernel": {
-                "bound": "hbm", "kernel": "fv_gradient_kernel<4,double>",
-                "achieved": round(ach_grad, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "kernel": "fv_gradient_tile_kernel<4>",
+                "budget_bytes_per_cell": B_GRAD,
+                "achieved": round(ach_grad, 1), "unit": "GB/s",
                 "frac": round(ach_grad / PEAK_HBM_GBS, 4),
-                "traffic": TRAFFIC_N96["gradient"] if args.n == 96 else None,
-                "bytes_per_cell": B_GRAD, "avg_us": round(k_us[0], 2)}}
-        # the dominant kernel (the longer launch) is the roofline; the other
-        # one and the whole step are in roofline_detail
-        dominant = max(kernels, key=lambda k: kernels[k]["avg_us"])
-        roofline = kernels.pop(dominant)
-        extra = dict(kernels)
-        extra["step"] = {"achieved": round(step_gbs, 1), "bytes_per_cell_stage": B_CELL_STAGE,
-                         "frac": round(step_gbs / PEAK_HBM_GBS, 4)}
+                "traffic": traffic["gradient"] if traffic else None,
+                "avg_us": round(k_us[0], 2)},
+            "step": {"achieved": round(step_gbs, 1), "bytes_per_cell_stage": B_CELL_STAGE,
+                     "frac": round(step_gbs / PEAK_HBM_GBS, 4),
+                     "note": "timed step loop (all kernels, 3 stages per step)"}}
         print("explicit: %.3f ms/step, %.3e cell-stage/s" % (ms_per_step, value),
               file=sys.stderr, flush=True)
         case.close()

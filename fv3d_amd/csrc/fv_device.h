@@ -66,7 +66,20 @@ struct DevMesh
     const float *__restrict__ fgA32;
     const float *__restrict__ fgB32;
     const float *__restrict__ crv32;
+    // cell tiles of the LDS-staged gradient kernel (null: not built): tile t
+    // is cells [t*kTile, (t+1)*kTile); its halo, the face neighbours outside
+    // the tile, is tile_halo[tile_halo_off[t] .. tile_halo_off[t+1]); nbl
+    // [N*maxf] gives each slot's neighbour as a row of the tile's LDS table
+    // (0..kTile-1 the tile's own cells, kTile + k halo cell k, 0xffff a
+    // boundary face); tile_rows = kTile + the largest halo
+    const int *__restrict__ tile_halo_off;
+    const int *__restrict__ tile_halo;
+    const unsigned short *__restrict__ nbl;
+    int tile_rows;
 };
+
+// cells per tile of the LDS-staged gradient kernel
+constexpr int kTile = 256;
 
 struct PhysDev
 {
@@ -83,6 +96,8 @@ struct DevScalars
     double res[5];
     int iter, max_iter_abs, done, final_pending;
     int transient, viscous, nan_flag, log_base;
+    int host_dt; // the host sets dt before each step (a time-step hook
+                 // that is not the built-in one); the finalize keeps it
 };
 
 // Outputs of the gradient/reconstruction kernel (null pointers are skipped).
@@ -156,6 +171,10 @@ void launch_gradient(hipStream_t s, const DevMesh &m, const PhysDev &P,
 void launch_residual(hipStream_t s, const DevMesh &m, const PhysDev &P,
                      const StageParams &sp, int flux, bool face_states,
                      bool viscous, bool fp32 = false);
+// The RK stage update of StageParams (U_out, k and, with do_reduce, the
+// partials of cell_blocks(N) workgroups) from the residual R [N][5].
+void launch_stage_update(hipStream_t s, const DevMesh &m, const PhysDev &P,
+                         const StageParams &sp, const double *R);
 void launch_dt_partials(hipStream_t s, const DevMesh &m, const PhysDev &P,
                         const double *U, double *partials, int partials_cap);
 // Primitive rows [N][5] (u, v, w, p, T) of U (con_to_prim).

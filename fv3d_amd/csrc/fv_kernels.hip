@@ -159,7 +159,7 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
 #pragma unroll
     for (int s = 0; s < MAXF; ++s)
     {
-        const int2 e = cfn[s];
+        const int2 e = cfn[s < m.maxf ? s : 0];
         fid[s] = e.x;
         own[s] = e.y >= 0;
         nbr[s] = own[s] ? e.y : -(e.y + 1);
@@ -313,14 +313,15 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
         {
             if (f32)
             {
-                const float4 q = reinterpret_cast<const float4 *>(m.crv32)[(size_t)c * m.maxf + s];
+                const float4 q =
+                    reinterpret_cast<const float4 *>(m.crv32)[(size_t)c * m.maxf + (s < m.maxf ? s : 0)];
                 rv[s][0] = q.x;
                 rv[s][1] = q.y;
                 rv[s][2] = q.z;
             }
             else
             {
-                const double *r = m.crv + ((size_t)c * m.maxf + s) * 3;
+                const double *r = m.crv + ((size_t)c * m.maxf + (s < m.maxf ? s : 0)) * 3;
                 rv[s][0] = Real(r[0]);
                 rv[s][1] = Real(r[1]);
                 rv[s][2] = Real(r[2]);
@@ -363,6 +364,229 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// K1 with the tile's states in LDS: the same gradients, bounds and face
+// states as fv_gradient_kernel (FP64, without the test outputs), but the
+// workgroup first stages the states of its tile of kTile consecutive cells
+// (contiguous loads) and of the tile's halo (the face neighbours outside the
+// tile and the ghost rows of the tile's boundary faces, gathered once per
+// tile) into LDS as primitive rows, so the per-face neighbour reads are LDS
+// reads and each state is converted to primitives once per tile instead of
+// once per face.  LDS: 10 doubles per row, variable-major (tl[j*rows + r]).
+// ---------------------------------------------------------------------------
+#ifndef FVH_K1T_MINW
+#define FVH_K1T_MINW 2
+#endif
+template <int MAXF>
+__global__ __launch_bounds__(kTile, FVH_K1T_MINW) void fv_gradient_tile_kernel(
+    DevMesh m, PhysDev P, const double *__restrict__ U,
+    const double *__restrict__ gphi, GradOut out, int linear,
+    const DevScalars *sc)
+{
+    if (sc && sc->done)
+        return;
+    extern __shared__ double tl[];
+    const int N = m.N;
+    const int rows = m.tile_rows;
+    const int t = out.c_begin / kTile + xcd_tile(gridDim.x);
+    const int base = t * kTile;
+    const int h0 = m.tile_halo_off[t];
+    const int nrow = kTile + (m.tile_halo_off[t + 1] - h0);
+    for (int i = threadIdx.x; i < nrow; i += kTile)
+    {
+        double q[10];
+        if (i < kTile)
+        {
+            if (base + i >= N)
+                continue;
+            load_cons(U, base + i, q);
+            con_to_prim(P, q);
+        }
+        else
+        {
+            const int g = m.tile_halo[h0 + i - kTile];
+            if (g < N)
+            {
+                load_cons(U, g, q);
+                con_to_prim(P, q);
+            }
+            else
+            {
+                const double *gp = gphi + (size_t)(g - N) * 10;
+#pragma unroll
+                for (int j = 0; j < 10; ++j)
+                    q[j] = gp[j];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 10; ++j)
+            tl[j * rows + i] = q[j];
+    }
+    __syncthreads();
+    // cells past the launch range (the last tile) take part in the barriers
+    // of the output passes without computing
+    const int c = base + threadIdx.x;
+    const bool valid = c < out.c_end;
+    const int cc = valid ? c : base;
+
+    double pc[10];
+#pragma unroll
+    for (int j = 0; j < 10; ++j)
+        pc[j] = tl[j * rows + threadIdx.x];
+    double gx[10], gy[10], gz[10], mn[10], mx[10];
+#pragma unroll
+    for (int j = 0; j < 10; ++j)
+    {
+        gx[j] = 0.0;
+        gy[j] = 0.0;
+        gz[j] = 0.0;
+        mn[j] = pc[j];
+        mx[j] = pc[j];
+    }
+    const int nf = valid ? m.ncf[c] : 0;
+    const int2 *cfn = m.cfn + (size_t)cc * m.maxf;
+    const unsigned short *nbl = m.nbl + (size_t)cc * m.maxf;
+    int fid[MAXF], loc[MAXF];
+    bool own[MAXF];
+#pragma unroll
+    for (int s = 0; s < MAXF; ++s)
+    {
+        // slots past the mesh's face count per cell (MAXF = 6 with 5-face
+        // cells) take slot 0, like the slots past a cell's own count
+        const int sl = s < m.maxf ? s : 0;
+        const int2 e = cfn[sl];
+        fid[s] = e.x;
+        own[s] = e.y >= 0;
+        loc[s] = nbl[sl];
+    }
+    // the face records of all faces first (one round of gathers)
+    double gn[MAXF][5];
+#pragma unroll
+    for (int s = 0; s < MAXF; ++s)
+    {
+        const double2 *g = reinterpret_cast<const double2 *>(m.fgB + (size_t)fid[s] * kFgB);
+        const double2 g0 = g[0], g1 = g[1], g2 = g[2];
+        gn[s][0] = g0.x, gn[s][1] = g0.y, gn[s][2] = g1.x;
+        gn[s][3] = g1.y, gn[s][4] = g2.x;
+    }
+#pragma unroll
+    for (int s = 0; s < MAXF; ++s)
+    {
+        if (s >= nf)
+            break;
+        double pn[10];
+#pragma unroll
+        for (int j = 0; j < 10; ++j)
+            pn[j] = tl[j * rows + loc[s]];
+        const bool own0 = own[s];
+        const double n0 = gn[s][0], n1 = gn[s][1], n2 = gn[s][2], area = gn[s][3],
+                     lam = gn[s][4];
+#pragma unroll
+        for (int j = 0; j < 10; ++j)
+        {
+            const double p0 = own0 ? pc[j] : pn[j];
+            const double p1 = own0 ? pn[j] : pc[j];
+            const double pm = p0 + lam * (p1 - p0);
+            const double ax = pm * n0 * area;
+            const double ay = pm * n1 * area;
+            const double az = pm * n2 * area;
+            if (j != 8)
+            {
+                if (own0)
+                {
+                    gx[j] += ax;
+                    gy[j] += ay;
+                    gz[j] += az;
+                }
+                else
+                {
+                    gx[j] -= ax;
+                    gy[j] -= ay;
+                    gz[j] -= az;
+                }
+            }
+            if (j < 8)
+            {
+                mn[j] = rmin(mn[j], pn[j]);
+                mx[j] = rmax(mx[j], pn[j]);
+            }
+        }
+    }
+    const double sv = m.svol[cc];
+#pragma unroll
+    for (int j = 0; j < 10; ++j)
+    {
+        gx[j] *= sv;
+        gy[j] *= sv;
+        gz[j] *= sv;
+    }
+    double rv[MAXF][3];
+    if (linear && out.fs)
+    {
+#pragma unroll
+        for (int s = 0; s < MAXF; ++s)
+        {
+            const double *r = m.crv + ((size_t)cc * m.maxf + (s < m.maxf ? s : 0)) * 3;
+            rv[s][0] = r[0];
+            rv[s][1] = r[1];
+            rv[s][2] = r[2];
+        }
+    }
+
+    // Output passes through LDS (the states table is no longer read), so
+    // that each store instruction of a wave writes whole contiguous lines:
+    // the tile's viscous gradient rows are one contiguous block; the face
+    // states go out one pair of slots at a time (two 64-byte rows of a cell,
+    // one 128-byte line).
+    const int ncell = min(kTile, out.c_end - base);
+    __syncthreads();
+    if (out.vg)
+    {
+        double *o = tl + threadIdx.x * 12;
+        o[0] = gx[1], o[1] = gx[2], o[2] = gx[3], o[3] = gx[9];
+        o[4] = gy[1], o[5] = gy[2], o[6] = gy[3], o[7] = gy[9];
+        o[8] = gz[1], o[9] = gz[2], o[10] = gz[3], o[11] = gz[9];
+        __syncthreads();
+        const double2 *src = reinterpret_cast<const double2 *>(tl);
+        double2 *dst = reinterpret_cast<double2 *>(out.vg + (size_t)base * 12);
+        for (int i = threadIdx.x; i < ncell * 6; i += kTile)
+            dst[i] = src[i];
+        __syncthreads();
+    }
+    if (!out.fs)
+        return;
+#pragma unroll
+    for (int s0 = 0; s0 < MAXF; s0 += 2)
+    {
+        const int ns = min(2, m.maxf - s0);
+        if (ns <= 0)
+            break;
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+        {
+            const int s = s0 + h;
+            if (h >= ns)
+                break;
+            double *o = tl + threadIdx.x * 16 + h * 8;
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                o[j] = linear ? limited(pc[j], gx[j], gy[j], gz[j], rv[s], mn[j], mx[j], P.bj)
+                              : pc[j];
+        }
+        __syncthreads();
+        // 4 double2 per row, ns rows per cell
+        const double2 *src = reinterpret_cast<const double2 *>(tl);
+        const int per = 4 * ns;
+        for (int i = threadIdx.x; i < ncell * per; i += kTile)
+        {
+            const int cl = i / per, q = i - cl * per;
+            double2 *dst = reinterpret_cast<double2 *>(
+                out.fs + ((size_t)(base + cl) * m.maxf + s0) * 8);
+            dst[q] = src[cl * 8 + q];
+        }
+        __syncthreads();
+    }
+}
+
 // K2: residual dU/dt = -(sum_f +-flux_f A_f)/V (finite_volume.c:96-123) in
 // lane groups: LANES consecutive lanes per cell (4 for tetrahedra, 8 for
 // hexahedra), one face per lane.  The group's contributions are summed in
@@ -384,7 +608,7 @@ __global__ __launch_bounds__(kBlock, (FS && VISC && std::is_same<Real, double>::
                                          : FVH_K2_MINW) void fv_residual_group_kernel(
     DevMesh m, PhysDev P, StageParams sp)
 {
-    __shared__ double sh[kBlock];
+    __shared__ double sh[8 * kBlock / 64];
     const int N = m.N;
     const int tid = xcd_tile(gridDim.x) * kBlock + threadIdx.x;
     const int c = sp.c_begin + tid / LANES;
@@ -561,36 +785,54 @@ __global__ __launch_bounds__(kBlock, (FS && VISC && std::is_same<Real, double>::
     }
 
     double part[7] = {0.0, 0.0, 0.0, 0.0, 0.0, DBL_MAX, DBL_MAX};
-    if (cell_ok && s == 0)
+    if (cell_ok)
     {
+        // every lane of the group has the sums; each computes the update of
+        // all five variables (the same instructions as one lane) and stores
+        // the variables j = s, s + LANES, ..., so that a store instruction
+        // writes the group's row, not one value per group
         double R5[5];
 #pragma unroll
         for (int j = 0; j < 5; ++j)
             R5[j] = -double(acc[j]) / vol;
-        if (sp.R_out)
-        {
-#pragma unroll
-            for (int j = 0; j < 5; ++j)
-                sp.R_out[(size_t)c * 5 + j] = R5[j];
-        }
+        double kk[5], unew[5];
         if (sp.do_update)
         {
-            double pc[10];
-#pragma unroll
-            for (int j = 0; j < 5; ++j)
-                pc[j] = u5[j];
             const double dt = sp.case_dt ? sp.case_dt[sp.cell_case[c]] : sp.sc->dt;
-            double unew[5];
 #pragma unroll
             for (int j = 0; j < 5; ++j)
             {
                 double kj = R5[j];
                 if (!sp.stage0)
                     kj = R5[j] + k5[j] * sp.rk_a;
-                sp.k[(size_t)c * 5 + j] = kj;
-                unew[j] = pc[j] + kj * dt * sp.rk_g;
-                sp.U_out[(size_t)c * 5 + j] = unew[j];
+                kk[j] = kj;
+                unew[j] = u5[j] + kj * dt * sp.rk_g;
             }
+        }
+        auto pick = [](const double *a, int j) {
+            return j == 0 ? a[0] : j == 1 ? a[1] : j == 2 ? a[2] : j == 3 ? a[3] : a[4];
+        };
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+        {
+            const int j = q * LANES + s;
+            if (j < 5)
+            {
+                if (sp.R_out)
+                    sp.R_out[(size_t)c * 5 + j] = pick(R5, j);
+                if (sp.do_update)
+                {
+                    sp.k[(size_t)c * 5 + j] = pick(kk, j);
+                    sp.U_out[(size_t)c * 5 + j] = pick(unew, j);
+                }
+            }
+        }
+        if (s == 0 && sp.do_update)
+        {
+            double pc[10];
+#pragma unroll
+            for (int j = 0; j < 5; ++j)
+                pc[j] = u5[j];
             if (sp.prims_out)
             {
                 double q[10];
@@ -631,11 +873,7 @@ __global__ __launch_bounds__(kBlock, (FS && VISC && std::is_same<Real, double>::
 
     if (sp.do_reduce)
     {
-#pragma unroll
-        for (int q = 0; q < 5; ++q)
-            block_sum(sh, part[q]);
-        block_min(sh, part[5]);
-        block_min(sh, part[6]);
+        block_reduce7(sh, part);
         if (threadIdx.x == 0)
         {
 #pragma unroll
@@ -643,6 +881,79 @@ __global__ __launch_bounds__(kBlock, (FS && VISC && std::is_same<Real, double>::
                 sp.partials[blockIdx.x * kPartialStride + q] = part[q];
         }
     }
+}
+
+// The 2N-storage RK stage (explicit.c:113-140) from a residual R computed
+// elsewhere (the host-mirror path): one thread per cell, the same arithmetic
+// as the residual kernel's epilogue, partials of the last stage per
+// workgroup of kBlock cells.
+__global__ __launch_bounds__(kBlock) void fv_stage_update_kernel(DevMesh m, PhysDev P,
+                                                                StageParams sp,
+                                                                const double *__restrict__ R)
+{
+    __shared__ double sh[8 * kBlock / 64];
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool ok = c < m.N;
+    double part[7] = {0.0, 0.0, 0.0, 0.0, 0.0, DBL_MAX, DBL_MAX};
+    if (ok && sp.sc->done)
+    {
+#pragma unroll
+        for (int j = 0; j < 5; ++j)
+            sp.U_out[(size_t)c * 5 + j] = sp.U_in[(size_t)c * 5 + j];
+    }
+    else if (ok)
+    {
+        const double vol = m.vol[c];
+        const double dt = sp.sc->dt;
+        double pc[10], R5[5], unew[5];
+#pragma unroll
+        for (int j = 0; j < 5; ++j)
+        {
+            pc[j] = sp.U_in[(size_t)c * 5 + j];
+            R5[j] = R[(size_t)c * 5 + j];
+            double kj = R5[j];
+            if (!sp.stage0)
+                kj = R5[j] + sp.k[(size_t)c * 5 + j] * sp.rk_a;
+            sp.k[(size_t)c * 5 + j] = kj;
+            unew[j] = pc[j] + kj * dt * sp.rk_g;
+            sp.U_out[(size_t)c * 5 + j] = unew[j];
+        }
+        if (sp.do_reduce)
+        {
+            con_to_prim(P, pc);
+#pragma unroll
+            for (int j = 0; j < 5; ++j)
+                part[j] = rabs(R5[j]) * vol;
+            const double dx0 = m.cdx[(size_t)c * 3], dx1 = m.cdx[(size_t)c * 3 + 1],
+                         dx2 = m.cdx[(size_t)c * 3 + 2];
+            const double s_rho = 1.0 / unew[0];
+            const double cs = sqrt(P.kappa * pc[8] * s_rho);
+            const double ds1 = (rabs(pc[5]) + cs) * dx0 + (rabs(pc[6]) + cs) * dx1 +
+                               (rabs(pc[7]) + cs) * dx2;
+            part[5] = vol / ds1;
+            const double ds2 = dot3(dx0, dx1, dx2, dx0, dx1, dx2);
+            part[6] = s_rho * ds2 / vol;
+        }
+    }
+    if (sp.do_reduce)
+    {
+        block_reduce7(sh, part);
+        if (threadIdx.x == 0)
+#pragma unroll
+            for (int q = 0; q < 7; ++q)
+                sp.partials[blockIdx.x * kPartialStride + q] = part[q];
+    }
+}
+
+void launch_stage_update(hipStream_t s, const DevMesh &m, const PhysDev &P,
+                         const StageParams &sp, const double *R)
+{
+    if (!sp.sc)
+        throw std::runtime_error("launch_stage_update: no time step");
+    if (sp.do_reduce && cell_blocks(m.N) > sp.partials_cap)
+        throw std::runtime_error("reduction partials buffer too small");
+    hipLaunchKernelGGL(fv_stage_update_kernel, dim3(cell_blocks(m.N)), dim3(kBlock), 0, s, m, P,
+                       sp, R);
 }
 
 // Time-step partials of a state whose primitives are current (the first
@@ -704,12 +1015,7 @@ __global__ __launch_bounds__(kBlock) void fv_fold_partials_kernel(
         v[5] = rmin(v[5], p[5]);
         v[6] = rmin(v[6], p[6]);
     }
-    if (mode == 1)
-#pragma unroll
-        for (int q = 0; q < 5; ++q)
-            block_sum(sh, v[q]);
-    block_min(sh, v[5]);
-    block_min(sh, v[6]);
+    block_reduce7(sh, v);
     if (t == 0)
 #pragma unroll
         for (int q = 0; q < 7; ++q)
@@ -737,12 +1043,7 @@ __global__ __launch_bounds__(kBlock) void fv_finalize_kernel(
         v[5] = rmin(v[5], p[5]);
         v[6] = rmin(v[6], p[6]);
     }
-    if (mode == 1)
-#pragma unroll
-        for (int q = 0; q < 5; ++q)
-            block_sum(sh, v[q]);
-    block_min(sh, v[5]);
-    block_min(sh, v[6]);
+    block_reduce7(sh, v);
     if (t != 0)
         return;
     if (sc->done)
@@ -790,6 +1091,8 @@ __global__ __launch_bounds__(kBlock) void fv_finalize_kernel(
             sc->done = 1;
             return;
         }
+        if (sc->host_dt)
+            return;
     }
 
     double lc = v[5] * P.cfl;
@@ -917,6 +1220,19 @@ void launch_gradient(hipStream_t s, const DevMesh &m, const PhysDev &P,
             hipLaunchKernelGGL((fv_gradient_kernel<6, double, true>), dim3(grid),
                                dim3(kBlock), FVH_K1_LDS, s, m, P, U, gphi, out, linear,
                                sc);
+        return;
+    }
+    if (m.tile_halo_off && !out.prims && out.c_begin % kTile == 0 &&
+        (out.c_end % kTile == 0 || out.c_end == m.N))
+    {
+        const size_t lds = (size_t)max(10 * m.tile_rows, 16 * kTile) * sizeof(double);
+        const int tiles = (out.c_end - out.c_begin + kTile - 1) / kTile;
+        if (m.maxf <= 4)
+            hipLaunchKernelGGL((fv_gradient_tile_kernel<4>), dim3(tiles), dim3(kTile), lds, s,
+                               m, P, U, gphi, out, linear, sc);
+        else
+            hipLaunchKernelGGL((fv_gradient_tile_kernel<6>), dim3(tiles), dim3(kTile), lds, s,
+                               m, P, U, gphi, out, linear, sc);
         return;
     }
     if (m.maxf <= 4)

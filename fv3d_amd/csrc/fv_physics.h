@@ -525,36 +525,83 @@ __device__ __forceinline__ void invert5(const double *A, double *Ainv)
             Ainv[r * 5 + q] = M[r][5 + q];
 }
 
+// Block reductions: a butterfly over the 64 lanes of each wavefront (lane
+// shuffles), then the waves' results combined in wave order by every thread
+// from LDS.  One barrier per reduction (plus one before sh is reused); the
+// order is fixed, so results are bitwise reproducible.  blockDim.x must be a
+// multiple of 64; sh holds at least blockDim.x / 64 doubles.
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1)
+        v = v + __shfl_xor(v, m, 64);
+    return v;
+}
+
+__device__ __forceinline__ double wave_min(double v)
+{
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1)
+        v = rmin(v, __shfl_xor(v, m, 64));
+    return v;
+}
+
 __device__ __forceinline__ void block_sum(double *sh, double &v)
 {
-    const int t = threadIdx.x;
-    sh[t] = v;
+    const double w = wave_sum(v);
+    const int nw = blockDim.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        sh[threadIdx.x >> 6] = w;
     __syncthreads();
-#pragma unroll
-    for (int s = kBlock / 2; s > 0; s >>= 1)
-    {
-        if (t < s)
-            sh[t] = sh[t] + sh[t + s];
-        __syncthreads();
-    }
-    v = sh[0];
+    double a = sh[0];
+    for (int i = 1; i < nw; ++i)
+        a = a + sh[i];
+    v = a;
     __syncthreads();
 }
 
 __device__ __forceinline__ void block_min(double *sh, double &v)
 {
-    const int t = threadIdx.x;
-    sh[t] = v;
+    const double w = wave_min(v);
+    const int nw = blockDim.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        sh[threadIdx.x >> 6] = w;
     __syncthreads();
+    double a = sh[0];
+    for (int i = 1; i < nw; ++i)
+        a = rmin(a, sh[i]);
+    v = a;
+    __syncthreads();
+}
+
+// The 7 partials of the time loop (sums of v[0..4], minima of v[5..6]) over
+// the block, with a single barrier; the result is valid in thread 0 only.
+// sh holds at least 8 * blockDim.x / 64 doubles.
+__device__ __forceinline__ void block_reduce7(double *sh, double *v)
+{
+    double w[7];
 #pragma unroll
-    for (int s = kBlock / 2; s > 0; s >>= 1)
-    {
-        if (t < s)
-            sh[t] = rmin(sh[t], sh[t + s]);
-        __syncthreads();
-    }
-    v = sh[0];
+    for (int q = 0; q < 5; ++q)
+        w[q] = wave_sum(v[q]);
+    w[5] = wave_min(v[5]);
+    w[6] = wave_min(v[6]);
+    const int nw = blockDim.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int q = 0; q < 7; ++q)
+            sh[(threadIdx.x >> 6) * 8 + q] = w[q];
     __syncthreads();
+    if (threadIdx.x == 0)
+    {
+#pragma unroll
+        for (int q = 0; q < 7; ++q)
+        {
+            double a = sh[q];
+            for (int i = 1; i < nw; ++i)
+                a = q < 5 ? a + sh[i * 8 + q] : rmin(a, sh[i * 8 + q]);
+            v[q] = a;
+        }
+    }
 }
 
 } // namespace dev

@@ -27,6 +27,7 @@
 #include "runtime.h"
 #include "implicit.h"
 #include "krylov.h"
+#include "mirror.h"
 
 // ---------------------------------------------------------------------------
 // reference globals (solver_module.h:16-18, navier_stokes.c:16-55)
@@ -171,6 +172,12 @@ struct Device
     int log_cap = 0;
     double *mirror_phi = nullptr; // AoS staging for the host mirror
     double *mirror_dt = nullptr;
+    // host-mirror residual path (allocated on first use): gradients
+    // [(N+B)][10] x 3, face states [F][10] x 2, fluxes [F][5], residual,
+    // distance vectors [F][3] x 2
+    double *mg[3] = {nullptr, nullptr, nullptr};
+    double *mL = nullptr, *mR = nullptr, *mflux = nullptr, *mres = nullptr;
+    double *md1 = nullptr, *md2 = nullptr;
     double vglob = 0.0;
     bool r_valid = false;
     // reference-semantics implicit step (allocated on first use)
@@ -239,11 +246,20 @@ template <class T> T *xcopy(const std::vector<T> &v)
     return p;
 }
 
+// The residual hooks are the built-in Navier-Stokes ones: the fused device
+// kernels run.  Otherwise the residual takes the host-mirror path.
 bool builtin_hooks()
 {
     return g_calc_flux == calc_ns_flux && g_update == ns_update &&
            g_update_gradients == ns_update_gradients;
 }
+
+// The time-step hook is the built-in one (computed on the device inside the
+// time loop); otherwise the host calls it before every step.
+bool builtin_timestep() { return g_calc_timestep == calc_ns_timestep; }
+
+// Time passed to host hooks (the stage time of the current residual).
+double g_hook_time = 0.0;
 
 dev::PhysDev phys()
 {
@@ -274,10 +290,9 @@ void require_ready()
                                  "(init_finite_volume)");
     if (g_reconstruction < 0 || g_limiter < 0)
         throw std::runtime_error("reconstruction/limiter not initialised");
-    if (!builtin_hooks())
-        throw std::runtime_error(
-            "only the built-in Navier-Stokes device hooks are supported "
-            "(calc_ns_flux, ns_update, ns_update_gradients)");
+    if (!g_calc_flux || !g_update || !g_update_gradients || !g_calc_timestep)
+        throw std::runtime_error("a flux, update, update-gradients or time-step hook is "
+                                 "not set");
 }
 
 // The reconstruction's distance vectors of the mesh by cell face slot
@@ -405,6 +420,136 @@ void ensure_prims()
     }
 }
 
+// ---- host-mirror residual path ---------------------------------------------
+mirror::MirrorMesh mirror_mesh()
+{
+    const dev::DevMesh &d = g_dev.m;
+    mirror::MirrorMesh m{};
+    m.N = d.N;
+    m.F = d.F;
+    m.B = d.B;
+    m.maxf = d.maxf;
+    m.cf = d.cf;
+    m.ncf = d.ncf;
+    m.fcell = d.fcell;
+    m.fgA = d.fgA;
+    m.fgB = d.fgB;
+    m.d1 = g_dev.md1;
+    m.d2 = g_dev.md2;
+    m.vol = d.vol;
+    m.svol = d.svol;
+    m.bnd_face = d.bnd_face;
+    m.bnd_region = d.bnd_region;
+    m.region_type = d.region_type;
+    return m;
+}
+
+void ensure_mirror_buffers()
+{
+    if (g_dev.mL)
+        return;
+    const size_t N = g_dev.m.N, F = g_dev.m.F, B = g_dev.m.B;
+    for (int d = 0; d < 3; ++d)
+        g_dev.mg[d] = dmalloc<double>((N + B) * 10);
+    g_dev.mL = dmalloc<double>(F * 10);
+    g_dev.mR = dmalloc<double>(F * 10);
+    g_dev.mflux = dmalloc<double>(F * 5);
+    g_dev.mres = dmalloc<double>(N * 5);
+    g_dev.md1 = dupload(solver_mesh->faces->dist_cell_1, F * 3);
+    g_dev.md2 = dupload(solver_mesh->faces->dist_cell_2, F * 3);
+}
+
+void d2h(double *h, const double *d, size_t n)
+{
+    HIP_CHECK(hipMemcpyAsync(h, d, n * sizeof(double), hipMemcpyDeviceToHost, g_dev.stream));
+    HIP_CHECK(hipStreamSynchronize(g_dev.stream));
+}
+
+void h2d(double *d, const double *h, size_t n)
+{
+    HIP_CHECK(hipMemcpyAsync(d, h, n * sizeof(double), hipMemcpyHostToDevice, g_dev.stream));
+    HIP_CHECK(hipStreamSynchronize(g_dev.stream));
+}
+
+// One residual evaluation in the reference's passes (finite_volume.c:81-124
+// with reconstruction.c:142-255 and the hooks): R_out = dU/dt(U_in).  Each
+// hook that is not a built-in runs on the host between the device passes,
+// on the host mirror (solver_data), which is synchronised before and
+// uploaded after the call.  host_all: leave every solver_data array
+// (phi_total, the gradients, the face states, the fluxes) current on the
+// host, as the reference's residual does.
+void mirror_residual(double t, const double *U_in, double *R_out, bool host_all)
+{
+    ensure_mirror_buffers();
+    hipStream_t st = g_dev.stream;
+    const int N = g_dev.m.N, F = g_dev.m.F, B = g_dev.m.B;
+    const size_t rows = (size_t)(N + B) * 10;
+    const mirror::MirrorMesh mm = mirror_mesh();
+    const dev::PhysDev P = phys();
+    double *phi = g_dev.mirror_phi;
+    double **g = g_dev.mg;
+    double *hg[3] = {solver_data->grad_phi_total_x, solver_data->grad_phi_total_y,
+                     solver_data->grad_phi_total_z};
+    bool host_phi = false, host_grad = false;
+    // update hook (ns_update: primitives of the domain cells, ghost rows)
+    dev::launch_mirror(st, g_dev.m, P, U_in, nullptr, phi, nullptr);
+    if (g_update != ns_update)
+    {
+        d2h(solver_data->phi_total, phi, rows);
+        g_update(t);
+        h2d(phi, solver_data->phi_total, rows);
+        host_phi = true;
+    }
+    // gradients, then the update-gradients hook on the ghost rows
+    mirror::gradients(st, mm, phi, g[0], g[1], g[2]);
+    mirror::ghost_raw(st, mm, phi, g[0], g[1], g[2]);
+    if (g_update_gradients != ns_update_gradients)
+    {
+        for (int d = 0; d < 3; ++d)
+            d2h(hg[d], g[d], rows);
+        g_update_gradients(t);
+        for (int d = 0; d < 3; ++d)
+            h2d(g[d], hg[d], rows);
+        host_grad = true;
+    }
+    else
+        mirror::ghost_gradients(st, mm, g[0], g[1], g[2]);
+    mirror::reconstruct(st, mm, P, is_linear(), phi, g[0], g[1], g[2], g_dev.mL, g_dev.mR);
+    bool host_faces = false;
+    if (g_calc_flux != calc_ns_flux)
+    {
+        if (!host_phi)
+            d2h(solver_data->phi_total, phi, rows);
+        if (!host_grad)
+            for (int d = 0; d < 3; ++d)
+                d2h(hg[d], g[d], rows);
+        d2h(solver_data->phi_total_left, g_dev.mL, (size_t)F * 10);
+        d2h(solver_data->phi_total_right, g_dev.mR, (size_t)F * 10);
+        host_phi = host_grad = host_faces = true;
+        g_calc_flux(t);
+        h2d(g_dev.mflux, solver_data->flux, (size_t)F * 5);
+    }
+    else
+        mirror::flux(st, mm, P, g_flux, is_viscous(), g_dev.mL, g_dev.mR, g[0], g[1], g[2],
+                     g_dev.mflux);
+    mirror::sum_fluxes(st, mm, g_dev.mflux, R_out);
+    if (host_all)
+    {
+        if (!host_phi)
+            d2h(solver_data->phi_total, phi, rows);
+        if (!host_grad)
+            for (int d = 0; d < 3; ++d)
+                d2h(hg[d], g[d], rows);
+        if (!host_faces)
+        {
+            d2h(solver_data->phi_total_left, g_dev.mL, (size_t)F * 10);
+            d2h(solver_data->phi_total_right, g_dev.mR, (size_t)F * 10);
+        }
+        if (g_calc_flux == calc_ns_flux)
+            d2h(solver_data->flux, g_dev.mflux, (size_t)F * 5);
+    }
+}
+
 void enqueue_residual(const double *U_in, double *U_out, double *R_out,
                       int stage, bool update, bool reduce, dev::DevScalars *sc,
                       const double *prims_in = nullptr, double *prims_out = nullptr,
@@ -412,6 +557,31 @@ void enqueue_residual(const double *U_in, double *U_out, double *R_out,
                       const float *U32 = nullptr)
 {
     const dev::PhysDev P = phys();
+    if (!builtin_hooks())
+    {
+        // a host hook: the residual through the host mirror, then the stage
+        if (fp32)
+            throw std::runtime_error("the FP32 residual needs the built-in hooks");
+        double *R = R_out ? R_out : g_dev.R;
+        mirror_residual(g_hook_time, U_in, R, false);
+        if (update)
+        {
+            dev::StageParams sp{};
+            sp.U_in = U_in;
+            sp.U_out = U_out;
+            sp.k = g_dev.k;
+            sp.rk_a = g_rk_a[stage];
+            sp.rk_g = g_rk_g[stage];
+            sp.stage0 = stage == 0;
+            sp.do_update = 1;
+            sp.do_reduce = reduce;
+            sp.sc = sc;
+            sp.partials = g_dev.partials;
+            sp.partials_cap = g_dev.partials_cap;
+            dev::launch_stage_update(g_dev.stream, g_dev.m, P, sp, R);
+        }
+        return;
+    }
     ensure_gradient_buffers();
     // the FP32 residual keeps its face states and face records in FP32 and
     // may take its state in FP32
@@ -425,16 +595,16 @@ void enqueue_residual(const double *U_in, double *U_out, double *R_out,
     dev::launch_ghost(g_dev.stream, g_dev.m, P, U_in, g_dev.gphi,
                       is_linear() && !fs32 ? g_dev.fs : nullptr, sc,
                       fs32 ? g_dev.fs32 : nullptr, U32);
+    dev::GradOut go = grad_out();
+    go.prims = prims_in;
+    go.U32 = U32;
+    if (fs32)
+    {
+        go.fs = nullptr;
+        go.fs32 = g_dev.fs32;
+    }
     if (needs_gradients())
     {
-        dev::GradOut go = grad_out();
-        go.prims = prims_in;
-        go.U32 = U32;
-        if (fs32)
-        {
-            go.fs = nullptr;
-            go.fs32 = g_dev.fs32;
-        }
         if (tm)
             tm->start();
         dev::launch_gradient(g_dev.stream, g_dev.m, P, U_in, g_dev.gphi, go,
@@ -469,6 +639,69 @@ void enqueue_residual(const double *U_in, double *U_out, double *R_out,
     if (tm)
         tm->stop(1);
 }
+
+// Tiles of the LDS-staged gradient kernel: per tile of dev::kTile
+// consecutive cells the sorted halo (face neighbours outside the tile; ghost
+// rows as N + boundary) and per cell face slot the neighbour's row in the
+// tile's LDS table.  Not built (the plain gather kernel runs) when
+// FVH_K1_TILE=0 or when a tile's table would exceed 64 KiB of LDS.
+void setup_tile_halo()
+{
+    const char *e = std::getenv("FVH_K1_TILE");
+    if (e && std::atoi(e) == 0)
+        return;
+    const int N = g_dev.m.N, maxf = g_dev.m.maxf, T = dev::kTile;
+    const int nt = (N + T - 1) / T;
+    const Cells_t *cells = solver_mesh->cells;
+    const Faces_t *faces = solver_mesh->faces;
+    std::vector<int> off(nt + 1, 0), halo;
+    std::vector<unsigned short> nbl((size_t)N * maxf, 0);
+    int hmax = 0;
+    std::vector<int> nbs;
+    for (int t = 0; t < nt; ++t)
+    {
+        const int b0 = t * T, b1 = std::min(N, b0 + T);
+        nbs.clear();
+        for (int c = b0; c < b1; ++c)
+            for (int s = 0; s < cells->n_faces[c]; ++s)
+            {
+                const int f = cells->faces[(size_t)c * maxf + s];
+                const int nb = faces->cells[2 * f] == c ? faces->cells[2 * f + 1]
+                                                        : faces->cells[2 * f];
+                if (nb < b0 || nb >= b1)
+                    nbs.push_back(nb);
+            }
+        std::sort(nbs.begin(), nbs.end());
+        nbs.erase(std::unique(nbs.begin(), nbs.end()), nbs.end());
+        hmax = std::max(hmax, (int)nbs.size());
+        for (int c = b0; c < b1; ++c)
+        {
+            const int nf = cells->n_faces[c];
+            for (int s = 0; s < maxf; ++s)
+            {
+                const int f = cells->faces[(size_t)c * maxf + (s < nf ? s : 0)];
+                const int nb = faces->cells[2 * f] == c ? faces->cells[2 * f + 1]
+                                                        : faces->cells[2 * f];
+                int r;
+                if (nb >= b0 && nb < b1)
+                    r = nb - b0;
+                else
+                    r = T + (int)(std::lower_bound(nbs.begin(), nbs.end(), nb) - nbs.begin());
+                nbl[(size_t)c * maxf + s] = (unsigned short)r;
+            }
+        }
+        halo.insert(halo.end(), nbs.begin(), nbs.end());
+        off[t + 1] = (int)halo.size();
+    }
+    const int rows = T + hmax;
+    if ((size_t)10 * rows * sizeof(double) > 65536)
+        return;
+    g_dev.m.tile_halo_off = dupload(off.data(), off.size());
+    g_dev.m.tile_halo = dupload(halo.data(), halo.size());
+    g_dev.m.nbl = dupload(nbl.data(), nbl.size());
+    g_dev.m.tile_rows = rows;
+}
+
 
 void sync_mirror(bool with_dt)
 {
@@ -781,12 +1014,15 @@ void print_row(int iter, const double *row, bool out, int inner = 0, int lsoe = 
 }
 
 // One explicit step (timestep_lserkw2, explicit.c:96-141) + bookkeeping.
-// One explicit step.
-void enqueue_explicit_step(dev::DevScalars *sc, KernelTimer *tm = nullptr)
+// t, dt: the step's time and time step as the host knows them (host hooks
+// receive the stage time t + b_s dt; the device kernels read dt from sc).
+void enqueue_explicit_step(dev::DevScalars *sc, KernelTimer *tm = nullptr, double t = 0.0,
+                           double dt = 0.0)
 {
     for (int s = 0; s < g_rk_stages; ++s)
     {
         const bool last = s == g_rk_stages - 1;
+        g_hook_time = t + g_rk_b[s] * dt;
         double *Uin = g_dev.U[g_dev.cur];
         double *Uout = g_dev.U[1 - g_dev.cur];
         // the primitive rows are not passed here: the gradient kernel is
@@ -799,10 +1035,31 @@ void enqueue_explicit_step(dev::DevScalars *sc, KernelTimer *tm = nullptr)
     }
     if (tm)
         tm->start();
-    dev::launch_finalize(g_dev.stream, sc, g_dev.partials, dev::residual_grid(g_dev.m), g_dev.fold, g_dev.fold_cap, phys(), 1, g_dev.log,
-                         g_dev.log_cap);
+    const int n_partials =
+        builtin_hooks() ? dev::residual_grid(g_dev.m) : dev::cell_blocks(g_dev.m.N);
+    dev::launch_finalize(g_dev.stream, sc, g_dev.partials, n_partials, g_dev.fold,
+                         g_dev.fold_cap, phys(), 1, g_dev.log, g_dev.log_cap);
     if (tm)
         tm->stop(2);
+}
+
+// A time-step hook that is not the built-in one, called on the host before a
+// step (timedisc.c:164-172): the host mirror is refreshed so that the hook
+// reads the current state, and the step is clamped to t_end.  h holds the
+// device scalars just read; they are written back with the new dt.
+void host_timestep(dev::DevScalars &h)
+{
+    sync_mirror(false);
+    double dt = g_calc_timestep(h.t);
+    h.viscous = g_viscous_dt ? 1 : 0;
+    if (h.t + dt > h.t_end)
+    {
+        dt = h.t_end - h.t;
+        h.final_pending = 1;
+    }
+    h.dt = dt;
+    HIP_CHECK(hipMemcpyAsync(g_dev.sc, &h, sizeof(h), hipMemcpyHostToDevice, g_dev.stream));
+    HIP_CHECK(hipStreamSynchronize(g_dev.stream));
 }
 
 void init_scalars(int iter0, double t0, int max_iter, bool transient,
@@ -819,9 +1076,13 @@ void init_scalars(int iter0, double t0, int max_iter, bool transient,
     h.max_iter_abs = iter0 + max_iter;
     h.transient = transient;
     h.log_base = iter0;
+    h.host_dt = builtin_timestep() ? 0 : 1;
     HIP_CHECK(hipMemcpyAsync(g_dev.sc, &h, sizeof(h), hipMemcpyHostToDevice,
                              g_dev.stream));
-    // initial dt from the current state (primitives refreshed from U)
+    // initial dt from the current state (primitives refreshed from U); with
+    // a host time-step hook the host sets it before each step
+    if (h.host_dt)
+        return;
     dev::launch_dt_partials(g_dev.stream, g_dev.m, phys(), g_dev.U[g_dev.cur],
                             g_dev.partials, g_dev.partials_cap);
     dev::launch_finalize(g_dev.stream, g_dev.sc, g_dev.partials, dev::dt_partials_grid(g_dev.m), g_dev.fold, g_dev.fold_cap, phys(), 0, nullptr, 0);
@@ -863,6 +1124,8 @@ int run_explicit(int max_iter)
 
     int printed = 0;
     int status = 0;
+    // host hooks need the host to know t and dt at every step
+    const bool host_driven = !builtin_hooks() || !builtin_timestep();
     while (true)
     {
         int chunk = std::max(1, g_sync_interval);
@@ -872,8 +1135,17 @@ int run_explicit(int max_iter)
             int to_next = g_i_output_data - (it % g_i_output_data);
             chunk = std::min(chunk, to_next);
         }
-        for (int i = 0; i < chunk; ++i)
-            enqueue_explicit_step(g_dev.sc);
+        if (host_driven)
+        {
+            read_scalars();
+            dev::DevScalars h = *g_dev.sc_host;
+            if (!builtin_timestep())
+                host_timestep(h);
+            enqueue_explicit_step(g_dev.sc, nullptr, h.t, h.dt);
+        }
+        else
+            for (int i = 0; i < chunk; ++i)
+                enqueue_explicit_step(g_dev.sc);
         g_dev.r_valid = true;
         read_scalars();
         const dev::DevScalars &h = *g_dev.sc_host;
@@ -1043,7 +1315,10 @@ int run_implicit(int max_iter)
     int status = 0, printed = 0;
     while (true)
     {
-        const dev::DevScalars h0 = *g_dev.sc_host;
+        dev::DevScalars h0 = *g_dev.sc_host;
+        if (!builtin_timestep())
+            host_timestep(h0);
+        g_hook_time = h0.t + h0.dt;
         const int inner = implicit_step(h0.iter, h0.t, h0.dt, g_transient);
         g_dev.r_valid = true;
         imp::step_partials(g_dev.stream, g_dev.m, phys(), g_dev.U[g_dev.cur], g_dev.R,
@@ -1085,6 +1360,11 @@ int run_implicit(int max_iter)
 RuntimeView runtime_view()
 {
     require_ready();
+    if (!builtin_hooks())
+        throw std::runtime_error(
+            "the Newton-Krylov, learned-preconditioner and batched paths run the fused "
+            "device residual and need the built-in hooks (calc_ns_flux, ns_update, "
+            "ns_update_gradients)");
     ensure_gradient_buffers();
     RuntimeView v{};
     v.stream = g_dev.stream;
@@ -1300,6 +1580,14 @@ void init_finite_volume(void)
         solver_data = xalloc<data_t>(1);
         solver_data->phi_total = xalloc<double>((size_t)(N + B) * 10);
         solver_data->phi_dt = xalloc<double>((size_t)(N + B) * 5);
+        // finite_volume.c:39-52 (filled by finite_volume_time_derivative and
+        // before host hooks; zero pages until then)
+        solver_data->flux = xalloc<double>((size_t)F * 5);
+        solver_data->phi_total_left = xalloc<double>((size_t)F * 10);
+        solver_data->phi_total_right = xalloc<double>((size_t)F * 10);
+        solver_data->grad_phi_total_x = xalloc<double>((size_t)(N + B) * 10);
+        solver_data->grad_phi_total_y = xalloc<double>((size_t)(N + B) * 10);
+        solver_data->grad_phi_total_z = xalloc<double>((size_t)(N + B) * 10);
 
         HIP_CHECK(hipStreamCreateWithFlags(&g_dev.stream, hipStreamNonBlocking));
         dev::DevMesh &m = g_dev.m;
@@ -1425,6 +1713,7 @@ void init_finite_volume(void)
         g_dev.cur = 0;
         g_dev.vglob = solver_mesh->global_volume;
         g_dev.ready = true;
+        setup_tile_halo();
 
         // set_solution (finite_volume.c:189-201): exact function per domain
         // cell on the host, then the update hook's work on the device
@@ -1453,6 +1742,9 @@ void free_finite_volume(void)
     {
         std::free(solver_data->phi_total);
         std::free(solver_data->phi_dt);
+        std::free(solver_data->flux);
+        std::free(solver_data->phi_total_left);
+        std::free(solver_data->phi_total_right);
         std::free(solver_data->grad_phi_total_x);
         std::free(solver_data->grad_phi_total_y);
         std::free(solver_data->grad_phi_total_z);
@@ -1468,8 +1760,18 @@ void finite_volume_time_derivative(double time)
     try
     {
         require_ready();
-        enqueue_residual(g_dev.U[g_dev.cur], nullptr, g_dev.R, 0, false, false,
-                         nullptr);
+        g_hook_time = time;
+        if (builtin_hooks())
+        {
+            enqueue_residual(g_dev.U[g_dev.cur], nullptr, g_dev.R, 0, false, false,
+                             nullptr);
+            // the reference's residual also leaves its face states, gradients
+            // and fluxes in solver_data: the host-mirror passes fill them
+            ensure_mirror_buffers();
+            mirror_residual(time, g_dev.U[g_dev.cur], g_dev.mres, true);
+        }
+        else
+            mirror_residual(time, g_dev.U[g_dev.cur], g_dev.R, true);
         g_dev.r_valid = true;
         sync_mirror(true);
     }

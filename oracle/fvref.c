@@ -99,6 +99,7 @@ struct fvref
     double cfl_scale, dfl_scale, mu_mix, R_mix, Pr, kappa;
     double kappa_m1, kappa_p1, s_kappa, s_kappa_m1, s_kappa_p1;
     double cv, cp, kappa_pr, lambda;
+    double fixed_dt; /* > 0: a time-step hook returning this constant */
     int visc_dt_mode;
     int is_viscous_dt;
 
@@ -950,7 +951,9 @@ int fvref_timedisc_explicit(fvref_t *s, int scheme, int max_iter,
 
     while (1)
     {
-        double dt = calc_ns_timestep(s);
+        /* timedisc.c:164: the time-step hook (calc_ns_timestep, or a hook
+         * returning a constant) */
+        double dt = s->fixed_dt > 0.0 ? s->fixed_dt : calc_ns_timestep(s);
         if (t + dt > t_end_loc)
         {
             dt = t_end_loc - t;
@@ -1379,6 +1382,8 @@ void fvref_set_region(fvref_t *s, int region, int type, const double *prim5)
         phi[IP_T] = prim5[4];
     }
 }
+
+void fvref_set_fixed_dt(fvref_t *s, double dt) { s->fixed_dt = dt; }
 
 void fvref_set_state(fvref_t *s, const double *U)
 {

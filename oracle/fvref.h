@@ -70,6 +70,9 @@ void fvref_set_region(fvref_t *s, int region, int type, const double *prim5);
 /* Set conservative state of the domain cells (U[N*5]) and run the update hook
  * (con_to_prim + ghost cells), as set_solution does after the IC. */
 void fvref_set_state(fvref_t *s, const double *U);
+/* The time step of fvref_timedisc_explicit from a time-step hook that
+ * returns dt (> 0), instead of calc_ns_timestep (0). */
+void fvref_set_fixed_dt(fvref_t *s, double dt);
 
 /* One residual evaluation (finite_volume_time_derivative). */
 void fvref_time_derivative(fvref_t *s, double t);

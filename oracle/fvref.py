@@ -42,6 +42,7 @@ def lib():
     L.fvref_set_physics.argtypes = [c_void_p] + [c_double] * 6 + [c_int] * 4
     L.fvref_set_region.argtypes = [c_void_p, c_int, c_int, PD]
     L.fvref_set_state.argtypes = [c_void_p, PD]
+    L.fvref_set_fixed_dt.argtypes = [c_void_p, c_double]
     L.fvref_time_derivative.argtypes = [c_void_p, c_double]
     L.fvref_calc_timestep.restype = c_double
     L.fvref_calc_timestep.argtypes = [c_void_p, PI]
@@ -130,6 +131,10 @@ class Oracle:
     def set_region(self, region, type_name, prim5):
         p = np.ascontiguousarray(prim5 if prim5 is not None else [0.0] * 5, dtype=np.float64)
         self.L.fvref_set_region(self.h, region, BC[type_name], p.ctypes.data_as(PD))
+
+    def set_fixed_dt(self, dt):
+        """timedisc with a time-step hook returning dt (0: calc_ns_timestep)."""
+        self.L.fvref_set_fixed_dt(self.h, dt)
 
     def set_state(self, U):
         U = np.ascontiguousarray(U, dtype=np.float64)
