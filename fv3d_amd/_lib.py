@@ -99,6 +99,7 @@ SIGNATURES = [
     ("fvh_get_face_shift", c_int, [P_DOUBLE]),
     ("fvh_set_visc_dt_mode", c_int, [c_int]),
     ("fvh_set_ic", c_int, [c_int, P_DOUBLE, c_int]),
+    ("fvh_time_derivative_noalloc", c_int, [c_void_p, c_void_p, c_void_p]),
     ("fvh_exact_function_builtin", None, [c_int, c_double, P_DOUBLE, P_DOUBLE]),
     ("fvh_get_state", c_int, [P_DOUBLE]),
     ("fvh_set_state", c_int, [P_DOUBLE]),

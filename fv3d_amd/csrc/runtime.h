@@ -38,7 +38,13 @@ struct RuntimeView
 
 // Throws unless init_finite_volume ran with the built-in hooks and the
 // reconstruction and limiter are set.
-RuntimeView runtime_view();
+// allocate = false: nothing is allocated (the gradient and face-state
+// buffers may then be null).
+RuntimeView runtime_view(bool allocate = true);
+
+// Set-up of the allocation-free residual on the current device (the device
+// address of its scratch array); called by init_finite_volume.
+void noalloc_setup();
 
 // The explicit stepping buffers: the ping-pong state pair (U[*cur] is
 // current), the RK register k, the residual R and the LSERK tables of the

@@ -1357,15 +1357,16 @@ int run_implicit(int max_iter)
 
 } // namespace
 
-RuntimeView runtime_view()
+RuntimeView runtime_view(bool allocate)
 {
     require_ready();
     if (!builtin_hooks())
         throw std::runtime_error(
-            "the Newton-Krylov, learned-preconditioner and batched paths run the fused "
-            "device residual and need the built-in hooks (calc_ns_flux, ns_update, "
-            "ns_update_gradients)");
-    ensure_gradient_buffers();
+            "the Newton-Krylov, learned-preconditioner, batched and allocation-free paths "
+            "run the fused device residual and need the built-in hooks (calc_ns_flux, "
+            "ns_update, ns_update_gradients)");
+    if (allocate)
+        ensure_gradient_buffers();
     RuntimeView v{};
     v.stream = g_dev.stream;
     v.m = g_dev.m;
@@ -1713,6 +1714,7 @@ void init_finite_volume(void)
         g_dev.cur = 0;
         g_dev.vglob = solver_mesh->global_volume;
         g_dev.ready = true;
+        noalloc_setup();
         setup_tile_halo();
 
         // set_solution (finite_volume.c:189-201): exact function per domain

@@ -117,6 +117,20 @@ int fvh_last_status(int *iterations);
 int fvh_bench_explicit(int n_steps, int n_warmup, double *seconds,
                        double *k_us);
 
+/* ---- allocation-free residual ------------------------------------------- */
+/* dU/dt of the state U (device memory, [N*5] AoS, the domain cells) into R
+ * (device memory, [N*5]): finite_volume_time_derivative's arithmetic on the
+ * current mesh and physics, with the built-in hooks, enqueued on `stream`
+ * (a hipStream_t; NULL = the library's stream) without synchronising.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops).  It
+ * allocates nothing: the intermediate ghost rows, face states and viscous
+ * gradients live in a fixed 64 MiB device array (meshes up to about 190,000
+ * viscous second-order tetrahedra; larger meshes are rejected with
+ * FVH_ERR_STATE and a message).  Every call on a device shares that
+ * device's array: calls that overlap on one GPU race. */
+int fvh_time_derivative_noalloc(const double *U, double *R, void *stream);
+
 /* ---- device information ----------------------------------------------- */
 int fvh_device_info(char *name, int name_len, int *cu_count,
                     double *hbm_gbps_peak);
