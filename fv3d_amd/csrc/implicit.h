@@ -31,6 +31,16 @@ void jac_column(hipStream_t s, int N, const double *Rp, const double *Rm,
 void block_solve(hipStream_t s, int N, const double *jac, const double *f,
                  double *dY);
 
+// b_i = J_i x_i per cell (implicit.c:362-385, the reference's order)
+void block_matvec(hipStream_t s, int N, const double *jac, const double *x, double *b);
+// Element operations of the linear solvers: op 0 o1 = x / a; 1 o1 = x - a y;
+// 2 o1 = x + a (o1 - b y); 3 o1 += a x; 4 o1 += a x + b y and o2 = y - b z.
+void lsoe_elem(hipStream_t s, int n, int op, double a, double b, const double *x,
+               const double *y, const double *z, double *o1, double *o2);
+// x += sum_{i<k} y[i] V_i in order of i (y in device memory)
+void add_combination(hipStream_t s, int n, int k, const double *V, size_t ld, const double *y,
+                     double *x);
+
 // sums[k] = sum_i U_ik^2, k < 5 (fixed order); partials: kDotBlocks*64
 void column_sumsq(hipStream_t s, int N, const double *U, double *partials,
                   double *sums);

@@ -236,6 +236,97 @@ void jac_column(hipStream_t s, int N, const double *Rp, const double *Rm, int k,
                        Rm, k, fac, inv_dt, jac);
 }
 
+// ---- element kernels of the linear solvers (the oracle's expressions) ----
+__global__ __launch_bounds__(kBlock) void k_block_matvec(int N, const double *__restrict__ jac,
+                                                        const double *__restrict__ x,
+                                                        double *__restrict__ b)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N)
+        return;
+    const double *J = jac + (size_t)i * 25;
+    double xi[5], bi[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+        xi[k] = x[(size_t)i * 5 + k];
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+        bi[j] = J[j] * xi[0];
+#pragma unroll
+    for (int k = 1; k < 5; ++k)
+#pragma unroll
+        for (int j = 0; j < 5; ++j)
+            bi[j] += J[k * 5 + j] * xi[k];
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+        b[(size_t)i * 5 + j] = bi[j];
+}
+
+__global__ __launch_bounds__(kBlock) void k_lsoe_elem(int n, int op, double a, double b,
+                                                     const double *__restrict__ x,
+                                                     const double *__restrict__ y,
+                                                     const double *__restrict__ z,
+                                                     double *__restrict__ o1,
+                                                     double *__restrict__ o2)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n)
+        return;
+    switch (op)
+    {
+    case 0: // o1 = x / a
+        o1[q] = x[q] / a;
+        break;
+    case 1: // o1 = x - a y
+        o1[q] = x[q] - a * y[q];
+        break;
+    case 2: // p = r + beta (p - omega v): o1 = p, x = r, y = v
+        o1[q] = x[q] + a * (o1[q] - b * y[q]);
+        break;
+    case 3: // o1 += a x
+        o1[q] += a * x[q];
+        break;
+    default: // o1 += a x + b y; o2 = y - b z  (x += alpha p + omega s; r = s - omega t)
+        o1[q] += a * x[q] + b * y[q];
+        o2[q] = y[q] - b * z[q];
+        break;
+    }
+}
+
+// x += sum_i y[i] V_i, sequentially in i (y in device memory)
+__global__ __launch_bounds__(kBlock) void k_add_combination(int n, int k,
+                                                           const double *__restrict__ V,
+                                                           size_t ld, const double *__restrict__ y,
+                                                           double *__restrict__ x)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n)
+        return;
+    double a = x[q];
+    for (int i = 0; i < k; ++i)
+        a += y[i] * V[(size_t)i * ld + q];
+    x[q] = a;
+}
+
+void block_matvec(hipStream_t s, int N, const double *jac, const double *x, double *b)
+{
+    hipLaunchKernelGGL(k_block_matvec, dim3(blocks_for(N)), dim3(kBlock), 0, s, N, jac, x, b);
+}
+
+void lsoe_elem(hipStream_t s, int n, int op, double a, double b, const double *x,
+               const double *y, const double *z, double *o1, double *o2)
+{
+    hipLaunchKernelGGL(k_lsoe_elem, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, op, a, b, x, y,
+                       z, o1, o2);
+}
+
+void add_combination(hipStream_t s, int n, int k, const double *V, size_t ld, const double *y,
+                     double *x)
+{
+    hipLaunchKernelGGL(k_add_combination, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, k, V, ld,
+                       y, x);
+}
+
 void block_solve(hipStream_t s, int N, const double *jac, const double *f, double *dY)
 {
     hipLaunchKernelGGL(k_block_solve, dim3(blocks_for(N)), dim3(kBlock), 0, s, N, jac, f,

@@ -185,6 +185,8 @@ struct Device
     double *imp_f = nullptr, *imp_dY = nullptr, *imp_jac = nullptr;
     double *imp_Rp = nullptr, *imp_Rm = nullptr, *imp_Ut = nullptr;
     double *imp_red = nullptr; // reduction partials [1024*64] + results
+    double *imp_kv = nullptr;  // Krylov vectors of the linear solve
+    int imp_kv_cols = 0;
 };
 Device g_dev;
 
@@ -260,6 +262,9 @@ bool builtin_timestep() { return g_calc_timestep == calc_ns_timestep; }
 
 // Time passed to host hooks (the stage time of the current residual).
 double g_hook_time = 0.0;
+
+// Iterations of the last linear solve of the implicit step.
+int g_last_lsoe = 0;
 
 dev::PhysDev phys()
 {
@@ -1205,6 +1210,159 @@ void ensure_implicit_buffers()
     g_dev.imp_Rm = dmalloc<double>(n);
     g_dev.imp_Ut = dmalloc<double>(n);
     g_dev.imp_red = dmalloc<double>((size_t)1024 * 64 + 64);
+    // dY_n is the next linear solve's initial guess (implicit.c:285-300)
+    HIP_CHECK(hipMemset(g_dev.imp_dY, 0, n * sizeof(double)));
+}
+
+// Dots <a_j, b> for the columns a_j = A + j*ld, j < k, into host h.
+void device_dots(int n, int k, const double *A, size_t ld, const double *b, double *h)
+{
+    double *out = g_dev.imp_red + (size_t)1024 * 64;
+    kry::multidot<double>(g_dev.stream, n, k, A, ld, b, g_dev.imp_red, out);
+    HIP_CHECK(hipMemcpyAsync(h, out, (size_t)k * sizeof(double), hipMemcpyDeviceToHost,
+                             g_dev.stream));
+    HIP_CHECK(hipStreamSynchronize(g_dev.stream));
+}
+
+// The implicit step's linear solve J dY = f (implicit.c:285-300) on the
+// device, as the oracle restates basec's solve_gmres_n_m /
+// solve_bicgstab_n_m: restarted GMRES(max_krylov_dims) with classical
+// Gram-Schmidt applied twice and Givens rotations, or BiCGStab; dY holds
+// the previous solution as the initial guess; stops at |f - J dY| <= tol
+// or max_iter_lsoe iterations.  Returns the iterations.
+int lsoe_solve(double tol)
+{
+    hipStream_t s = g_dev.stream;
+    const int N = g_dev.m.N, n = 5 * N;
+    const size_t ld = ((size_t)n + 63) / 64 * 64;
+    const double *jac = g_dev.imp_jac, *f = g_dev.imp_f;
+    double *x = g_dev.imp_dY;
+    const int m = std::max(1, std::min(g_max_krylov_dims, kry::kMaxDots - 1));
+    const int cols = g_implicit_solver == BiCGStab ? 6 : m + 1;
+    if (g_dev.imp_kv_cols < cols)
+    {
+        g_dev.imp_kv = dmalloc<double>(cols * ld);
+        g_dev.imp_kv_cols = cols;
+    }
+    double *V = g_dev.imp_kv;
+    const int max_it = g_max_iter_lsoe;
+    int it = 0;
+    if (g_implicit_solver == BiCGStab)
+    {
+        double *r = V, *rh = V + ld, *p = V + 2 * ld, *v = V + 3 * ld, *sv = V + 4 * ld,
+               *t = V + 5 * ld;
+        imp::block_matvec(s, N, jac, x, r);
+        kry::rhs_minus<double>(s, n, f, r);
+        HIP_CHECK(hipMemcpyAsync(rh, r, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        HIP_CHECK(hipMemsetAsync(p, 0, n * sizeof(double), s));
+        HIP_CHECK(hipMemsetAsync(v, 0, n * sizeof(double), s));
+        double rho = 1.0, alpha = 1.0, omega = 1.0, d[2];
+        device_dots(n, 1, r, ld, r, d);
+        double rn = std::sqrt(d[0]);
+        while (rn > tol && it < max_it)
+        {
+            ++it;
+            device_dots(n, 1, rh, ld, r, d);
+            const double rho1 = d[0];
+            const double beta = (rho1 / rho) * (alpha / omega);
+            imp::lsoe_elem(s, n, 2, beta, omega, r, v, nullptr, p, nullptr);
+            imp::block_matvec(s, N, jac, p, v);
+            device_dots(n, 1, rh, ld, v, d);
+            alpha = rho1 / d[0];
+            imp::lsoe_elem(s, n, 1, alpha, 0.0, r, v, nullptr, sv, nullptr);
+            device_dots(n, 1, sv, ld, sv, d);
+            const double sn = std::sqrt(d[0]);
+            if (sn <= tol)
+            {
+                imp::lsoe_elem(s, n, 3, alpha, 0.0, p, nullptr, nullptr, x, nullptr);
+                break;
+            }
+            imp::block_matvec(s, N, jac, sv, t);
+            device_dots(n, 1, t, ld, sv, d);
+            device_dots(n, 1, t, ld, t, d + 1);
+            omega = d[0] / d[1];
+            imp::lsoe_elem(s, n, 4, alpha, omega, p, sv, t, x, r);
+            device_dots(n, 1, r, ld, r, d);
+            rn = std::sqrt(d[0]);
+            rho = rho1;
+        }
+        HIP_CHECK(hipStreamSynchronize(s));
+        return it;
+    }
+    std::vector<double> H((size_t)(m + 1) * m, 0.0), g(m + 1), cs(m), sn(m), y(m),
+        h1(m + 1), h2(m + 1);
+    double *yd = g_dev.imp_red + (size_t)1024 * 64; // reuses the dot results slot
+    for (int cycle = 0; cycle <= g_max_krylov_restarts; ++cycle)
+    {
+        imp::block_matvec(s, N, jac, x, V);
+        kry::rhs_minus<double>(s, n, f, V);
+        double d;
+        device_dots(n, 1, V, ld, V, &d);
+        const double beta = std::sqrt(d);
+        if (beta <= tol || it >= max_it)
+            break;
+        imp::lsoe_elem(s, n, 0, beta, 0.0, V, nullptr, nullptr, V, nullptr);
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        int k = 0;
+        double resid = beta;
+        for (int j = 0; j < m && it < max_it; ++j)
+        {
+            double *vj1 = V + (size_t)(j + 1) * ld;
+            imp::block_matvec(s, N, jac, V + (size_t)j * ld, vj1);
+            ++it;
+            for (int pass = 0; pass < 2; ++pass)
+            {
+                std::vector<double> &h = pass == 0 ? h1 : h2;
+                device_dots(n, j + 1, V, ld, vj1, h.data());
+                HIP_CHECK(hipMemcpyAsync(yd, h.data(), (size_t)(j + 1) * sizeof(double),
+                                         hipMemcpyHostToDevice, s));
+                kry::multiaxpy<double>(s, n, j + 1, V, ld, yd, vj1);
+            }
+            for (int i = 0; i <= j; ++i)
+                H[(size_t)i * m + j] = h1[i] + h2[i];
+            device_dots(n, 1, vj1, ld, vj1, &d);
+            const double hn = std::sqrt(d);
+            H[(size_t)(j + 1) * m + j] = hn;
+            if (hn > 0.0)
+                imp::lsoe_elem(s, n, 0, hn, 0.0, vj1, nullptr, nullptr, vj1, nullptr);
+            for (int i = 0; i < j; ++i)
+            {
+                const double a = H[(size_t)i * m + j], c = H[(size_t)(i + 1) * m + j];
+                H[(size_t)i * m + j] = cs[i] * a + sn[i] * c;
+                H[(size_t)(i + 1) * m + j] = -sn[i] * a + cs[i] * c;
+            }
+            const double a = H[(size_t)j * m + j], c = H[(size_t)(j + 1) * m + j];
+            const double r = std::sqrt(a * a + c * c);
+            cs[j] = r > 0.0 ? a / r : 1.0;
+            sn[j] = r > 0.0 ? c / r : 0.0;
+            H[(size_t)j * m + j] = r;
+            H[(size_t)(j + 1) * m + j] = 0.0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            resid = std::fabs(g[j + 1]);
+            k = j + 1;
+            if (resid <= tol)
+                break;
+        }
+        for (int i = k - 1; i >= 0; --i)
+        {
+            double v = g[i];
+            for (int q = i + 1; q < k; ++q)
+                v -= H[(size_t)i * m + q] * y[q];
+            y[i] = v / H[(size_t)i * m + i];
+        }
+        if (k > 0)
+        {
+            HIP_CHECK(hipMemcpyAsync(yd, y.data(), (size_t)k * sizeof(double),
+                                     hipMemcpyHostToDevice, s));
+            imp::add_combination(s, n, k, V, ld, yd, x);
+        }
+        if (resid <= tol || it >= max_it)
+            break;
+    }
+    HIP_CHECK(hipStreamSynchronize(s));
+    return it;
 }
 
 double device_norm2(const double *v, int n)
@@ -1265,7 +1423,7 @@ int implicit_step(int iter, double t, double dt, bool transient)
         }
         if (e_old >= e0)
         {
-            imp::block_solve(s, N, g_dev.imp_jac, g_dev.imp_f, g_dev.imp_dY);
+            g_last_lsoe = lsoe_solve(g_tolerance_lsoe * e_old);
             kry::update(s, n, U, g_dev.imp_dY, 1.0);
             eval_residual(U, g_dev.R);
             imp::bdf_rhs(s, n, U, old[0], old[1], n_loc, a0, a1, b, dt, g_dev.R,
@@ -1338,7 +1496,7 @@ int run_implicit(int max_iter)
         const int it = iter0 + printed + 1;
         const bool last = h.done > 0;
         const bool out = last || (g_i_output_data > 0 && it % g_i_output_data == 0);
-        print_row(it, &g_log[(size_t)printed * 8], out, inner, 1);
+        print_row(it, &g_log[(size_t)printed * 8], out, inner, g_last_lsoe);
         for (int j = 0; j < 5; ++j)
             g_residual[j] = g_log[(size_t)printed * 8 + 3 + j];
         g_viscous_dt = g_log[(size_t)printed * 8 + 2] != 0.0;
@@ -1898,6 +2056,8 @@ void init_implicit(implicit_scheme_t implicit_scheme,
         fatal("init_implicit: unknown scheme");
     if (implicit_solver != BiCGStab && implicit_solver != GMRes)
         fatal("init_implicit: unknown solver");
+    if (max_krylov_dims < 1 || max_krylov_dims > 63)
+        fatal("init_implicit: max_krylov_dims must be in 1..63");
     g_bdf_scheme = implicit_scheme;
     g_implicit_solver = implicit_solver;
     g_max_iter_inner = max_iter_inner;

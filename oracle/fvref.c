@@ -100,6 +100,14 @@ struct fvref
     double kappa_m1, kappa_p1, s_kappa, s_kappa_m1, s_kappa_p1;
     double cv, cp, kappa_pr, lambda;
     double fixed_dt; /* > 0: a time-step hook returning this constant */
+    /* the implicit step's linear solve (implicit.c:285-300): solver 0
+     * BiCGStab, 1 GMRES(dims) with restarts, -1 the exact block solve;
+     * tolerance relative to |f|, iteration limit; dY is kept across calls
+     * (the reference's dY_n buffer is the next solve's initial guess) */
+    int lsoe_solver, lsoe_max_iter, lsoe_dims, lsoe_restarts;
+    double lsoe_tol;
+    double *lsoe_dY;
+    int lsoe_last_iter;
     int visc_dt_mode;
     int is_viscous_dt;
 
@@ -1043,6 +1051,177 @@ static void solve_block5(const double *jac, const double *rhs, double *x)
     }
 }
 
+/* b_i = J_i x_i, J_i column-major (implicit.c:362-385, its order) */
+static void block_matvec(const double *jac, int n, const double *x, double *b)
+{
+    for (int i = 0; i < n; ++i)
+    {
+        const double *J = &jac[i * 25];
+        const double *xi = &x[i * NSOL];
+        double *bi = &b[i * NSOL];
+        for (int j = 0; j < NSOL; ++j)
+            bi[j] = J[j] * xi[0];
+        for (int k = 1; k < NSOL; ++k)
+            for (int j = 0; j < NSOL; ++j)
+                bi[j] += J[k * NSOL + j] * xi[k];
+    }
+}
+
+/* The linear solvers behind basec's solve_gmres_n_m / solve_bicgstab_n_m
+ * (polynomialchaos/basec, version unpinned, not in the reference tree), as
+ * the reference calls them (implicit.c:285-300): b = f, x = dY (its previous
+ * content is the initial guess), *res in = the absolute tolerance
+ * tolerance_lsoe * |f|, *n_iter in = max_iter_lsoe; out: the iterations
+ * done and the final residual norm.  basec's own iteration is not
+ * available here, so these are the textbook methods:
+ *   GMRES(m) with restarts: Saad, Iterative Methods for Sparse Linear
+ *     Systems, 2nd ed., Algorithm 6.9 with Givens rotations (Section 6.5.3),
+ *     Arnoldi by classical Gram-Schmidt applied twice; at most restarts + 1
+ *     cycles of m steps and *n_iter steps in total; the residual is
+ *     recomputed at the start of each cycle;
+ *   BiCGStab: van der Vorst (1992), SIAM J. Sci. Stat. Comput. 13:631-644,
+ *     Algorithm 1, r_hat = r_0, stopping on |r| or |s| <= tolerance. */
+static void gmres_nm(const double *jac, int n, const double *b, double *x, int m,
+                     int restarts, int *n_iter, double *res)
+{
+    const int nv = n * NSOL;
+    const double tol = *res;
+    const int max_it = *n_iter;
+    double *V = xcalloc((size_t)(m + 1) * nv, sizeof(double));
+    double *w = xcalloc((size_t)nv, sizeof(double));
+    double *H = xcalloc((size_t)(m + 1) * m, sizeof(double));
+    double *g = xcalloc((size_t)m + 1, sizeof(double));
+    double *cs = xcalloc((size_t)m, sizeof(double));
+    double *sn = xcalloc((size_t)m, sizeof(double));
+    double *y = xcalloc((size_t)m, sizeof(double));
+    double h1[64], h2[64];
+    int it = 0;
+    double resid = 0.0;
+    for (int cycle = 0; cycle <= restarts; ++cycle)
+    {
+        block_matvec(jac, n, x, w);
+        for (int q = 0; q < nv; ++q)
+            w[q] = b[q] - w[q];
+        const double beta = sqrt(dot_n(w, w, nv));
+        resid = beta;
+        if (beta <= tol || it >= max_it)
+            break;
+        for (int q = 0; q < nv; ++q)
+            V[q] = w[q] / beta;
+        for (int i = 0; i <= m; ++i)
+            g[i] = 0.0;
+        g[0] = beta;
+        int k = 0;
+        for (int j = 0; j < m && it < max_it; ++j)
+        {
+            double *vj1 = &V[(size_t)(j + 1) * nv];
+            block_matvec(jac, n, &V[(size_t)j * nv], vj1);
+            ++it;
+            for (int pass = 0; pass < 2; ++pass)
+            {
+                double *h = pass == 0 ? h1 : h2;
+                for (int i = 0; i <= j; ++i)
+                    h[i] = dot_n(&V[(size_t)i * nv], vj1, nv);
+                for (int i = 0; i <= j; ++i)
+                    for (int q = 0; q < nv; ++q)
+                        vj1[q] -= h[i] * V[(size_t)i * nv + q];
+            }
+            for (int i = 0; i <= j; ++i)
+                H[i * m + j] = h1[i] + h2[i];
+            const double hn = sqrt(dot_n(vj1, vj1, nv));
+            H[(j + 1) * m + j] = hn;
+            if (hn > 0.0)
+                for (int q = 0; q < nv; ++q)
+                    vj1[q] = vj1[q] / hn;
+            for (int i = 0; i < j; ++i)
+            {
+                const double a = H[i * m + j], c = H[(i + 1) * m + j];
+                H[i * m + j] = cs[i] * a + sn[i] * c;
+                H[(i + 1) * m + j] = -sn[i] * a + cs[i] * c;
+            }
+            const double a = H[j * m + j], c = H[(j + 1) * m + j];
+            const double r = sqrt(a * a + c * c);
+            cs[j] = r > 0.0 ? a / r : 1.0;
+            sn[j] = r > 0.0 ? c / r : 0.0;
+            H[j * m + j] = r;
+            H[(j + 1) * m + j] = 0.0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            resid = fabs(g[j + 1]);
+            k = j + 1;
+            if (resid <= tol)
+                break;
+        }
+        for (int i = k - 1; i >= 0; --i)
+        {
+            double v = g[i];
+            for (int q = i + 1; q < k; ++q)
+                v -= H[i * m + q] * y[q];
+            y[i] = v / H[i * m + i];
+        }
+        for (int i = 0; i < k; ++i)
+            for (int q = 0; q < nv; ++q)
+                x[q] += y[i] * V[(size_t)i * nv + q];
+        if (resid <= tol || it >= max_it)
+            break;
+    }
+    *n_iter = it;
+    *res = resid;
+    free(V), free(w), free(H), free(g), free(cs), free(sn), free(y);
+}
+
+static void bicgstab_nm(const double *jac, int n, const double *b, double *x, int *n_iter,
+                        double *res)
+{
+    const int nv = n * NSOL;
+    const double tol = *res;
+    const int max_it = *n_iter;
+    double *r = xcalloc((size_t)nv, sizeof(double)), *rh = xcalloc((size_t)nv, sizeof(double));
+    double *p = xcalloc((size_t)nv, sizeof(double)), *v = xcalloc((size_t)nv, sizeof(double));
+    double *sv = xcalloc((size_t)nv, sizeof(double)), *t = xcalloc((size_t)nv, sizeof(double));
+    block_matvec(jac, n, x, r);
+    for (int q = 0; q < nv; ++q)
+    {
+        r[q] = b[q] - r[q];
+        rh[q] = r[q];
+    }
+    double rho = 1.0, alpha = 1.0, omega = 1.0;
+    double rn = sqrt(dot_n(r, r, nv));
+    int it = 0;
+    while (rn > tol && it < max_it)
+    {
+        ++it;
+        const double rho1 = dot_n(rh, r, nv);
+        const double beta = (rho1 / rho) * (alpha / omega);
+        for (int q = 0; q < nv; ++q)
+            p[q] = r[q] + beta * (p[q] - omega * v[q]);
+        block_matvec(jac, n, p, v);
+        alpha = rho1 / dot_n(rh, v, nv);
+        for (int q = 0; q < nv; ++q)
+            sv[q] = r[q] - alpha * v[q];
+        const double sn = sqrt(dot_n(sv, sv, nv));
+        if (sn <= tol)
+        {
+            for (int q = 0; q < nv; ++q)
+                x[q] += alpha * p[q];
+            rn = sn;
+            break;
+        }
+        block_matvec(jac, n, sv, t);
+        omega = dot_n(t, sv, nv) / dot_n(t, t, nv);
+        for (int q = 0; q < nv; ++q)
+        {
+            x[q] += alpha * p[q] + omega * sv[q];
+            r[q] = sv[q] - omega * t[q];
+        }
+        rn = sqrt(dot_n(r, r, nv));
+        rho = rho1;
+    }
+    *n_iter = it;
+    *res = rn;
+    free(r), free(rh), free(p), free(v), free(sv), free(t);
+}
+
 /* f = -(Y - old_0)/dt + b R - sum_s a_s/dt old_s  (implicit.c:254-270) */
 static void bdf_rhs(int nv, const double *Y, double *const *old, int n_loc,
                     const double *a, double b, double dt, const double *R,
@@ -1136,8 +1315,24 @@ static int timestep_newton(fvref_t *s, int scheme, int iter, double t,
 
         if (e_old >= e0)
         {
-            for (int i = 0; i < n; ++i)
-                solve_block5(&jac[i * 25], &f[i * NSOL], &dY[i * NSOL]);
+            if (s->lsoe_solver < 0)
+                for (int i = 0; i < n; ++i)
+                    solve_block5(&jac[i * 25], &f[i * NSOL], &dY[i * NSOL]);
+            else
+            {
+                /* implicit.c:285-300: initial guess the previous dY */
+                if (!s->lsoe_dY)
+                    s->lsoe_dY = xcalloc((size_t)nv, sizeof(double));
+                copy_n(s->lsoe_dY, nv, dY);
+                int n_it = s->lsoe_max_iter;
+                double res = s->lsoe_tol * e_old;
+                if (s->lsoe_solver == 0)
+                    bicgstab_nm(jac, n, f, dY, &n_it, &res);
+                else
+                    gmres_nm(jac, n, f, dY, s->lsoe_dims, s->lsoe_restarts, &n_it, &res);
+                copy_n(dY, nv, s->lsoe_dY);
+                s->lsoe_last_iter = n_it;
+            }
             for (int i = 0; i < n; ++i)
                 for (int j = 0; j < NSOL; ++j)
                 {
@@ -1271,6 +1466,13 @@ fvref_t *fvref_create(int n_cells, int max_cell_faces, int n_faces,
                       const int *region_is_boundary)
 {
     fvref_t *s = xcalloc(1, sizeof(fvref_t));
+    /* the reference's TimeDisc/Implicit defaults (src/FV3D/src/timedisc/
+     * implicit.c:12-18): GMRES(15), 2 restarts, 1e-12, 100 iterations */
+    s->lsoe_solver = 1;
+    s->lsoe_tol = 1e-12;
+    s->lsoe_max_iter = 100;
+    s->lsoe_dims = 15;
+    s->lsoe_restarts = 2;
     s->n_cells = n_cells;
     s->max_cell_faces = max_cell_faces;
     s->n_faces = n_faces;
@@ -1326,7 +1528,7 @@ void fvref_destroy(fvref_t *s)
                     s->boundary_faces, s->boundary_id, s->boundary_face,
                     s->region_is_boundary, s->region_type, s->region_phi,
                     s->flux, s->phi_left, s->phi_right, s->grad_x, s->grad_y,
-                    s->grad_z, s->phi_dt, s->phi_total};
+                    s->grad_z, s->phi_dt, s->phi_total, s->lsoe_dY};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i)
         free(ptrs[i]);
     free(s);
@@ -1384,6 +1586,20 @@ void fvref_set_region(fvref_t *s, int region, int type, const double *prim5)
 }
 
 void fvref_set_fixed_dt(fvref_t *s, double dt) { s->fixed_dt = dt; }
+
+void fvref_set_lsoe(fvref_t *s, int solver, double tol, int max_iter, int dims,
+                    int restarts)
+{
+    s->lsoe_solver = solver;
+    s->lsoe_tol = tol;
+    s->lsoe_max_iter = max_iter;
+    s->lsoe_dims = dims < 1 ? 1 : (dims > 64 ? 64 : dims);
+    s->lsoe_restarts = restarts;
+    free(s->lsoe_dY);
+    s->lsoe_dY = NULL;
+}
+
+int fvref_lsoe_iterations(const fvref_t *s) { return s->lsoe_last_iter; }
 
 void fvref_set_state(fvref_t *s, const double *U)
 {

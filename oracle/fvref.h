@@ -96,14 +96,24 @@ double fvref_lserk_amplification(int scheme, double z);
 
 /* Reference-semantics implicit step (timestep_newton): BDF1/BDF2 with the
  * lumped finite-difference block Jacobian; the block-diagonal linear system
- * is solved exactly per cell (the reference hands it to an external Krylov
- * solver, see fvref.c).  old_levels [2*N*5] is the BDF history (phi_old[0],
+ * is solved by the Krylov method set with fvref_set_lsoe (the reference's
+ * external basec solvers, restated in fvref.c).  old_levels [2*N*5] is the BDF history (phi_old[0],
  * phi_old[1]) the caller keeps across steps; phi_dt must hold R of the
  * current state.  Returns the inner iterations (-2: max_iter_inner
  * reached, where the reference aborts). */
 int fvref_timestep_newton(fvref_t *s, int scheme, int iter, double t,
                           double dt, int transient, int max_iter_inner,
                           double *old_levels, double *jac_out);
+
+/* The implicit step's linear solve (implicit.c:285-300): solver 0
+ * BiCGStab, 1 restarted GMRES(dims) with `restarts` restarts (the default,
+ * with tol 1e-12, max_iter 100, dims 15, restarts 2), -1 the exact per-cell
+ * block solve.  The solve stops at |f - J dY| <= tol |f| or max_iter
+ * iterations; its previous solution is the initial guess of the next one.
+ * fvref_lsoe_iterations: the iterations of the last solve. */
+void fvref_set_lsoe(fvref_t *s, int solver, double tol, int max_iter, int dims,
+                    int restarts);
+int fvref_lsoe_iterations(const fvref_t *s);
 
 /* timedisc() with the implicit step (scheme 0 Euler, 1 BDF2); log rows as
  * fvref_timedisc_explicit.  Returns the steps done, -1 on NaN/Inf, -2 when

@@ -43,6 +43,9 @@ def lib():
     L.fvref_set_region.argtypes = [c_void_p, c_int, c_int, PD]
     L.fvref_set_state.argtypes = [c_void_p, PD]
     L.fvref_set_fixed_dt.argtypes = [c_void_p, c_double]
+    L.fvref_set_lsoe.argtypes = [c_void_p, c_int, c_double, c_int, c_int, c_int]
+    L.fvref_lsoe_iterations.restype = c_int
+    L.fvref_lsoe_iterations.argtypes = [c_void_p]
     L.fvref_time_derivative.argtypes = [c_void_p, c_double]
     L.fvref_calc_timestep.restype = c_double
     L.fvref_calc_timestep.argtypes = [c_void_p, PI]
@@ -135,6 +138,14 @@ class Oracle:
     def set_fixed_dt(self, dt):
         """timedisc with a time-step hook returning dt (0: calc_ns_timestep)."""
         self.L.fvref_set_fixed_dt(self.h, dt)
+
+    def set_lsoe(self, solver="GMRes", tol=1e-12, max_iter=100, dims=15, restarts=2):
+        """Linear solve of the implicit step: "GMRes", "BiCGStab" or "exact"."""
+        self.L.fvref_set_lsoe(self.h, {"BiCGStab": 0, "GMRes": 1, "exact": -1}[solver], tol,
+                              max_iter, dims, restarts)
+
+    def lsoe_iterations(self):
+        return self.L.fvref_lsoe_iterations(self.h)
 
     def set_state(self, U):
         U = np.ascontiguousarray(U, dtype=np.float64)

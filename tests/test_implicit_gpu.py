@@ -1,7 +1,8 @@
 """GPU parity of the reference-semantics implicit mode (timedisc with
-init_implicit: BDF1/BDF2, lumped finite-difference block Jacobian, block
-solve; src/Solver/src/timedisc/implicit.c) against the oracle, through the
-C-ABI.
+init_implicit: BDF1/BDF2, lumped finite-difference block Jacobian, the
+linear solve by restarted GMRES or BiCGStab with the TimeDisc/Implicit
+tolerance, iteration and Krylov-dimension parameters; src/Solver/src/
+timedisc/implicit.c) against the oracle, through the C-ABI.
 
 Bar: 1e-12 relative per component (as the explicit path) for the state
 after N steps on the periodic density wave, dt and time to 1e-13, the
@@ -63,6 +64,7 @@ def test_channel_walls_implicit_steady():
     assert rc == 0 and n == 3
     Uo = o.state()
     rng = np.random.default_rng(0)
+    o.set_lsoe()  # the same initial guess of the first linear solve
     o.set_state((U0 * (1 + 2.0 ** -52 * rng.choice([-1, 0, 1], U0.shape))).reshape(-1))
     o.timedisc_implicit("Euler", 3, False, 1e-10, 0.0, 1e300)
     sens = rel_err(o.state(), Uo)
@@ -70,4 +72,35 @@ def test_channel_walls_implicit_steady():
     _print("implicit channel", rel=e, oracle_ulp_sensitivity=sens)
     assert e <= max(TOL, 20 * sens)
     assert np.allclose(log_g[:, 3:8], log_o[:, 3:8], rtol=max(1e-11, 20 * sens), atol=0)
+    case.close()
+
+
+@pytest.mark.parametrize("solver,tol,dims,restarts", [
+    ("GMRes", 1e-4, 5, 1), ("GMRes", 1e-12, 15, 2), ("BiCGStab", 1e-4, 15, 2)])
+def test_density_wave_implicit_krylov_parameters(solver, tol, dims, restarts):
+    """The linear solve honours tolerance_lsoe, max_krylov_dims and
+    max_krylov_restarts: with a loose tolerance the step differs from the
+    exact block solve by up to about the tolerance, and the device matches
+    the oracle run with the same parameters.  The dots of the Krylov method
+    are sums over all cells, which the device adds in a different order, so
+    the bar is a few orders of magnitude below the solve's own tolerance."""
+    cfg = _implicit(configs.density_wave_box(n=6, steps=4, flux="Rusanov"), "BDF2", 4)
+    cfg["physics"]["cfl_scale"] = 2.0
+    cfg["numerics"].update(solver=solver, tolerance_lsoe=tol, max_krylov_dims=dims,
+                           max_krylov_restarts=restarts, max_iter_lsoe=100)
+    case, o, _ = make_pair(cfg)
+    U0 = case.state()
+    o.set_lsoe(solver, tol=tol, max_iter=100, dims=dims, restarts=restarts)
+    rc, log_g = case.run()
+    n, log_o = o.timedisc_implicit("BDF2", 4, True, 1e-10, 0.0, 1e300)
+    assert rc == 0 and n == 4
+    e = rel_err(case.state(), o.state())
+    o.set_lsoe("exact")
+    o.set_state(U0.reshape(-1))
+    o.timedisc_implicit("BDF2", 4, True, 1e-10, 0.0, 1e300)
+    e_exact = rel_err(case.state(), o.state())
+    _print("implicit krylov " + solver, tol=tol, rel_vs_oracle=e, rel_vs_exact_solve=e_exact)
+    assert e <= max(1e-12, 1e-3 * tol)
+    if tol > 1e-8:
+        assert e_exact > 10 * e
     case.close()

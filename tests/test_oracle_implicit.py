@@ -80,9 +80,12 @@ def test_bdf2_history_single_steps_equal_driver():
     o.set_physics(cfl_scale=1.0, mu_mix=0.0, flux="Rusanov", visc_dt_mode=1)
     U0 = _density_wave(o, mesh)
     N = mesh["n_cells"]
+    o.set_lsoe()
     o.set_state(U0.reshape(-1))
     n, log = o.timedisc_implicit("BDF2", 3, True, 1e-10, 0.0, 1e300)
     U_driver = o.state()
+    # the linear solves start from the previous solve's result: reset it
+    o.set_lsoe()
     o.set_state(U0.reshape(-1))
     o.time_derivative()
     old = np.zeros((2, N * 5))
@@ -95,3 +98,115 @@ def test_bdf2_history_single_steps_equal_driver():
     assert np.array_equal(o.state(), U_driver)
     # the history holds the last two step-start states
     assert not np.array_equal(old[0], old[1])
+
+
+# ---- the linear solve of the step (implicit.c:285-300) ---------------------
+def _one_step(o, U0, scheme="Euler"):
+    """One steady-mode implicit Euler step from U0 (one inner iteration, so
+    the linear system is J dY = f with f = R(U0)); returns f, J, dY."""
+    N = U0.shape[0]
+    o.set_state(U0.reshape(-1))
+    f = o.time_derivative().reshape(-1)
+    old = np.zeros((2, N * 5))
+    dt, _ = o.timestep()
+    k, jac = o.timestep_newton(scheme, 0, 0.0, dt, old, transient=False, want_jac=True)
+    assert k == 1
+    J = jac.reshape(N, 5, 5).transpose(0, 2, 1)  # J[i, row, col]
+    dY = (o.state() - U0).reshape(-1)
+    return f, J, dY
+
+
+def _apply(J, x):
+    return np.einsum("irc,ic->ir", J, x.reshape(-1, 5)).reshape(-1)
+
+
+@pytest.mark.parametrize("solver", ["GMRes", "BiCGStab"])
+def test_krylov_tight_tolerance_equals_block_solve(solver):
+    cfg = configs.density_wave_box(n=5, steps=1)
+    o, mesh = _oracle_on(cfg)
+    o.set_physics(cfl_scale=5.0, mu_mix=0.0, flux="Rusanov", visc_dt_mode=1)
+    U0 = _density_wave(o, mesh, amp=0.1)
+    o.set_lsoe(solver, tol=1e-14, max_iter=400, dims=30, restarts=10)
+    f, J, dY = _one_step(o, U0)
+    exact = np.linalg.solve(J, f.reshape(-1, 5)[..., None])[..., 0].reshape(-1)
+    assert np.abs(dY - exact).max() <= 1e-10 * np.abs(exact).max()
+    assert o.lsoe_iterations() > 1
+    # the exact per-cell solve gives the same step
+    o.set_lsoe("exact")
+    _, _, dY2 = _one_step(o, U0)
+    assert np.abs(dY2 - exact).max() <= 1e-12 * np.abs(exact).max()
+
+
+def test_gmres_one_step_is_the_minimal_residual_step():
+    """GMRES(1), one iteration from x0 = 0: x = alpha f with alpha =
+    <J f, f> / <J f, J f>, the minimiser of |f - alpha J f|."""
+    cfg = configs.density_wave_box(n=4, steps=1)
+    o, mesh = _oracle_on(cfg)
+    o.set_physics(cfl_scale=5.0, mu_mix=0.0, flux="Rusanov", visc_dt_mode=1)
+    U0 = _density_wave(o, mesh, amp=0.1)
+    o.set_lsoe("GMRes", tol=1e-300, max_iter=1, dims=1, restarts=0)
+    f, J, dY = _one_step(o, U0)
+    Jf = _apply(J, f)
+    alpha = Jf @ f / (Jf @ Jf)
+    assert np.allclose(dY, alpha * f, rtol=1e-11, atol=1e-14 * np.abs(alpha * f).max())
+    assert o.lsoe_iterations() == 1
+
+
+def test_bicgstab_one_iteration_closed_form():
+    """BiCGStab (van der Vorst 1992), one iteration from x0 = 0, r_hat = r0 =
+    f: alpha = <f, f>/<f, J f>, s = f - alpha J f, omega = <J s, s>/<J s, J s>,
+    x = alpha f + omega s."""
+    cfg = configs.density_wave_box(n=4, steps=1)
+    o, mesh = _oracle_on(cfg)
+    o.set_physics(cfl_scale=5.0, mu_mix=0.0, flux="Rusanov", visc_dt_mode=1)
+    U0 = _density_wave(o, mesh, amp=0.1)
+    o.set_lsoe("BiCGStab", tol=1e-300, max_iter=1)
+    f, J, dY = _one_step(o, U0)
+    Jf = _apply(J, f)
+    alpha = (f @ f) / (f @ Jf)
+    s = f - alpha * Jf
+    Js = _apply(J, s)
+    omega = (Js @ s) / (Js @ Js)
+    expect = alpha * f + omega * s
+    assert np.allclose(dY, expect, rtol=1e-11, atol=1e-14 * np.abs(expect).max())
+
+
+@pytest.mark.parametrize("solver", ["GMRes", "BiCGStab"])
+def test_krylov_loose_tolerance_stops_early(solver):
+    """tolerance_lsoe is relative to |f|: the solve stops as soon as
+    |f - J dY| <= tol |f|, in fewer iterations than the tight solve."""
+    cfg = configs.density_wave_box(n=5, steps=1)
+    o, mesh = _oracle_on(cfg)
+    o.set_physics(cfl_scale=20.0, mu_mix=0.0, flux="Rusanov", visc_dt_mode=1)
+    U0 = _density_wave(o, mesh, amp=0.1)
+    o.set_lsoe(solver, tol=1e-3, max_iter=400, dims=30, restarts=10)
+    f, J, dY = _one_step(o, U0)
+    loose = o.lsoe_iterations()
+    r = f - _apply(J, dY)
+    assert np.linalg.norm(r) <= 1e-3 * np.linalg.norm(f) * (1 + 1e-9)
+    o.set_lsoe(solver, tol=1e-13, max_iter=400, dims=30, restarts=10)
+    _one_step(o, U0)
+    assert loose < o.lsoe_iterations()
+
+
+def test_krylov_iteration_limit_and_initial_guess():
+    """max_iter_lsoe bounds the iterations, and each solve starts from the
+    previous solve's result (the reference keeps dY_n between calls): two
+    GMRES(1) single iterations from the same state give x1 = a1 f, then
+    x2 = x1 + a2 r1 with r1 = f - J x1, a2 = <J r1, r1> / <J r1, J r1>."""
+    cfg = configs.density_wave_box(n=4, steps=1)
+    o, mesh = _oracle_on(cfg)
+    o.set_physics(cfl_scale=5.0, mu_mix=0.0, flux="Rusanov", visc_dt_mode=1)
+    U0 = _density_wave(o, mesh, amp=0.1)
+    o.set_lsoe("GMRes", tol=1e-300, max_iter=0, dims=15, restarts=2)
+    _, _, dY0 = _one_step(o, U0)
+    assert o.lsoe_iterations() == 0 and np.all(dY0 == 0.0)
+    o.set_lsoe("GMRes", tol=1e-300, max_iter=1, dims=1, restarts=0)
+    f, J, x1 = _one_step(o, U0)
+    _, _, x2 = _one_step(o, U0)
+    r1 = f - _apply(J, x1)
+    Jr = _apply(J, r1)
+    a2 = Jr @ r1 / (Jr @ Jr)
+    expect = x1 + a2 * r1
+    assert not np.allclose(x2, x1, rtol=1e-6, atol=0)
+    assert np.allclose(x2, expect, rtol=1e-9, atol=1e-13 * np.abs(expect).max())
