@@ -175,16 +175,19 @@ void launch_residual(hipStream_t s, const DevMesh &m, const PhysDev &P,
 // partials of cell_blocks(N) workgroups) from the residual R [N][5].
 void launch_stage_update(hipStream_t s, const DevMesh &m, const PhysDev &P,
                          const StageParams &sp, const double *R);
+// prims: the primitive rows [N][5] to use with U's density (null: from U).
 void launch_dt_partials(hipStream_t s, const DevMesh &m, const PhysDev &P,
-                        const double *U, double *partials, int partials_cap);
+                        const double *U, double *partials, int partials_cap,
+                        const double *prims = nullptr);
 // Primitive rows [N][5] (u, v, w, p, T) of U (con_to_prim).
 void launch_prims(hipStream_t s, const DevMesh &m, const PhysDev &P, const double *U,
                   double *prims);
-// Host-mirror rows: AoS phi_total[(N+B)*10] (primitives from con_to_prim,
-// ghost rows from the boundary conditions) and phi_dt[N*5] (R may be null).
+// Host-mirror rows: AoS phi_total[(N+B)*10] (conservative columns of U,
+// primitives by con_to_prim of U_prims and ghost rows by the boundary
+// conditions from U_prims; U_prims null: U) and phi_dt[N*5] (R may be null).
 void launch_mirror(hipStream_t s, const DevMesh &m, const PhysDev &P,
                    const double *U, const double *R, double *phi_total_aos,
-                   double *phi_dt_aos);
+                   double *phi_dt_aos, const double *U_prims = nullptr);
 // Rows of the first-level fold of n_partials partials rows.
 int fold_rows(int n_partials);
 // Folds the partials (into fold, fold_cap rows) and runs the time-loop

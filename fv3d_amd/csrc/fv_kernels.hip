@@ -615,15 +615,10 @@ __global__ __launch_bounds__(kBlock, (FS && VISC && std::is_same<Real, double>::
     const int s = tid & (LANES - 1);
     const bool cell_ok = c < sp.c_end;
 
+    // the run has stopped: this stage does nothing (both state buffers keep
+    // their contents; the host finds the final one from the step count)
     if (sp.sc && sp.sc->done)
-    {
-        // the run has stopped: carry the state through this stage unchanged
-        if (sp.do_update && cell_ok && s == 0)
-#pragma unroll
-            for (int j = 0; j < 5; ++j)
-                sp.U_out[(size_t)c * 5 + j] = sp.U_in[(size_t)c * 5 + j];
         return;
-    }
 
     const int nf = cell_ok ? m.ncf[c] : 0;
     // the epilogue's operands (the cell's state and RK register, element j in
@@ -895,13 +890,9 @@ __global__ __launch_bounds__(kBlock) void fv_stage_update_kernel(DevMesh m, Phys
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     const bool ok = c < m.N;
     double part[7] = {0.0, 0.0, 0.0, 0.0, 0.0, DBL_MAX, DBL_MAX};
-    if (ok && sp.sc->done)
-    {
-#pragma unroll
-        for (int j = 0; j < 5; ++j)
-            sp.U_out[(size_t)c * 5 + j] = sp.U_in[(size_t)c * 5 + j];
-    }
-    else if (ok)
+    if (sp.sc->done)
+        return;
+    if (ok)
     {
         const double vol = m.vol[c];
         const double dt = sp.sc->dt;
@@ -960,7 +951,7 @@ void launch_stage_update(hipStream_t s, const DevMesh &m, const PhysDev &P,
 // step: the primitives were refreshed from the initial state).
 __global__ __launch_bounds__(kBlock) void fv_dt_partials_kernel(
     DevMesh m, PhysDev P, const double *__restrict__ U,
-    double *__restrict__ partials)
+    double *__restrict__ partials, const double *__restrict__ prims)
 {
     __shared__ double sh[kBlock];
     const int N = m.N;
@@ -969,8 +960,7 @@ __global__ __launch_bounds__(kBlock) void fv_dt_partials_kernel(
     if (c < N)
     {
         double pc[10];
-        load_cons(U, c, pc);
-        con_to_prim(P, pc);
+        load_phi(P, U, prims, c, pc);
         const double dx0 = m.cdx[(size_t)c * 3], dx1 = m.cdx[(size_t)c * 3 + 1],
                      dx2 = m.cdx[(size_t)c * 3 + 2];
         const double vol = m.vol[c];
@@ -1123,7 +1113,7 @@ __global__ __launch_bounds__(kBlock) void fv_finalize_kernel(
 __global__ __launch_bounds__(kBlock) void fv_mirror_kernel(
     DevMesh m, PhysDev P, const double *__restrict__ U,
     const double *__restrict__ R, double *__restrict__ phi_aos,
-    double *__restrict__ dt_aos)
+    double *__restrict__ dt_aos, const double *__restrict__ Up)
 {
     const int N = m.N;
     const int row = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1132,8 +1122,9 @@ __global__ __launch_bounds__(kBlock) void fv_mirror_kernel(
     double pc[10];
     if (row < N)
     {
-        load_cons(U, row, pc);
+        load_cons(Up, row, pc);
         con_to_prim(P, pc);
+        load_cons(U, row, pc);
         if (R && dt_aos)
 #pragma unroll
             for (int j = 0; j < 5; ++j)
@@ -1145,7 +1136,7 @@ __global__ __launch_bounds__(kBlock) void fv_mirror_kernel(
         const int f = m.bnd_face[b];
         const int c = m.fcell[2 * f];
         double pi[10];
-        load_cons(U, c, pi);
+        load_cons(Up, c, pi);
         con_to_prim(P, pi);
         Frame fr;
         load_frame(m, f, fr);
@@ -1310,12 +1301,13 @@ void launch_residual(hipStream_t s, const DevMesh &m, const PhysDev &P,
 }
 
 void launch_dt_partials(hipStream_t s, const DevMesh &m, const PhysDev &P,
-                        const double *U, double *partials, int partials_cap)
+                        const double *U, double *partials, int partials_cap,
+                        const double *prims)
 {
     if (dt_partials_grid(m) > partials_cap)
         throw std::runtime_error("reduction partials buffer too small");
     hipLaunchKernelGGL(fv_dt_partials_kernel, dim3(dt_partials_grid(m)),
-                       dim3(kBlock), 0, s, m, P, U, partials);
+                       dim3(kBlock), 0, s, m, P, U, partials, prims);
 }
 
 __global__ __launch_bounds__(kBlock) void fv_prims_kernel(int N, PhysDev P,
@@ -1342,11 +1334,11 @@ void launch_prims(hipStream_t s, const DevMesh &m, const PhysDev &P, const doubl
 
 void launch_mirror(hipStream_t s, const DevMesh &m, const PhysDev &P,
                    const double *U, const double *R, double *phi_total_aos,
-                   double *phi_dt_aos)
+                   double *phi_dt_aos, const double *U_prims)
 {
     hipLaunchKernelGGL(fv_mirror_kernel, dim3(cell_blocks(m.N + m.B)),
                        dim3(kBlock), 0, s, m, P, U, R, phi_total_aos,
-                       phi_dt_aos);
+                       phi_dt_aos, U_prims ? U_prims : U);
 }
 
 int fold_rows(int n_partials) { return (n_partials + kFoldRows - 1) / kFoldRows; }

@@ -152,6 +152,13 @@ struct Device
     double *U[2] = {nullptr, nullptr};
     double *prims[2] = {nullptr, nullptr}; // primitives of U[i] during stepping
     int cur = 0;
+    // the buffer whose primitives the reference's phi_total holds: after an
+    // RK step, the last stage's input (its update hook ran on that state);
+    // otherwise the current state
+    int uprim = 0;
+    // primitive columns of a restart file [N][5] for the first time step
+    double *restart_prims = nullptr;
+    bool restart_prims_valid = false;
     double *k = nullptr;
     double *R = nullptr;
     double *gphi = nullptr; // [10*B] ghost rows
@@ -713,7 +720,7 @@ void sync_mirror(bool with_dt)
     const int N = g_dev.m.N, B = g_dev.m.B;
     dev::launch_mirror(g_dev.stream, g_dev.m, phys(), g_dev.U[g_dev.cur],
                        (with_dt && g_dev.r_valid) ? g_dev.R : nullptr,
-                       g_dev.mirror_phi, g_dev.mirror_dt);
+                       g_dev.mirror_phi, g_dev.mirror_dt, g_dev.U[g_dev.uprim]);
     HIP_CHECK(hipMemcpyAsync(solver_data->phi_total, g_dev.mirror_phi,
                              (size_t)(N + B) * 10 * sizeof(double),
                              hipMemcpyDeviceToHost, g_dev.stream));
@@ -733,6 +740,8 @@ void upload_state_from_mirror()
             aos[(size_t)c * 5 + j] = solver_data->phi_total[(size_t)c * 10 + j];
     HIP_CHECK(hipMemcpy(g_dev.U[g_dev.cur], aos.data(), aos.size() * 8,
                         hipMemcpyHostToDevice));
+    g_dev.uprim = g_dev.cur;
+    g_dev.restart_prims_valid = false;
     g_dev.r_valid = false;
 }
 
@@ -1088,8 +1097,11 @@ void init_scalars(int iter0, double t0, int max_iter, bool transient,
     // a host time-step hook the host sets it before each step
     if (h.host_dt)
         return;
+    // after a restart, the first dt takes the stored primitives with the
+    // stored density, as the reference's calc_ns_timestep reads phi_total
     dev::launch_dt_partials(g_dev.stream, g_dev.m, phys(), g_dev.U[g_dev.cur],
-                            g_dev.partials, g_dev.partials_cap);
+                            g_dev.partials, g_dev.partials_cap,
+                            g_dev.restart_prims_valid ? g_dev.restart_prims : nullptr);
     dev::launch_finalize(g_dev.stream, g_dev.sc, g_dev.partials, dev::dt_partials_grid(g_dev.m), g_dev.fold, g_dev.fold_cap, phys(), 0, nullptr, 0);
 }
 
@@ -1140,6 +1152,7 @@ int run_explicit(int max_iter)
             int to_next = g_i_output_data - (it % g_i_output_data);
             chunk = std::min(chunk, to_next);
         }
+        const int cur0 = g_dev.cur, iter_before = iter0 + printed;
         if (host_driven)
         {
             read_scalars();
@@ -1154,6 +1167,18 @@ int run_explicit(int max_iter)
         g_dev.r_valid = true;
         read_scalars();
         const dev::DevScalars &h = *g_dev.sc_host;
+        {
+            // the steps after the one that ended the run did nothing: the
+            // final state is where that step left it (a NaN step ends the
+            // run without counting)
+            const int steps = h.iter - iter_before + (h.done < 0 ? 1 : 0);
+            g_dev.cur = (cur0 + g_rk_stages * steps) & 1;
+            if (steps > 0)
+            {
+                g_dev.uprim = 1 - g_dev.cur;
+                g_dev.restart_prims_valid = false;
+            }
+        }
         int done_rows = h.iter - iter0;
         if (done_rows > printed)
         {
@@ -1479,6 +1504,8 @@ int run_implicit(int max_iter)
         g_hook_time = h0.t + h0.dt;
         const int inner = implicit_step(h0.iter, h0.t, h0.dt, g_transient);
         g_dev.r_valid = true;
+        g_dev.uprim = g_dev.cur;
+        g_dev.restart_prims_valid = false;
         imp::step_partials(g_dev.stream, g_dev.m, phys(), g_dev.U[g_dev.cur], g_dev.R,
                            g_dev.partials);
         dev::launch_finalize(g_dev.stream, g_dev.sc, g_dev.partials,
@@ -1921,6 +1948,8 @@ void finite_volume_time_derivative(double time)
     {
         require_ready();
         g_hook_time = time;
+        g_dev.uprim = g_dev.cur;
+        g_dev.restart_prims_valid = false;
         if (builtin_hooks())
         {
             enqueue_residual(g_dev.U[g_dev.cur], nullptr, g_dev.R, 0, false, false,
@@ -2197,9 +2226,49 @@ void read_restart_data(cstring_t restart_file)
             fatal("restart file does not match the mesh");
         if (g_dev.ready)
         {
+            // phi_total as stored: the conservative columns become the
+            // state; the primitive columns (those of the last stage's input
+            // when the file was written after an RK step) stay in the host
+            // mirror and give the first time step, as in the reference
             std::memcpy(solver_data->phi_total, phi.data(), phi.size() * 8);
             upload_state_from_mirror();
-            sync_mirror(false);
+            std::vector<double> pr((size_t)N * 5);
+            for (int c = 0; c < N; ++c)
+                for (int j = 0; j < 5; ++j)
+                    pr[(size_t)c * 5 + j] = phi[(size_t)c * 10 + 5 + j];
+            if (!g_dev.restart_prims)
+                g_dev.restart_prims = dmalloc<double>((size_t)N * 5);
+            HIP_CHECK(hipMemcpy(g_dev.restart_prims, pr.data(), pr.size() * 8,
+                                hipMemcpyHostToDevice));
+            g_dev.restart_prims_valid = true;
+            // phi_dt: the residual of the stored state (the implicit step's
+            // first right-hand side)
+            if (r.has("phi_dt"))
+            {
+                std::vector<double> dt = r.doubles("phi_dt");
+                if (dt.size() != (size_t)N * 5)
+                    fatal("restart file: phi_dt does not match the mesh");
+                std::memcpy(solver_data->phi_dt, dt.data(), dt.size() * 8);
+                HIP_CHECK(hipMemcpy(g_dev.R, dt.data(), dt.size() * 8, hipMemcpyHostToDevice));
+                g_dev.r_valid = true;
+            }
+            // the BDF history of the implicit scheme (restart.c:87-110)
+            if (!g_is_explicit && r.has("n_stages"))
+            {
+                const int ns = r.ints("n_stages")[0];
+                const int want = g_bdf_scheme == BDF2 ? 2 : 1;
+                if (ns != want)
+                    fatal("restart file: n_stages does not match the implicit scheme");
+                ensure_implicit_buffers();
+                for (int i = 0; i < ns; ++i)
+                {
+                    std::vector<double> o = r.doubles("phi_old:" + std::to_string(i));
+                    if (o.size() != (size_t)N * 5)
+                        fatal("restart file: phi_old does not match the mesh");
+                    HIP_CHECK(hipMemcpy(g_dev.imp_old[i], o.data(), o.size() * 8,
+                                        hipMemcpyHostToDevice));
+                }
+            }
         }
     }
     catch (const std::exception &e)
@@ -2515,6 +2584,8 @@ int fvh_set_state(const double *U)
         HIP_CHECK(hipStreamSynchronize(g_dev.stream));
         HIP_CHECK(hipMemcpy(g_dev.U[g_dev.cur], U, (size_t)5 * g_dev.m.N * 8,
                             hipMemcpyHostToDevice));
+        g_dev.uprim = g_dev.cur;
+        g_dev.restart_prims_valid = false;
         g_dev.r_valid = false;
     });
     return FVH_OK;

@@ -89,10 +89,9 @@ def test_driver_run_equals_direct_case(tmp_path):
     finally:
         os.chdir(cwd)
     # the restarted run does steps 4..6 (timedisc.c: max_iter more steps from
-    # the restart iteration).  Its first dt uses the primitives of the
-    # restored state, where the uninterrupted run used those of step 3's last
-    # stage input (the reference keeps that stale copy in phi_total), so the
-    # states agree to the size of that dt change, not bitwise.
+    # the restart iteration).  The file holds phi_total as the reference
+    # keeps it (the primitives of step 3's last stage input, which the first
+    # dt after the restart reads), so the restarted run repeats the
+    # uninterrupted one bit for bit.
     assert rc == 0 and rc2 == 0 and len(log) == 3
-    e = np.abs(U_restart - U_direct).max() / np.abs(U_direct).max()
-    assert e < 1e-4, e
+    assert np.array_equal(U_restart, U_direct)
