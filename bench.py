@@ -11,9 +11,12 @@ Beside it, in the same JSON line:
   499,200 cells, Ma 0.5): wall time to the 1e-8 stop norm of plain FP64
   block-Jacobi JFNK and of the variants (FP32 Krylov; learned diagonal
   blocks; the multicolour block-ILU coupling; their combinations), each
-  with its speedup over plain and the relative difference of its converged
-  solution; `headline` names the learned-preconditioner mixed-precision
-  variant (learned FP32 with the coupling) and its speedup;
+  with its speedup over plain and the relative difference of its solution;
+  `headline` names the learned-preconditioner mixed-precision variant
+  (learned FP32 with the coupling) and its speedup; `converged_1e-12` the
+  relative difference of the plain and headline solutions driven to stop
+  norm 1e-12; `survey_outlet_limiter` a bounded plain run of the channel
+  with the zero-gradient outlet and the limiter;
 - `batched`: 64 periodic cases of log-uniform size (seeded), one device
   state, graph-replayed steps: aggregate cell updates per second and the
   per-case step time (each case alone) p50/p99;
@@ -106,6 +109,8 @@ def main():
     ap.add_argument("--jfnk-variants", default="",
                     help="comma-separated subset of the JFNK variants after the plain "
                          "FP64 block-Jacobi run (profiling)")
+    ap.add_argument("--no-jfnk-agreement", action="store_true",
+                    help="skip the JFNK runs to stop norm 1e-12 and the survey-outlet run")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-cases", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=20)
@@ -200,7 +205,8 @@ def main():
     jfnk = None if args.no_jfnk else bench_jfnk([int(x) for x in args.jfnk_mesh.split(",")],
                                                  args.jfnk_max_newton,
                                                  args.jfnk_variants.split(",")
-                                                 if args.jfnk_variants else None)
+                                                 if args.jfnk_variants else None,
+                                                 not args.no_jfnk_agreement)
     batched = None if args.no_batch else bench_batch(args.batch_cases, args.batch_steps,
                                                      not args.no_batch_alone)
 
@@ -236,7 +242,7 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def bench_jfnk(n, max_newton=400, only=None):
+def bench_jfnk(n, max_newton=400, only=None, agreement=True):
     """Plain FP64 block-Jacobi JFNK against the learned-preconditioner,
     FP32-Krylov JFNK on the bump channel, both from the free stream to the
     1e-8 stop norm."""
@@ -287,7 +293,37 @@ def bench_jfnk(n, max_newton=400, only=None):
         print("jfnk %s: %.2f s, %d newton, %d gmres, fallback %d" % (
             name, st["seconds"], st["newton_steps"], st["gmres_iterations"],
             st["precond_fallbacks"]), file=sys.stderr, flush=True)
+    if agreement and have_net and (only is None or "learned_fp32_dilu" in only):
+        # the converged solutions (stop norm 1e-12) of plain FP64 block
+        # Jacobi and of the learned FP32 + DILU variant
+        rows = {}
+        sol = {}
+        for name, kw in (("plain_fp64_block", dict(precision="fp64", precond="block")),
+                         ("learned_fp32_dilu", dict(precision="fp32", precond="learned",
+                                                    **coupled))):
+            case.set_state(U0)
+            st, _ = case.jfnk(tol=1e-12, max_newton=max_newton, **kw)
+            rows[name] = _jfnk_row(st)
+            sol[name] = case.state()
+        rows["solution_rel_diff"] = float(np.linalg.norm(sol["learned_fp32_dilu"] -
+                                                         sol["plain_fp64_block"]) /
+                                          np.linalg.norm(sol["plain_fp64_block"]))
+        out["converged_1e-12"] = rows
+        print("jfnk converged to 1e-12: rel diff %.3g" % rows["solution_rel_diff"],
+              file=sys.stderr, flush=True)
     case.close()
+    if agreement:
+        # the channel as the survey states it: zero-gradient outlet and the
+        # Barth-Jespersen limiter (plain FP64 block Jacobi, bounded run)
+        cfg_s = configs.bump_channel(nx=n[0], ny=n[1], nz=n[2])
+        case = S.Case(**cfg_s)
+        st, _ = case.jfnk(tol=1e-8, precision="fp64", precond="block", max_newton=150)
+        out["survey_outlet_limiter"] = dict(_jfnk_row(st), note=(
+            "zero-gradient outlet + Barth-Jespersen limiter; status 1 = the Newton step "
+            "limit (150) was reached before the 1e-8 stop norm"))
+        print("jfnk survey outlet/limiter: status %d, norm %.3g after %d newton" % (
+            st["status"], st["norm"], st["newton_steps"]), file=sys.stderr, flush=True)
+        case.close()
     best = out.get("learned_fp32_dilu")
     if best:
         out["headline"] = {"variant": "learned_fp32_dilu",

@@ -10,6 +10,10 @@
   agrees with the oracle's own JFNK solution to <= 1e-10 relative.
 - The FP32-Krylov variant converges to the same solution as FP64
   (<= 1e-10 relative).
+- Every variant (FP32 Krylov with block Jacobi, with the coupled DILU
+  preconditioner, and with the learned blocks in DILU) reaches the oracle's
+  FP64 JFNK solution to <= 1e-10 relative at stop norm 1e-12; each error is
+  printed.
 """
 import math
 
@@ -114,4 +118,29 @@ def test_jfnk_block_beats_scalar():
            gmres_scalar=sts["gmres_iterations"], status_scalar=sts["status"])
     assert stb["status"] == 0
     assert stb["gmres_iterations"] < sts["gmres_iterations"]
+    case.close()
+
+
+def test_jfnk_variants_vs_oracle_fp64():
+    from fv3d_amd import learned
+    cfg = configs.bump_channel_steady(nx=16, ny=4, nz=3)
+    case, o, mesh = make_pair(cfg)
+    U0 = case.state()
+    oj = OracleJfnk(mesh, cfg["physics"], cfg["regions"], cfg["numerics"])
+    Uo, st_o, _ = oj.solve(U0, tol=1e-12)
+    assert st_o["status"] == 0
+    learned.load_pretrained()
+    coupled = dict(coupling="dilu", krylov_dims=40, max_restarts=2)
+    variants = {"fp64_block": dict(precision="fp64", precond="block"),
+                "fp32_block": dict(precision="fp32", precond="block"),
+                "fp32_dilu": dict(precision="fp32", precond="block", **coupled),
+                "learned_fp32_dilu": dict(precision="fp32", precond="learned", **coupled)}
+    for name, kw in variants.items():
+        case.set_state(U0)
+        st, _ = case.jfnk(tol=1e-12, max_newton=400, **kw)
+        e = _rel(case.state(), Uo)
+        _print("variant vs oracle FP64 JFNK", variant=name, rel=e, newton=st["newton_steps"],
+               fallback=st["precond_fallbacks"])
+        assert st["status"] == 0
+        assert e <= 1e-10, name
     case.close()
