@@ -18,8 +18,9 @@ Beside it, in the same JSON line:
   norm 1e-12; `survey_outlet_limiter` a bounded plain run of the channel
   with the zero-gradient outlet and the limiter;
 - `batched`: 64 periodic cases of log-uniform size (seeded), one device
-  state, graph-replayed steps: aggregate cell updates per second and the
-  per-case step time (each case alone) p50/p99;
+  state, one persistent launch per stage on the autotuned schedule,
+  graph-replayed steps: aggregate cell updates per second and the per-case
+  step time inside the batched launches, p50/p99;
 - `roofline` of the explicit stage (the gradient and residual kernels that
   make one RK stage): 1,276 algorithmic bytes per cell-stage times the cells
   over the two kernels' HIP-event-timed average durations, with the measured
@@ -114,8 +115,10 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-cases", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=20)
-    ap.add_argument("--no-batch-alone", action="store_true",
-                    help="skip the per-case runs of the batched leg")
+    ap.add_argument("--batch-alone", action="store_true",
+                    help="also time every batched case alone")
+    ap.add_argument("--batch-autotune", action="store_true",
+                    help="re-run the batched autotuner instead of using its cache")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (torch) whose HBM "
                          "byte counters calibrate FETCH_SIZE/WRITE_SIZE")
@@ -208,7 +211,7 @@ def main():
                                                  if args.jfnk_variants else None,
                                                  not args.no_jfnk_agreement)
     batched = None if args.no_batch else bench_batch(args.batch_cases, args.batch_steps,
-                                                     not args.no_batch_alone)
+                                                     args.batch_alone, args.batch_autotune)
 
     cpu = None
     if not args.no_cpu_baseline and not args.no_explicit:
@@ -341,36 +344,83 @@ def _jfnk_row(st):
             "precond_fallbacks": st["precond_fallbacks"]}
 
 
-def bench_batch(n_cases, steps, alone=True):
+def bench_batch(n_cases, steps, alone=False, retune=False):
+    """The batched cases on the schedule the autotuner picked (cached in
+    fv3d_amd/data/batch_tuning.json for this case set; tuned here when the
+    cache is missing or --batch-autotune): aggregate cell updates per second
+    of graph-replayed steps, the per-group schedule beside it, and the
+    per-case step time inside the batched launches (p50/p99)."""
     import numpy as np
     from fv3d_amd import batch as B
 
     cases = B.c5_cases(n_cases)
+    tuned = None if retune else B.load_tuning(cases)
+    tuned_here = False
+    if tuned is None:
+        tuned = B.autotune(cases)
+        B.save_tuning(cases, tuned[0], tuned[1], device="MI355X")
+        tuned_here = True
+    best, table = tuned
+    sched = dict(persistent=best["persistent"], block=best["block"] or 256,
+                 grid=best["grid"], sub_tiles=best.get("sub_tiles") or 1)
+    pers = [r for r in table if r["persistent"]]
+    best_p = min(pers, key=lambda r: r["ms_per_step"]) if pers else None
     t0 = time.time()
-    b = B.Batch(cases, tile=8)
+    b = B.Batch(cases, tile=best["tile"], schedule=sched)
     setup = time.time() - t0
     b.run(3, graph=False)
     b.run(2, graph=True)
     sec = b.run(steps, graph=True)
     sec_eager = b.run(steps, graph=False)
+    b.set_schedule(False)
+    b.run(1, graph=True)
+    sec_group = b.run(steps, graph=True)
+    # per-case step times inside the shared launches: timestamps of the
+    # persistent schedule's tiles (its best autotuned configuration)
+    per, sec_pers, p_sched = None, None, None
+    if best_p is not None:
+        p_sched = dict(persistent=True, block=best_p["block"], grid=best_p["grid"],
+                       sub_tiles=best_p.get("sub_tiles") or 1)
+        b.set_schedule(**p_sched)
+        b.run(1, graph=True)
+        sec_pers = b.run(steps, graph=True)
+        per = b.case_step_times(5)
     cells = b.n_cells
     ms = sec / steps * 1e3
     out = {"workload": "%d periodic cases, 2e4-4e5 tets log-uniform (seeds 0..%d), Mach 0.2-2.0, "
                        "Rusanov/AUSM by seed parity, Navier-Stokes every 4th, first order, RK3-3"
                        % (n_cases, n_cases - 1),
            "cells": cells, "cases": n_cases, "setup_s": round(setup, 1),
+           "schedule": dict(sched, mesh_tile=best["tile"],
+                            source="autotuned here" if tuned_here else
+                            "autotuner cache fv3d_amd/data/batch_tuning.json"),
            "ms_per_step_graph": ms, "ms_per_step_eager": sec_eager / steps * 1e3,
+           "ms_per_step_group_schedule_graph": sec_group / steps * 1e3,
+           "ms_per_step_persistent_schedule_graph": (sec_pers / steps * 1e3
+                                                     if sec_pers else None),
+           "persistent_schedule": p_sched,
            "cell_updates_per_s": cells * steps / sec,
            "cell_stage_updates_per_s": 3 * cells * steps / sec,
            "roofline_first_order_frac": round(B_CELL_STAGE_FO * 3 * cells * steps / sec / 1e9
                                               / PEAK_HBM_GBS, 4)}
+    if per is not None:
+        out["per_case_step_ms_in_batch"] = {"p50": float(np.percentile(per, 50)),
+                                            "p99": float(np.percentile(per, 99)),
+                                            "max": float(per.max()),
+                                            "schedule": "persistent, as persistent_schedule"}
     b.close()
+    print("batched: %.3f ms/step (%d cells), group schedule %.3f ms, persistent %s ms, "
+          "in-batch per-case p50 %s p99 %s ms"
+          % (ms, cells, sec_group / steps * 1e3, out["ms_per_step_persistent_schedule_graph"],
+             out.get("per_case_step_ms_in_batch", {}).get("p50"),
+             out.get("per_case_step_ms_in_batch", {}).get("p99")),
+          file=sys.stderr, flush=True)
     if not alone:
         return out
     # per-case step time, each case alone (graph replay)
     per = []
     for c in cases:
-        bb = B.Batch([c], tile=8)
+        bb = B.Batch([c], tile=best["tile"], schedule=sched)
         bb.run(2, graph=False)
         bb.run(1, graph=True)
         per.append(bb.run(10, graph=True) / 10 * 1e3)
@@ -379,9 +429,6 @@ def bench_batch(n_cases, steps, alone=True):
     out["per_case_step_ms_alone"] = {"p50": float(np.percentile(per, 50)),
                                      "p99": float(np.percentile(per, 99)),
                                      "sum": float(per.sum())}
-    print("batched: %.3f ms/step (%d cells), alone p50 %.3f p99 %.3f sum %.3f ms" % (
-        ms, cells, out["per_case_step_ms_alone"]["p50"], out["per_case_step_ms_alone"]["p99"],
-        per.sum()), file=sys.stderr, flush=True)
     return out
 
 

@@ -126,6 +126,8 @@ SIGNATURES = [
     ("fvh_batch_run", c_int, [c_int, c_int, P_DOUBLE]),
     ("fvh_batch_info", c_int, [P_INT, P_INT, c_int]),
     ("fvh_batch_case_scalars", c_int, [P_DOUBLE, P_DOUBLE, P_DOUBLE]),
+    ("fvh_batch_set_schedule", c_int, [c_int, c_int, c_int, c_int]),
+    ("fvh_batch_case_step_times", c_int, [c_int, P_DOUBLE]),
     ("fvh_nn_param_count", c_int, []),
     ("fvh_nn_init", c_int, [ctypes.c_ulonglong]),
     ("fvh_nn_get_params", c_int, [P_FLOAT]),

@@ -1,14 +1,20 @@
 """Batched cases (fvh_mesh_batch / fvh_batch_* of include/fv3d_hip.h):
 many independent periodic Euler / Navier-Stokes cases of uneven size,
-advanced together in one device state, one launch per kernel and stage
-(one per (flux, viscous) group), optionally replayed from a HIP graph.
+advanced together in one device state: per stage one persistent launch for
+every case (a work queue of tiles, each with its case's flux and
+viscosity; or, selectable, one launch per (flux, viscous) group), replayed
+from a HIP graph.  `autotune` picks the workgroup size (tile size), the
+persistent grid and the cell-ordering tile of the case meshes by timing,
+and keeps the table in fv3d_amd/data/batch_tuning.json.
 
 `c5_cases` draws the benchmark set: case sizes log-uniform in tetrahedra,
 Mach numbers uniform, the flux alternating with the seed's parity and every
 fourth case viscous.
 """
 import ctypes
+import json
 import math
+import os
 
 import numpy as np
 
@@ -57,8 +63,9 @@ class Batch:
     """The cases, reordered so that equal (flux, viscous) groups are
     adjacent; `order[i]` is the input index of batch case i."""
 
-    def __init__(self, cases, tile=8, mu_ns=1e-3, length=1.0, cfl=0.5):
-        key = lambda i: (FLUX_ID[cases[i]["flux"]], int(cases[i]["viscous"]))
+    def __init__(self, cases, tile=8, mu_ns=1e-3, length=1.0, cfl=0.5, schedule=None):
+        # viscous cases together (one gradient launch), then by flux
+        key = lambda i: (int(cases[i]["viscous"]), FLUX_ID[cases[i]["flux"]])
         self.order = sorted(range(len(cases)), key=key)
         self.cases = [cases[i] for i in self.order]
         self.length = length
@@ -87,6 +94,17 @@ class Batch:
         S.check(self.L.fvh_batch_info(ctypes.byref(n), got.ctypes.data_as(P_INT), K + 1))
         assert n.value == K and np.array_equal(got, offs)
         self.n_cells = int(offs[-1])
+        if schedule is not None:
+            self.set_schedule(**schedule)
+
+    def set_schedule(self, persistent=True, block=256, grid=0, sub_tiles=1):
+        S.check(self.L.fvh_batch_set_schedule(1 if persistent else 0, block, grid, sub_tiles))
+
+    def case_step_times(self, steps=5):
+        """Per-case step time (ms) inside the batched launches."""
+        ms = np.zeros(len(self.cases))
+        S.check(self.L.fvh_batch_case_step_times(steps, ms.ctypes.data_as(P_DOUBLE)))
+        return ms
 
     def run(self, steps, graph=True):
         sec = ctypes.c_double(0)
@@ -109,20 +127,73 @@ class Batch:
         self.case.close()
 
 
-def autotune(cases, tiles=(0, 4, 8, 16), steps=10, warmup=2, graph_options=(False, True),
-             **batch_kwargs):
-    """Times the batch step for every cell-ordering tile (hexahedra per side
-    of a locality tile; 0 = lexicographic) and with and without graph replay;
-    returns (best dict, table of dicts with ms_per_step)."""
+TUNING_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
+                           "batch_tuning.json")
+
+
+def signature(cases):
+    """What the tuning depends on: the case sizes, fluxes and viscous flags."""
+    return [[c["n_hex"], c["flux"], bool(c["viscous"])] for c in cases]
+
+
+def _cus(L):
+    cus = ctypes.c_int(0)
+    name = ctypes.create_string_buffer(128)
+    gbs = ctypes.c_double(0)
+    S.check(L.fvh_device_info(name, 128, ctypes.byref(cus), ctypes.byref(gbs)))
+    return max(1, cus.value)
+
+
+def autotune(cases, tiles=(4, 8), blocks=(256, 512), grids=(2, 4, 8), subs=(1, 4, 16, 64),
+             steps=10, warmup=2, progress=None, **batch_kwargs):
+    """Times one graph-replayed batch step for every cell-ordering tile of
+    the case meshes (hexahedra per side of a locality tile), persistent
+    workgroup size (256 or 512 threads: runs of 64 or 128 tetrahedra),
+    persistent grid (2, 4, 8 workgroups per CU) and runs per work-queue tile
+    (1, 4, 16, 64), and the per-group schedule; returns (best, table).
+    `progress`, when given, is called with each table row as it is
+    measured.  The explicit path has no cell colouring, so there is no
+    colour-block size to tune."""
     table = []
     for tile in tiles:
         b = Batch(cases, tile=tile, **batch_kwargs)
+        cus = _cus(b.L)
         b.run(warmup, graph=False)
-        for graph in graph_options:
-            b.run(1, graph=graph)
-            sec = b.run(steps, graph=graph)
-            table.append(dict(tile=tile, graph=graph, ms_per_step=sec / steps * 1e3,
-                              cells=b.n_cells))
+        for block in blocks:
+            for sub in subs:
+                for g in grids:
+                    b.set_schedule(True, block, g * cus, sub)
+                    b.run(1, graph=True)
+                    sec = b.run(steps, graph=True)
+                    table.append(dict(tile=tile, persistent=True, block=block, grid=g * cus,
+                                      sub_tiles=sub, ms_per_step=sec / steps * 1e3))
+                    if progress:
+                        progress(table[-1])
+        b.set_schedule(False)
+        b.run(1, graph=True)
+        sec = b.run(steps, graph=True)
+        table.append(dict(tile=tile, persistent=False, block=0, grid=0, sub_tiles=0,
+                          ms_per_step=sec / steps * 1e3))
+        if progress:
+            progress(table[-1])
         b.close()
     best = min(table, key=lambda r: r["ms_per_step"])
     return best, table
+
+
+def load_tuning(cases):
+    """The cached autotuner result for these cases, (best, table), or None."""
+    try:
+        with open(TUNING_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    return (t["best"], t["table"]) if t.get("signature") == signature(cases) else None
+
+
+def save_tuning(cases, best, table, device=""):
+    os.makedirs(os.path.dirname(TUNING_FILE), exist_ok=True)
+    with open(TUNING_FILE, "w") as f:
+        json.dump({"signature": signature(cases), "device": device, "best": best,
+                   "table": table}, f, indent=1)
+        f.write("\n")

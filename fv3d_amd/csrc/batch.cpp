@@ -1,11 +1,16 @@
 // Batched mode: many independent periodic cases of uneven size in one mesh
 // (disjoint cell ranges, one device state), advanced together by the
-// explicit RK scheme.  Cases are grouped by (flux, viscous) into contiguous
-// cell ranges; each group is one launch of the gradient (viscous groups)
-// and residual/stage kernels per stage, every case steps with its own dt,
-// and per-case residual norms and time steps come from chunked reductions
-// folded per case.  A step is captured once into a HIP graph (one per
-// ping-pong parity) and replayed.
+// explicit RK scheme, every case with its own dt.
+//
+// Persistent schedule: per stage one launch of the gradient kernel over
+// each contiguous run of viscous cases, then one persistent launch of the
+// stage kernel per (flux, viscous) class of cases: workgroups take tiles
+// (block / lanes consecutive cells of one case) from the launch's work
+// queue and compute each with its case's viscosity and dt; the last stage
+// writes per-tile partials that a per-case fold turns into the residual
+// norms and the next dt.  Group schedule: one launch per (flux, viscous) group and stage, with
+// chunked per-case reductions.  Either step is captured once into a HIP
+// graph (one per ping-pong parity) and replayed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,6 +34,10 @@ struct Group
     double mu;
 };
 
+constexpr int kMaxStages = 5, kClasses = 4, kCounters = kMaxStages * kClasses;
+
+int case_class(int flux, int viscous) { return viscous * 2 + (flux == dev::FluxRusanov); }
+
 struct BatchState
 {
     bool ready = false;
@@ -46,7 +55,22 @@ struct BatchState
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     hipGraph_t graph[2] = {nullptr, nullptr};
     int steps = 0;
+    // persistent schedule
+    bool persistent = false; // the per-group schedule measured faster
+    int block = 256, grid = 0; // grid 0: min(tiles, 4 x CUs)
+    int sub_tiles = 1;         // runs of block / lanes cells per tile
+    // tiles by class (viscous * 2 + flux: AUSM and Rusanov inviscid, then
+    // viscous), each case's tiles consecutive
+    int n_tiles = 0, tile_cells = 0, cls_base[kClasses] = {}, cls_tiles[kClasses] = {};
+    int *tile_case = nullptr, *tile_begin = nullptr, *case_end = nullptr, *row_lo = nullptr,
+        *row_hi = nullptr; // a case's partial rows row_lo[k] .. row_hi[k] - 1
+    double *d_lam = nullptr, *tile_partials = nullptr;
+    unsigned int *counters = nullptr; // [stage][class]
+    unsigned long long *stamps = nullptr; // [stages][n_tiles][2] when timing
+    std::vector<std::pair<int, int>> visc_runs; // cell ranges of viscous cases
+    std::vector<int> h_tile_lo, h_tile_hi; // a case's tiles
 };
+
 
 BatchState g_b;
 std::vector<int> g_mesh_offsets; // from the last fvh_mesh_batch
@@ -139,10 +163,149 @@ void append_mesh(RawMesh &d, const RawMesh &s)
     cat(d.bnd_distance, s.bnd_distance);
 }
 
+// Tiles of the persistent schedule for `block` threads per workgroup (the
+// case's cells in order, sub_tiles * block / lanes per tile; the tiles of
+// one class together), uploaded to the device.
+void build_tiles(const RuntimeView &v)
+{
+    const int lanes = v.m.maxf <= 4 ? 4 : 8;
+    const int run = g_b.block / lanes, gc = run * g_b.sub_tiles;
+    std::vector<int> tc, tb, lo(g_b.K), hi(g_b.K), rlo(g_b.K), rhi(g_b.K);
+    for (int cls = 0; cls < kClasses; ++cls)
+    {
+        g_b.cls_base[cls] = (int)tc.size();
+        for (int k = 0; k < g_b.K; ++k)
+        {
+            if (case_class(g_b.case_flux[k], g_b.case_visc[k]) != cls)
+                continue;
+            lo[k] = (int)tc.size();
+            for (int c = g_b.offsets[k]; c < g_b.offsets[k + 1]; c += gc)
+            {
+                tc.push_back(k);
+                tb.push_back(c);
+            }
+            hi[k] = (int)tc.size();
+            // partial rows: one per run of the case's tiles
+            rlo[k] = lo[k] * g_b.sub_tiles;
+            rhi[k] = rlo[k] + (g_b.offsets[k + 1] - g_b.offsets[k] + run - 1) / run;
+        }
+        g_b.cls_tiles[cls] = (int)tc.size() - g_b.cls_base[cls];
+    }
+    g_b.n_tiles = (int)tc.size();
+    g_b.tile_cells = gc;
+    // earlier tables stay allocated until the batch is released
+    g_b.tile_case = balloc<int>(tc.size());
+    g_b.tile_begin = balloc<int>(tb.size());
+    g_b.row_lo = balloc<int>(g_b.K);
+    g_b.row_hi = balloc<int>(g_b.K);
+    g_b.tile_partials = balloc<double>((size_t)8 * tc.size() * g_b.sub_tiles);
+    g_b.stamps = nullptr; // sized for the tile count: allocated again when timing
+    FVH_HIP_CHECK(hipMemcpy(g_b.tile_case, tc.data(), tc.size() * sizeof(int),
+                            hipMemcpyHostToDevice));
+    FVH_HIP_CHECK(hipMemcpy(g_b.tile_begin, tb.data(), tb.size() * sizeof(int),
+                            hipMemcpyHostToDevice));
+    FVH_HIP_CHECK(hipMemcpy(g_b.row_lo, rlo.data(), rlo.size() * sizeof(int),
+                            hipMemcpyHostToDevice));
+    FVH_HIP_CHECK(hipMemcpy(g_b.row_hi, rhi.data(), rhi.size() * sizeof(int),
+                            hipMemcpyHostToDevice));
+    g_b.h_tile_lo = lo;
+    g_b.h_tile_hi = hi;
+}
+
+dev::BatchTiles tiles_view(int stage, int cls, unsigned long long *stamps)
+{
+    dev::BatchTiles bt{};
+    bt.tile_base = g_b.cls_base[cls];
+    bt.n_tiles = g_b.cls_tiles[cls];
+    bt.sub_tiles = g_b.sub_tiles;
+    bt.tile_case = g_b.tile_case;
+    bt.tile_begin = g_b.tile_begin;
+    bt.case_end = g_b.case_end;
+    bt.case_mu = g_b.d_mu;
+    bt.case_lam = g_b.d_lam;
+    bt.counter = g_b.counters + stage * kClasses + cls;
+    bt.stamps = stamps;
+    return bt;
+}
+
+int persistent_grid(int n_tiles)
+{
+    int grid = g_b.grid;
+    if (grid <= 0)
+    {
+        int dev = 0, cus = 256;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        grid = 4 * cus;
+    }
+    return std::max(1, std::min(grid, n_tiles));
+}
+
+// One RK step of every case on the persistent schedule; with `stamps`, the
+// tiles' wall-clock start and end of stage st go to stamps + st * 2 * tiles.
+// The stage launches of the two classes share the stage's state buffers
+// (disjoint cells).
+void enqueue_step_persistent(const RuntimeView &v, const StageBuffers &sb,
+                             unsigned long long *stamps)
+{
+    hipStream_t s = v.stream;
+    const dev::DevMesh &m = v.m;
+    for (int st = 0; st < sb.stages; ++st)
+    {
+        const bool last = st == sb.stages - 1;
+        double *Uin = sb.U[*sb.cur], *Uout = sb.U[1 - *sb.cur];
+        for (const auto &r : g_b.visc_runs)
+        {
+            dev::GradOut o{};
+            o.vg = g_b.vg;
+            o.c_begin = r.first;
+            o.c_end = r.second;
+            o.prims = sb.prims[*sb.cur];
+            dev::launch_gradient(s, m, v.P, Uin, nullptr, o, 0, nullptr);
+        }
+        dev::StageParams sp{};
+        sp.U_in = Uin;
+        sp.U_out = Uout;
+        sp.k = sb.k;
+        sp.R_out = last ? sb.R : nullptr;
+        sp.vg = g_b.vg;
+        sp.rk_a = sb.rk_a[st];
+        sp.rk_g = sb.rk_g[st];
+        sp.stage0 = st == 0;
+        sp.do_update = 1;
+        sp.do_reduce = last;
+        sp.partials = g_b.tile_partials;
+        sp.partials_cap = g_b.n_tiles * g_b.sub_tiles;
+        sp.c_begin = 0;
+        sp.c_end = m.N;
+        sp.case_dt = g_b.d_dt;
+        sp.cell_case = g_b.cell_case;
+        sp.prims_in = sb.prims[*sb.cur];
+        sp.prims_out = sb.prims[1 - *sb.cur];
+        unsigned long long *sst = stamps ? stamps + (size_t)st * 2 * g_b.n_tiles : nullptr;
+        for (int cls = 0; cls < kClasses; ++cls)
+            if (g_b.cls_tiles[cls] > 0)
+                dev::launch_batch_stage(s, m, v.P, sp, tiles_view(st, cls, sst),
+                                        cls % 2 ? dev::FluxRusanov : dev::FluxAUSM, cls >= 2,
+                                        g_b.block, persistent_grid(g_b.cls_tiles[cls]));
+        *sb.cur = 1 - *sb.cur;
+    }
+    // per-case residuals (dt of the step) and next dt from the tiles'
+    // partials, in tile order; then the work-queue counters for the next step
+    batch::case_fold(s, 1, v.P, g_b.K, g_b.row_lo, g_b.row_hi, g_b.tile_partials, g_b.d_mu,
+                     g_b.d_vol, g_b.d_dt, g_b.d_t, g_b.d_res);
+    batch::reset_counters(s, g_b.counters, kCounters);
+}
+
 // Enqueues one RK step of every case (the stages of every group, then the
 // per-case residuals and next time steps).
 void enqueue_step(const RuntimeView &v, const StageBuffers &sb)
 {
+    if (g_b.persistent)
+    {
+        enqueue_step_persistent(v, sb, nullptr);
+        return;
+    }
     hipStream_t s = v.stream;
     const dev::DevMesh &m = v.m;
     for (int st = 0; st < sb.stages; ++st)
@@ -187,8 +350,8 @@ void enqueue_step(const RuntimeView &v, const StageBuffers &sb)
     // stage's input primitives
     batch::chunk_partials(s, m, v.P, g_b.n_chunks, g_b.chunks, sb.U[*sb.cur],
                           sb.U[1 - *sb.cur], sb.R, g_b.partials);
-    batch::case_fold(s, 1, v.P, g_b.K, g_b.chunk_begin, g_b.partials, g_b.d_mu, g_b.d_vol,
-                     g_b.d_dt, g_b.d_t, g_b.d_res);
+    batch::case_fold(s, 1, v.P, g_b.K, g_b.chunk_begin, g_b.chunk_begin + 1, g_b.partials,
+                     g_b.d_mu, g_b.d_vol, g_b.d_dt, g_b.d_t, g_b.d_res);
 }
 
 } // namespace
@@ -255,6 +418,8 @@ int fvh_batch_setup(int n_cases, const int *flux, const int *viscous, const doub
         // groups: maximal runs of equal (flux, viscous, mu)
         for (int k = 0; k < n_cases; ++k)
         {
+            if (flux[k] != dev::FluxAUSM && flux[k] != dev::FluxRusanov)
+                throw std::runtime_error("fvh_batch_setup: flux must be AUSM or Rusanov");
             const int vis = viscous[k] != 0 && mu[k] != 0.0;
             g_b.case_flux.push_back(flux[k]);
             g_b.case_visc.push_back(vis);
@@ -308,6 +473,33 @@ int fvh_batch_setup(int n_cases, const int *flux, const int *viscous, const doub
                                 hipMemcpyHostToDevice));
         FVH_HIP_CHECK(hipMemcpy(g_b.d_vol, vol.data(), n_cases * sizeof(double),
                                 hipMemcpyHostToDevice));
+        // persistent schedule: per-case tables, tiles, counters, and the
+        // cell runs of viscous cases (one gradient launch each)
+        {
+            std::vector<int> cend(n_cases);
+            std::vector<double> lam(n_cases);
+            for (int k = 0; k < n_cases; ++k)
+            {
+                cend[k] = g_b.offsets[k + 1];
+                lam[k] = g_b.case_mu[k] * cp / Pr;
+                if (g_b.case_visc[k])
+                {
+                    if (!g_b.visc_runs.empty() && g_b.visc_runs.back().second == g_b.offsets[k])
+                        g_b.visc_runs.back().second = g_b.offsets[k + 1];
+                    else
+                        g_b.visc_runs.push_back({g_b.offsets[k], g_b.offsets[k + 1]});
+                }
+            }
+            g_b.case_end = balloc<int>(n_cases);
+            g_b.d_lam = balloc<double>(n_cases);
+            g_b.counters = balloc<unsigned int>(kCounters);
+            FVH_HIP_CHECK(hipMemcpy(g_b.case_end, cend.data(), n_cases * sizeof(int),
+                                    hipMemcpyHostToDevice));
+            FVH_HIP_CHECK(hipMemcpy(g_b.d_lam, lam.data(), n_cases * sizeof(double),
+                                    hipMemcpyHostToDevice));
+            batch::reset_counters(v.stream, g_b.counters, kCounters);
+            build_tiles(v);
+        }
         std::vector<double> zeros((size_t)5 * n_cases, 0.0);
         FVH_HIP_CHECK(hipMemcpy(g_b.d_t, zeros.data(), n_cases * sizeof(double),
                                 hipMemcpyHostToDevice));
@@ -316,8 +508,8 @@ int fvh_batch_setup(int n_cases, const int *flux, const int *viscous, const doub
         // initial time steps from the current state (primitives refreshed)
         batch::chunk_partials(v.stream, v.m, v.P, g_b.n_chunks, g_b.chunks, v.U, v.U, nullptr,
                               g_b.partials);
-        batch::case_fold(v.stream, 0, v.P, n_cases, g_b.chunk_begin, g_b.partials, g_b.d_mu,
-                         g_b.d_vol, g_b.d_dt, g_b.d_t, g_b.d_res);
+        batch::case_fold(v.stream, 0, v.P, n_cases, g_b.chunk_begin, g_b.chunk_begin + 1,
+                         g_b.partials, g_b.d_mu, g_b.d_vol, g_b.d_dt, g_b.d_t, g_b.d_res);
         FVH_HIP_CHECK(hipStreamSynchronize(v.stream));
         g_b.ready = true;
     });
@@ -404,5 +596,76 @@ int fvh_batch_case_scalars(double *dt, double *t, double *res)
         if (res)
             FVH_HIP_CHECK(hipMemcpy(res, g_b.d_res, (size_t)5 * g_b.K * sizeof(double),
                                     hipMemcpyDeviceToHost));
+    });
+}
+
+int fvh_batch_set_schedule(int persistent, int block, int grid, int sub_tiles)
+{
+    B_TRY({
+        if (persistent && block != 128 && block != 256 && block != 512)
+            throw std::runtime_error("fvh_batch_set_schedule: block must be 128, 256 or 512");
+        if (persistent && (sub_tiles < 1 || sub_tiles > 1024))
+            throw std::runtime_error("fvh_batch_set_schedule: sub_tiles must be 1 .. 1024");
+        const bool rebuild =
+            g_b.ready && persistent && (block != g_b.block || sub_tiles != g_b.sub_tiles);
+        g_b.persistent = persistent != 0;
+        if (persistent)
+        {
+            g_b.block = block;
+            g_b.sub_tiles = sub_tiles;
+        }
+        g_b.grid = grid;
+        release_graphs();
+        if (rebuild)
+            build_tiles(runtime_view());
+    });
+}
+
+int fvh_batch_case_step_times(int steps, double *ms)
+{
+    B_TRY({
+        if (!g_b.ready || !g_b.persistent)
+            throw std::runtime_error("fvh_batch_case_step_times: needs the persistent schedule");
+        if (steps < 1 || !ms)
+            throw std::runtime_error("fvh_batch_case_step_times: bad arguments");
+        RuntimeView v = runtime_view();
+        StageBuffers sb = stage_buffers();
+        hipStream_t s = v.stream;
+        dev::launch_prims(s, v.m, v.P, sb.U[*sb.cur], sb.prims[*sb.cur]);
+        const size_t n = (size_t)sb.stages * 2 * g_b.n_tiles;
+        if (!g_b.stamps)
+            g_b.stamps = balloc<unsigned long long>((size_t)kMaxStages * 2 * g_b.n_tiles);
+        std::vector<unsigned long long> h(n);
+        std::vector<double> acc(g_b.K, 0.0);
+        int dev = 0, khz = 100000;
+        FVH_HIP_CHECK(hipGetDevice(&dev));
+        FVH_HIP_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+        for (int it = 0; it < steps; ++it)
+        {
+            enqueue_step_persistent(v, sb, g_b.stamps);
+            FVH_HIP_CHECK(hipMemcpyAsync(h.data(), g_b.stamps, n * sizeof(unsigned long long),
+                                         hipMemcpyDeviceToHost, s));
+            FVH_HIP_CHECK(hipStreamSynchronize(s));
+            // a case's time in a stage: from the start of its first tile to
+            // the end of its last, inside the shared launch
+            for (int st = 0; st < sb.stages; ++st)
+            {
+                const unsigned long long *p = h.data() + (size_t)st * 2 * g_b.n_tiles;
+                for (int k = 0; k < g_b.K; ++k)
+                {
+                    unsigned long long lo = ~0ull, hi = 0;
+                    for (int t = g_b.h_tile_lo[k]; t < g_b.h_tile_hi[k]; ++t)
+                    {
+                        lo = std::min(lo, p[(size_t)t * 2]);
+                        hi = std::max(hi, p[(size_t)t * 2 + 1]);
+                    }
+                    acc[k] += (double)(hi - lo) / (double)khz;
+                }
+            }
+        }
+        for (int k = 0; k < g_b.K; ++k)
+            ms[k] = acc[k] / steps;
+        g_b.steps += steps;
+        state_changed();
     });
 }

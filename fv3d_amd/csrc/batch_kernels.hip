@@ -63,11 +63,13 @@ __global__ __launch_bounds__(kBlock) void k_chunk_partials(DevMesh m, PhysDev P,
             partials[(size_t)blockIdx.x * 8 + j] = v[j];
 }
 
-// One block per case: fold its chunks in order; residual with the step's
+// One block per case: fold its chunks chunk_begin[k] .. chunk_end[k] - 1 in
+// order; residual with the step's
 // dt (analyze.c:57-58), then the next dt (navier_stokes.c:182-192 with the
 // case's viscosity; the viscous limit is skipped for mu = 0).
 __global__ __launch_bounds__(kBlock) void k_case_fold(int mode, PhysDev P,
                                                       const int *__restrict__ chunk_begin,
+                                                      const int *__restrict__ chunk_end,
                                                       const double *__restrict__ partials,
                                                       const double *__restrict__ case_mu,
                                                       const double *__restrict__ case_vol,
@@ -78,7 +80,7 @@ __global__ __launch_bounds__(kBlock) void k_case_fold(int mode, PhysDev P,
     __shared__ double sh[kBlock];
     const int k = blockIdx.x;
     double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, DBL_MAX, DBL_MAX};
-    for (int i = chunk_begin[k] + threadIdx.x; i < chunk_begin[k + 1]; i += kBlock)
+    for (int i = chunk_begin[k] + threadIdx.x; i < chunk_end[k]; i += kBlock)
     {
         const double *p = partials + (size_t)i * 8;
 #pragma unroll
@@ -109,7 +111,18 @@ __global__ __launch_bounds__(kBlock) void k_case_fold(int mode, PhysDev P,
     case_dt[k] = (P.visc_dt_mode == 1 && mu == 0.0) ? lc : dev::rmin(lc, lv);
 }
 
+__global__ void k_reset_counters(unsigned int *c, int n)
+{
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+        c[i] = 0u;
+}
+
 } // namespace
+
+void reset_counters(hipStream_t s, unsigned int *counters, int n)
+{
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, counters, n);
+}
 
 void chunk_partials(hipStream_t s, const DevMesh &m, const PhysDev &P, int n_chunks,
                     const int3 *chunks, const double *U, const double *Uprim, const double *R,
@@ -120,11 +133,11 @@ void chunk_partials(hipStream_t s, const DevMesh &m, const PhysDev &P, int n_chu
 }
 
 void case_fold(hipStream_t s, int mode, const PhysDev &P, int n_cases, const int *chunk_begin,
-               const double *partials, const double *case_mu, const double *case_vol,
-               double *case_dt, double *case_t, double *case_res)
+               const int *chunk_end, const double *partials, const double *case_mu,
+               const double *case_vol, double *case_dt, double *case_t, double *case_res)
 {
     hipLaunchKernelGGL(k_case_fold, dim3(n_cases), dim3(kBlock), 0, s, mode, P, chunk_begin,
-                       partials, case_mu, case_vol, case_dt, case_t, case_res);
+                       chunk_end, partials, case_mu, case_vol, case_dt, case_t, case_res);
 }
 
 } // namespace batch

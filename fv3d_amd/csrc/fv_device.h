@@ -141,6 +141,22 @@ struct StageParams
 constexpr int kBlock = 256;
 constexpr int kPartialStride = 8;
 
+// Work queue of the batched stage kernel: tiles of consecutive cells, each
+// inside one case, taken by persistent workgroups through an atomic counter
+// (the counter only schedules: each tile's results do not depend on which
+// workgroup computes it).  Per case: flux kind, viscous flag, mu, lambda.
+struct BatchTiles
+{
+    int tile_base, n_tiles; // the launch takes tiles tile_base .. tile_base + n_tiles - 1
+    int sub_tiles;          // runs of block / lanes cells per tile
+    const int *tile_case;  // [all tiles]
+    const int *tile_begin; // [all tiles] first cell of the tile (tiles end with their case)
+    const int *case_end;   // [K] one past the case's last cell
+    const double *case_mu, *case_lam; // [K]
+    unsigned int *counter; // zero before the launch
+    unsigned long long *stamps; // [all tiles][2] wall-clock start/end per tile (null: off)
+};
+
 inline int cell_blocks(int N) { return (N + kBlock - 1) / kBlock; }
 
 // Workgroups of a launch_residual grid (one partials row each when the
@@ -179,6 +195,14 @@ void launch_stage_update(hipStream_t s, const DevMesh &m, const PhysDev &P,
 void launch_dt_partials(hipStream_t s, const DevMesh &m, const PhysDev &P,
                         const double *U, double *partials, int partials_cap,
                         const double *prims = nullptr);
+// One RK stage of batched first-order cases (StageParams with case_dt /
+// cell_case; partials per tile when do_reduce) over the tiles of bt, all of
+// cases with flux `flux` and viscosity on or off (`viscous`): `block`
+// threads per workgroup (128, 256 or 512; a tile is bt.sub_tiles runs of
+// block / lanes cells), `grid` persistent workgroups.
+void launch_batch_stage(hipStream_t s, const DevMesh &m, const PhysDev &P,
+                        const StageParams &sp, const BatchTiles &bt, int flux, bool viscous,
+                        int block, int grid);
 // Primitive rows [N][5] (u, v, w, p, T) of U (con_to_prim).
 void launch_prims(hipStream_t s, const DevMesh &m, const PhysDev &P, const double *U,
                   double *prims);

@@ -598,28 +598,17 @@ __global__ __launch_bounds__(kTile, FVH_K1T_MINW) void fv_gradient_tile_kernel(
 // FS: face states from K1 (reconstructed path); otherwise first order, the
 // face states are the cell values of U.
 // ---------------------------------------------------------------------------
-// The FP64 viscous kernel on face states is held to 3 waves per SIMD
-// (FVH_K2_VISC_W): with the epilogue operands loaded up front it needs about
-// 146 VGPRs, and at 4 waves its spills made it 1.6x slower; the other
-// instantiations keep the compiler's choice.
+// The work of one lane of a cell group: c the cell (cell_ok: inside the
+// launch range), s the lane.  sh: LDS for the block reduction (8 doubles per
+// wave); partial_row: where thread 0 writes the block's partials when the
+// stage reduces.  Shared by the residual kernel and the batched kernels.
 template <int FLUX, bool FS, bool VISC, int LANES, class Real>
-__global__ __launch_bounds__(kBlock, (FS && VISC && std::is_same<Real, double>::value)
-                                         ? FVH_K2_VISC_W
-                                         : FVH_K2_MINW) void fv_residual_group_kernel(
-    DevMesh m, PhysDev P, StageParams sp)
+__device__ __forceinline__ void residual_group_body(const DevMesh &m, const PhysDev &P,
+                                                    const StageParams &sp, int c, int s,
+                                                    bool cell_ok, double *sh,
+                                                    double *partial_row)
 {
-    __shared__ double sh[8 * kBlock / 64];
     const int N = m.N;
-    const int tid = xcd_tile(gridDim.x) * kBlock + threadIdx.x;
-    const int c = sp.c_begin + tid / LANES;
-    const int s = tid & (LANES - 1);
-    const bool cell_ok = c < sp.c_end;
-
-    // the run has stopped: this stage does nothing (both state buffers keep
-    // their contents; the host finds the final one from the step count)
-    if (sp.sc && sp.sc->done)
-        return;
-
     const int nf = cell_ok ? m.ncf[c] : 0;
     // the epilogue's operands (the cell's state and RK register, element j in
     // lane j % LANES of the group, and its volume), loaded before the face
@@ -873,9 +862,165 @@ __global__ __launch_bounds__(kBlock, (FS && VISC && std::is_same<Real, double>::
         {
 #pragma unroll
             for (int q = 0; q < 7; ++q)
-                sp.partials[blockIdx.x * kPartialStride + q] = part[q];
+                partial_row[q] = part[q];
         }
     }
+}
+
+// The FP64 viscous kernel on face states is held to 3 waves per SIMD
+// (FVH_K2_VISC_W): with the epilogue operands loaded up front it needs about
+// 146 VGPRs, and at 4 waves its spills made it 1.6x slower; the other
+// instantiations keep the compiler's choice.
+template <int FLUX, bool FS, bool VISC, int LANES, class Real>
+__global__ __launch_bounds__(kBlock, (FS && VISC && std::is_same<Real, double>::value)
+                                         ? FVH_K2_VISC_W
+                                         : FVH_K2_MINW) void fv_residual_group_kernel(
+    DevMesh m, PhysDev P, StageParams sp)
+{
+    __shared__ double sh[8 * kBlock / 64];
+    const int tid = xcd_tile(gridDim.x) * kBlock + threadIdx.x;
+    const int c = sp.c_begin + tid / LANES;
+    const int s = tid & (LANES - 1);
+    const bool cell_ok = c < sp.c_end;
+    // the run has stopped: this stage does nothing (both state buffers keep
+    // their contents; the host finds the final one from the step count)
+    if (sp.sc && sp.sc->done)
+        return;
+    residual_group_body<FLUX, FS, VISC, LANES, Real>(
+        m, P, sp, c, s, cell_ok, sh, sp.partials + (size_t)blockIdx.x * kPartialStride);
+}
+
+
+
+// ---------------------------------------------------------------------------
+// Batched cases: per stage one persistent launch for each (flux, viscous)
+// class of cases (one flux and viscosity variant per kernel keeps its
+// register count, and so its occupancy, that of the single-case kernel).
+// Each workgroup takes tiles from the launch's work queue until it is
+// empty; a tile is bt.sub_tiles runs of BS / LANES consecutive cells of one
+// case, computed one run after the other with that case's viscosity and
+// time step by the residual kernel's own lane-group code (first order).
+// With do_reduce, run u of tile t writes its partials to row
+// t * sub_tiles + u of sp.partials (a case's tiles are consecutive and its
+// last tile ends with it, so its rows are consecutive too and the per-case
+// fold reads them in a fixed order).  The launch bounds ask for the waves
+// per SIMD of the single-case kernel of the same variant (the tile loop
+// would otherwise keep enough loop-invariant values live to lose one).
+// ---------------------------------------------------------------------------
+template <int BS, int LANES, int FLUX, bool VISC>
+__global__ __launch_bounds__(BS, LANES == 4 ? (VISC ? 3 : 5) : (VISC ? 2 : 3)) void
+fv_batch_stage_kernel(DevMesh m, PhysDev P, StageParams sp,
+                                                           BatchTiles bt)
+{
+    __shared__ double sh[8 * BS / 64];
+    __shared__ int s_tile;
+    constexpr int GC = BS / LANES;
+    // One lane-0 section per iteration (end stamp of the previous tile, the
+    // next tile, its start stamp) and a wave-uniform tile index: the loop
+    // exit is a scalar branch and every barrier sits in code that all lanes
+    // run.  Lane 0 keeps the following queue index in flight while the tile
+    // is computed, so the atomic's latency overlaps the tile's loads.
+    int next = 0;
+    if (threadIdx.x == 0)
+        next = (int)atomicAdd(bt.counter, 1u);
+    int t = -1;
+    for (;;)
+    {
+        __syncthreads();
+        if (threadIdx.x == 0)
+        {
+            if (bt.stamps && t >= 0)
+                bt.stamps[(size_t)t * 2 + 1] = wall_clock64();
+            const int nt = next < bt.n_tiles ? bt.tile_base + next : -1;
+            s_tile = nt;
+            if (nt >= 0)
+            {
+                next = (int)atomicAdd(bt.counter, 1u);
+                if (bt.stamps)
+                    bt.stamps[(size_t)nt * 2] = wall_clock64();
+            }
+        }
+        __syncthreads();
+        t = __builtin_amdgcn_readfirstlane(s_tile);
+        if (t < 0)
+            return;
+        const int k = bt.tile_case[t];
+        const int begin = bt.tile_begin[t], end = bt.case_end[k];
+        PhysDev Pk = P;
+        Pk.mu = bt.case_mu[k];
+        Pk.lam = bt.case_lam[k];
+        const int s = threadIdx.x & (LANES - 1);
+        for (int u = 0; u < bt.sub_tiles; ++u)
+        {
+            StageParams q = sp;
+            q.c_begin = begin + u * GC;
+            if (q.c_begin >= end)
+                break;
+            q.c_end = min(q.c_begin + GC, end);
+            const int c = q.c_begin + (int)threadIdx.x / LANES;
+            double *row = sp.partials + ((size_t)t * bt.sub_tiles + u) * kPartialStride;
+            residual_group_body<FLUX, false, VISC, LANES, double>(m, Pk, q, c, s, c < q.c_end,
+                                                                 sh, row);
+        }
+    }
+}
+
+template <int LANES, int FLUX, bool VISC>
+static void launch_batch_lanes(hipStream_t s, const DevMesh &m, const PhysDev &P,
+                               const StageParams &sp, const BatchTiles &bt, int block, int grid)
+{
+    if (block == 128)
+        hipLaunchKernelGGL((fv_batch_stage_kernel<128, LANES, FLUX, VISC>), dim3(grid),
+                           dim3(128), 0, s, m, P, sp, bt);
+    else if (block == 256)
+        hipLaunchKernelGGL((fv_batch_stage_kernel<256, LANES, FLUX, VISC>), dim3(grid),
+                           dim3(256), 0, s, m, P, sp, bt);
+    else if (block == 512)
+        hipLaunchKernelGGL((fv_batch_stage_kernel<512, LANES, FLUX, VISC>), dim3(grid),
+                           dim3(512), 0, s, m, P, sp, bt);
+    else
+        throw std::runtime_error("launch_batch_stage: block must be 128, 256 or 512");
+}
+
+template <int LANES>
+static void launch_batch_class(hipStream_t s, const DevMesh &m, const PhysDev &P,
+                               const StageParams &sp, const BatchTiles &bt, int flux,
+                               bool viscous, int block, int grid)
+{
+    if (flux == FluxRusanov)
+    {
+        if (viscous)
+            launch_batch_lanes<LANES, FluxRusanov, true>(s, m, P, sp, bt, block, grid);
+        else
+            launch_batch_lanes<LANES, FluxRusanov, false>(s, m, P, sp, bt, block, grid);
+    }
+    else if (flux == FluxAUSM)
+    {
+        if (viscous)
+            launch_batch_lanes<LANES, FluxAUSM, true>(s, m, P, sp, bt, block, grid);
+        else
+            launch_batch_lanes<LANES, FluxAUSM, false>(s, m, P, sp, bt, block, grid);
+    }
+    else
+        throw std::runtime_error("launch_batch_stage: unknown flux");
+}
+
+void launch_batch_stage(hipStream_t s, const DevMesh &m, const PhysDev &P,
+                        const StageParams &sp, const BatchTiles &bt, int flux, bool viscous,
+                        int block, int grid)
+{
+    if (m.maxf > 6)
+        throw std::runtime_error("cells with more than 6 faces are not supported");
+    if (!sp.case_dt || !sp.cell_case || !sp.do_update)
+        throw std::runtime_error("launch_batch_stage: needs per-case time steps");
+    if (grid < 1 || bt.n_tiles < 1 || bt.tile_base < 0 || bt.sub_tiles < 1)
+        throw std::runtime_error("launch_batch_stage: empty launch");
+    if (viscous && !sp.vg)
+        throw std::runtime_error("launch_batch_stage: viscous tiles need the gradients");
+    if (m.maxf <= 4)
+        launch_batch_class<4>(s, m, P, sp, bt, flux, viscous, block, grid);
+    else
+        launch_batch_class<8>(s, m, P, sp, bt, flux, viscous, block, grid);
 }
 
 // The 2N-storage RK stage (explicit.c:113-140) from a residual R computed

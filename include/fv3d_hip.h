@@ -232,8 +232,8 @@ int fvh_jfnk_blocks(double cfl, int precond, double *dtau, double *minv);
 int fvh_jfnk_precond_apply(double cfl, int precond, int coupling, const double *v, double *z);
 
 /* ---- batched cases ------------------------------------------------------ */
-/* Many independent cases in one device state and one launch per kernel
- * and stage: n_cases periodic boxes [0,length]^3 of n_hex[k]^3 hexahedra,
+/* Many independent cases in one device state, advanced by one launch per
+ * stage (see fvh_batch_set_schedule): n_cases periodic boxes [0,length]^3 of n_hex[k]^3 hexahedra,
  * each Kuhn-split into 6 tetrahedra, stored as one mesh with the cases'
  * cells in order (case k owns cells offsets[k] .. offsets[k+1]).  Builds
  * solver_mesh like fvh_mesh_box; the usual init sequence follows
@@ -250,6 +250,20 @@ int fvh_batch_setup(int n_cases, const int *flux, const int *viscous,
  * replays a captured HIP graph of one step.  seconds: device time. */
 int fvh_batch_run(int steps, int use_graph, double *seconds);
 int fvh_batch_info(int *n_cases, int *offsets, int max_offsets);
+/* Step schedule: persistent = 0 (the default) launches the stage kernel per
+ * (flux, viscous) group of cases and reduces the per-case norms in a pass
+ * of its own; persistent != 0 runs each stage as one persistent launch per
+ * (flux, viscous) class of cases (workgroups of `block` = 128, 256 or 512
+ * threads take tiles of sub_tiles * block / lanes cells of one case from a
+ * work queue, lanes = 4 on tetrahedra; `grid` workgroups, 0 = min(tiles,
+ * 4 x CUs)) and reduces in its last stage.  Both run one gradient launch
+ * per run of viscous cases and give the same states and time steps. */
+int fvh_batch_set_schedule(int persistent, int block, int grid, int sub_tiles);
+/* Per-case step time inside the batched launches (persistent schedule):
+ * runs `steps` steps and returns in ms[K] each case's mean per step of the
+ * time from the start of its first tile to the end of its last, summed over
+ * the stages (device wall clock). */
+int fvh_batch_case_step_times(int steps, double *ms);
 /* Per case: next dt [K], time [K], residual norms of the last step [K*5]
  * (analyze.c norms with the case's volume and dt). */
 int fvh_batch_case_scalars(double *dt, double *t, double *res);

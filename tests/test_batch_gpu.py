@@ -42,10 +42,24 @@ def _oracle_runs():
     return out
 
 
+SCHEDULES = {"group": dict(persistent=False),
+             "persistent-128": dict(persistent=True, block=128, grid=0),
+             "persistent-256-grid3": dict(persistent=True, block=256, grid=3),
+             "persistent-512": dict(persistent=True, block=512, grid=0),
+             "persistent-256-sub4": dict(persistent=True, block=256, grid=0, sub_tiles=4),
+             "persistent-512-grid5-sub16": dict(persistent=True, block=512, grid=5,
+                                                sub_tiles=16)}
+
+
 @pytest.mark.parametrize("graph", [False, True])
-def test_batch_matches_single_case_oracle(graph):
+@pytest.mark.parametrize("schedule", list(SCHEDULES))
+def test_batch_matches_single_case_oracle(graph, schedule):
+    """Every schedule (per-group launches; persistent launches per class
+    with tiles of one, 4 or 16 runs of 32, 64 or 128 cells, some ending
+    inside a run, and with 3 or 5 workgroups looping over all tiles) gives
+    each case its own oracle run."""
     ref = _oracle_runs()
-    b = B.Batch(CASES, tile=TILE, mu_ns=MU)
+    b = B.Batch(CASES, tile=TILE, mu_ns=MU, schedule=SCHEDULES[schedule])
     b.run(STEPS, graph=graph)
     dt, t, res = b.scalars()
     worst = 0.0
@@ -56,5 +70,23 @@ def test_batch_matches_single_case_oracle(graph):
         assert e <= 1e-12, (i, e)
         assert abs(t[k] - log[-1, 0]) <= 1e-13 * log[-1, 0]
         assert np.allclose(res[k], log[-1, 3:8], rtol=1e-10, atol=0)
-    print("batch graph=%s worst rel=%g" % (graph, worst), flush=True)
+    print("batch %s graph=%s worst rel=%g" % (schedule, graph, worst), flush=True)
+    b.close()
+
+
+def test_batch_case_step_times_inside_the_launch():
+    """Per-case step times measured inside the shared persistent launches:
+    one positive time per case, the larger cases taking longer; the steps
+    they time are ordinary steps (the states still match the oracle)."""
+    ref = _oracle_runs()
+    b = B.Batch(CASES, tile=TILE, mu_ns=MU,
+                schedule=dict(persistent=True, block=256, grid=0, sub_tiles=4))
+    b.run(STEPS - 2, graph=True)
+    ms = b.case_step_times(2)
+    assert ms.shape == (len(CASES),) and np.all(ms > 0) and np.all(np.isfinite(ms))
+    sizes = np.array([c["n_hex"] ** 3 for c in b.cases])
+    assert ms[np.argmax(sizes)] >= ms[np.argmin(sizes)]
+    for k, i in enumerate(b.order):
+        assert rel_err(b.state(k), ref[i][0]) <= 1e-12
+    print("per-case step ms inside the batch", ms, flush=True)
     b.close()
