@@ -76,6 +76,16 @@ struct DevMesh
     const int *__restrict__ tile_halo;
     const unsigned short *__restrict__ nbl;
     int tile_rows;
+    // faces of each cell tile for the face-centred residual kernel (null: not
+    // built; tetrahedral meshes): tile t's faces are tile_faces[
+    // tile_face_off[t] .. tile_face_off[t+1]), each face once, {face, row of
+    // the left state, row of the right state (N*4 + b: ghost b), slots}; the
+    // rows index the face-state table ([N*4 + B] rows of 8), the slots are
+    // the tile-local slot (cell - t*kTile)*4 + s that receives +flux*area
+    // (low 16 bits) and the one that receives -flux*area (high 16 bits),
+    // 0xffff where that side's cell is outside the tile
+    const int *__restrict__ tile_face_off;
+    const int4 *__restrict__ tile_faces;
 };
 
 // cells per tile of the LDS-staged gradient kernel

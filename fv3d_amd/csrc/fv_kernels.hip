@@ -376,6 +376,9 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
 #ifndef FVH_K1T_MINW
 #define FVH_K1T_MINW 2
 #endif
+#ifndef FVH_K1T_PREREC
+#define FVH_K1T_PREREC 0
+#endif
 template <int MAXF>
 __global__ __launch_bounds__(kTile, FVH_K1T_MINW) void fv_gradient_tile_kernel(
     DevMesh m, PhysDev P, const double *__restrict__ U,
@@ -458,21 +461,27 @@ __global__ __launch_bounds__(kTile, FVH_K1T_MINW) void fv_gradient_tile_kernel(
         own[s] = e.y >= 0;
         loc[s] = nbl[sl];
     }
-    // the face records of all faces first (one round of gathers)
+    // each face's record in its own iteration (FVH_K1T_PREREC 0: 158 VGPRs,
+    // 3 waves per SIMD); the records of all faces first (one round of
+    // gathers) need 178, 2 waves, and ran 17% slower on tetrahedra
     double gn[MAXF][5];
-#pragma unroll
-    for (int s = 0; s < MAXF; ++s)
-    {
+    auto load_rec = [&](int s) {
         const double2 *g = reinterpret_cast<const double2 *>(m.fgB + (size_t)fid[s] * kFgB);
         const double2 g0 = g[0], g1 = g[1], g2 = g[2];
         gn[s][0] = g0.x, gn[s][1] = g0.y, gn[s][2] = g1.x;
         gn[s][3] = g1.y, gn[s][4] = g2.x;
-    }
+    };
+    if (FVH_K1T_PREREC)
+#pragma unroll
+        for (int s = 0; s < MAXF; ++s)
+            load_rec(s);
 #pragma unroll
     for (int s = 0; s < MAXF; ++s)
     {
         if (s >= nf)
             break;
+        if (!FVH_K1T_PREREC)
+            load_rec(s);
         double pn[10];
 #pragma unroll
         for (int j = 0; j < 10; ++j)
@@ -1381,10 +1390,259 @@ void launch_gradient(hipStream_t s, const DevMesh &m, const PhysDev &P,
                            sc);
 }
 
+// ---------------------------------------------------------------------------
+// K2 on face states, face-centred (tetrahedral meshes): the workgroup of tile
+// t evaluates each face of its kTile cells once (a face between two of the
+// tile's cells gives both their contributions; a face to a cell outside the
+// tile is also evaluated by that cell's tile), with the same states,
+// gradients and flux as the lane-per-slot kernel, and keeps the
+// contributions +-flux*area per cell slot in LDS.  Then one thread per
+// (cell, variable) sums the cell's slots in slot order (the lane-per-slot
+// kernel's order) and applies the 2N-storage RK stage, so that the stores of
+// a wave are contiguous; in the last stage one thread per cell forms the
+// residual-norm and time-step partials, one row per 64 cells (the rows of
+// the lane-per-slot kernel's grid).  LDS: kTile * 4 * 5 doubles.
+// ---------------------------------------------------------------------------
+#ifndef FVH_K2F_MINW
+#define FVH_K2F_MINW 3
+#endif
+// 1: the update's operands are loaded before the face work (20 more VGPRs)
+#ifndef FVH_K2F_PRE
+#define FVH_K2F_PRE 0
+#endif
+template <int FLUX, bool VISC>
+__global__ __launch_bounds__(kTile, FVH_K2F_MINW) void fv_residual_face_kernel(DevMesh m,
+                                                                              PhysDev P,
+                                                                              StageParams sp)
+{
+    if (sp.sc && sp.sc->done)
+        return;
+    extern __shared__ double ct[]; // [kTile * 4][5]
+    const int N = m.N;
+    const int t = xcd_tile(gridDim.x);
+    const int base = t * kTile;
+    const int ncell = min(kTile, N - base);
+    const int e0 = m.tile_face_off[t], e1 = m.tile_face_off[t + 1];
+    // element p = lc * 5 + j of the tile's rows; with FVH_K2F_PRE the
+    // update's operands are loaded before the face work so that their
+    // latency overlaps it
+    constexpr int kE = 5; // elements per thread: kTile * 5 / kTile
+    double u_in[kE], k_in[kE];
+    if (FVH_K2F_PRE)
+#pragma unroll
+        for (int i = 0; i < kE; ++i)
+        {
+            const int p = (int)threadIdx.x + i * kTile;
+            const size_t g = (size_t)base * 5 + (p < ncell * 5 ? p : 0);
+            u_in[i] = sp.do_update ? sp.U_in[g] : 0.0;
+            k_in[i] = (sp.do_update && !sp.stage0) ? sp.k[g] : 0.0;
+        }
+    // the next face's record is loaded while this face is computed
+    int e = e0 + (int)threadIdx.x;
+    int4 r_next = m.tile_faces[e < e1 ? e : e0];
+    for (; e < e1; e += kTile)
+    {
+        const int4 r = r_next;
+        if (e + kTile < e1)
+            r_next = m.tile_faces[e + kTile];
+        const int f = r.x;
+        const bool ghost = r.z >= N * 4;
+        Frame fr;
+        load_frame(m, f, fr);
+        double L[10], R[10];
+        {
+            const double2 *pl = reinterpret_cast<const double2 *>(sp.fs + (size_t)r.y * 8);
+            const double2 *pr = reinterpret_cast<const double2 *>(sp.fs + (size_t)r.z * 8);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+            {
+                const double2 a = pl[q], b = pr[q];
+                L[2 * q] = a.x;
+                L[2 * q + 1] = a.y;
+                R[2 * q] = b.x;
+                R[2 * q + 1] = b.y;
+            }
+            L[8] = L[9] = R[8] = R[9] = 0.0;
+        }
+        ViscGrad gL, gR;
+        if (VISC)
+        {
+            // the left cell's row, and the right cell's (the left one's
+            // stands in for a ghost, whose gradient follows from it)
+            load_vg(sp.vg, r.y >> 2, gL);
+            load_vg(sp.vg, ghost ? (r.y >> 2) : (r.z >> 2), gR);
+            if (ghost)
+            {
+                const int type = m.region_type[m.bnd_region[r.z - N * 4]];
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    ghost_grad(type, fr, gL.v[q], gL.v[4 + q], gL.v[8 + q], gR.v[q],
+                               gR.v[4 + q], gR.v[8 + q]);
+            }
+        }
+        double flux[5];
+        face_flux<FLUX, VISC>(P, fr, L, R, gL, gR, flux);
+        const int dA = r.w & 0xffff, dB = (r.w >> 16) & 0xffff;
+#pragma unroll
+        for (int j = 0; j < 5; ++j)
+        {
+            const double v = flux[j] * fr.area;
+            if (dA != 0xffff)
+                ct[dA * 5 + j] = v;
+            if (dB != 0xffff)
+                ct[dB * 5 + j] = -v;
+        }
+    }
+    __syncthreads();
+
+    // per (cell, variable)
+    const double dt = sp.do_update ? sp.sc->dt : 0.0;
+    double r_el[kE], u_el[kE];
+#pragma unroll
+    for (int i = 0; i < kE; ++i)
+    {
+        const int p = (int)threadIdx.x + i * kTile;
+        const int lc = p / 5, j = p - lc * 5;
+        r_el[i] = 0.0;
+        u_el[i] = 0.0;
+        if (lc >= ncell)
+            continue;
+        const int c = base + lc;
+        const int nf = m.ncf[c];
+        double a = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (q < nf)
+                a = a + ct[(lc * 4 + q) * 5 + j];
+        const double Rj = -a / m.vol[c];
+        const size_t g = (size_t)base * 5 + p;
+        r_el[i] = Rj;
+        if (sp.R_out)
+            sp.R_out[g] = Rj;
+        if (sp.do_update)
+        {
+            double kj = Rj;
+            if (!sp.stage0)
+                kj = Rj + (FVH_K2F_PRE ? k_in[i] : sp.k[g]) * sp.rk_a;
+            const double un = (FVH_K2F_PRE ? u_in[i] : sp.U_in[g]) + kj * dt * sp.rk_g;
+            u_el[i] = un;
+            sp.k[g] = kj;
+            sp.U_out[g] = un;
+        }
+    }
+    if (!sp.do_reduce && !(sp.do_update && sp.prims_out))
+        return;
+
+    // per cell: the new state and residual rows through LDS (the
+    // contributions are no longer read), then the primitives of the new
+    // state and the partials of the last stage
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kE; ++i)
+    {
+        const int p = (int)threadIdx.x + i * kTile;
+        if (p < ncell * 5)
+        {
+            ct[p] = u_el[i];
+            ct[kTile * 5 + p] = r_el[i];
+        }
+    }
+    __syncthreads();
+    const int lc = threadIdx.x;
+    const int c = base + lc;
+    double part[7] = {0.0, 0.0, 0.0, 0.0, 0.0, DBL_MAX, DBL_MAX};
+    if (lc < ncell && sp.do_update)
+    {
+        double unew[5], R5[5], pc[10];
+#pragma unroll
+        for (int j = 0; j < 5; ++j)
+        {
+            unew[j] = ct[lc * 5 + j];
+            R5[j] = ct[kTile * 5 + lc * 5 + j];
+            pc[j] = sp.U_in[(size_t)c * 5 + j];
+        }
+        if (sp.prims_out)
+        {
+            double q[10];
+#pragma unroll
+            for (int j = 0; j < 5; ++j)
+                q[j] = unew[j];
+            con_to_prim(P, q);
+#pragma unroll
+            for (int j = 0; j < 5; ++j)
+                sp.prims_out[(size_t)c * 5 + j] = q[5 + j];
+        }
+        if (sp.do_reduce)
+        {
+            // time step with the new density and the primitives of this
+            // stage's input (navier_stokes.c:171-187)
+            if (sp.prims_in)
+#pragma unroll
+                for (int j = 0; j < 5; ++j)
+                    pc[5 + j] = sp.prims_in[(size_t)c * 5 + j];
+            else
+                con_to_prim(P, pc);
+            const double vol = m.vol[c];
+#pragma unroll
+            for (int j = 0; j < 5; ++j)
+                part[j] = rabs(R5[j]) * vol;
+            const double dx0 = m.cdx[(size_t)c * 3], dx1 = m.cdx[(size_t)c * 3 + 1],
+                         dx2 = m.cdx[(size_t)c * 3 + 2];
+            const double s_rho = 1.0 / unew[0];
+            const double cs = sqrt(P.kappa * pc[8] * s_rho);
+            const double ds1 = (rabs(pc[5]) + cs) * dx0 + (rabs(pc[6]) + cs) * dx1 +
+                               (rabs(pc[7]) + cs) * dx2;
+            part[5] = vol / ds1;
+            const double ds2 = dot3(dx0, dx1, dx2, dx0, dx1, dx2);
+            part[6] = s_rho * ds2 / vol;
+        }
+    }
+    if (!sp.do_reduce)
+        return;
+    // one row per wave of 64 cells
+    const int wrow = (base + (lc & ~63)) / 64;
+    if (base + (lc & ~63) >= N)
+        return;
+    double w[7];
+#pragma unroll
+    for (int q = 0; q < 5; ++q)
+        w[q] = wave_sum(part[q]);
+    w[5] = wave_min(part[5]);
+    w[6] = wave_min(part[6]);
+    if ((lc & 63) == 0)
+#pragma unroll
+        for (int q = 0; q < 7; ++q)
+            sp.partials[(size_t)wrow * kPartialStride + q] = w[q];
+}
+
+// Whether the face-centred kernel runs a residual launch: FP64 face states
+// over the whole tetrahedral mesh with the face tables built, the stage's
+// dt from sc.
+static bool use_face_kernel(const DevMesh &m, const StageParams &sp, bool fs, bool fp32)
+{
+    return fs && !fp32 && m.tile_faces && m.maxf == 4 && sp.c_begin == 0 && sp.c_end == m.N &&
+           !sp.case_dt && sp.fs && (!sp.do_update || sp.sc);
+}
+
+template <int FLUX, bool VISC>
+static void launch_face_t(hipStream_t s, const DevMesh &m, const PhysDev &P,
+                          const StageParams &sp)
+{
+    const int tiles = (m.N + kTile - 1) / kTile;
+    const size_t lds = (size_t)kTile * 4 * 5 * sizeof(double);
+    hipLaunchKernelGGL((fv_residual_face_kernel<FLUX, VISC>), dim3(tiles), dim3(kTile), lds, s,
+                       m, P, sp);
+}
+
 template <int FLUX, bool FS, bool VISC>
 static void launch_res_t(hipStream_t s, const DevMesh &m, const PhysDev &P,
                          const StageParams &sp, bool fp32)
 {
+    if (FS && use_face_kernel(m, sp, FS, fp32))
+    {
+        launch_face_t<FLUX, VISC>(s, m, P, sp);
+        return;
+    }
     const int lanes = m.maxf <= 4 ? 4 : 8;
     const int grid = cell_blocks((sp.c_end - sp.c_begin) * lanes);
     if (fp32)

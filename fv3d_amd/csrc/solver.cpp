@@ -715,6 +715,76 @@ void setup_tile_halo()
 }
 
 
+// Face tables of the face-centred residual kernel (tetrahedral meshes): per
+// cell tile every face of its cells once, a face between two of its cells
+// from the side-0 cell, with the face-state rows of its two sides and the
+// tile-local slots that receive its contributions.  A cell that is its own
+// periodic neighbour keeps one entry per slot, as the lane-per-slot kernel
+// evaluates it: its own row left, the face's other slot right.
+void setup_tile_faces()
+{
+    const char *e = std::getenv("FVH_K2_FACE");
+    if (e && std::atoi(e) == 0)
+        return;
+    const int N = g_dev.m.N, maxf = g_dev.m.maxf, T = dev::kTile;
+    if (maxf != 4)
+        return;
+    const Cells_t *cells = solver_mesh->cells;
+    const Faces_t *faces = solver_mesh->faces;
+    auto slot_of = [&](int x, int f, int not_slot) {
+        for (int t = 0; t < cells->n_faces[x]; ++t)
+            if (cells->faces[(size_t)x * maxf + t] == f && t != not_slot)
+                return t;
+        fatal("face " + std::to_string(f) + " is missing from the face list of cell " +
+              std::to_string(x));
+        return -1;
+    };
+    const int nt = (N + T - 1) / T;
+    std::vector<int> off(nt + 1, 0);
+    std::vector<int4> ent;
+    ent.reserve((size_t)N * 5 / 2);
+    for (int t = 0; t < nt; ++t)
+    {
+        const int b0 = t * T, b1 = std::min(N, b0 + T);
+        auto in_tile = [&](int x) { return x >= b0 && x < b1; };
+        for (int c = b0; c < b1; ++c)
+            for (int s = 0; s < cells->n_faces[c]; ++s)
+            {
+                const int f = cells->faces[(size_t)c * maxf + s];
+                const int c0 = faces->cells[2 * f], c1 = faces->cells[2 * f + 1];
+                int rowL, rowR, dA, dB = 0xffff;
+                if (c0 == c1)
+                {
+                    rowL = c * 4 + s;
+                    rowR = c * 4 + slot_of(c, f, s);
+                    dA = (c - b0) * 4 + s;
+                }
+                else
+                {
+                    const bool interior = c1 < N;
+                    if (interior && c == c1 && in_tile(c0))
+                        continue; // evaluated from its side-0 cell
+                    const int s0 = c == c0 ? s : slot_of(c0, f, -1);
+                    rowL = c0 * 4 + s0;
+                    dA = in_tile(c0) ? (c0 - b0) * 4 + s0 : 0xffff;
+                    if (interior)
+                    {
+                        const int s1 = c == c1 ? s : slot_of(c1, f, -1);
+                        rowR = c1 * 4 + s1;
+                        if (in_tile(c1))
+                            dB = (c1 - b0) * 4 + s1;
+                    }
+                    else
+                        rowR = N * 4 + (c1 - N);
+                }
+                ent.push_back(make_int4(f, rowL, rowR, dA | (dB << 16)));
+            }
+        off[t + 1] = (int)ent.size();
+    }
+    g_dev.m.tile_face_off = dupload(off.data(), off.size());
+    g_dev.m.tile_faces = dupload(ent.data(), ent.size());
+}
+
 void sync_mirror(bool with_dt)
 {
     const int N = g_dev.m.N, B = g_dev.m.B;
@@ -1901,6 +1971,7 @@ void init_finite_volume(void)
         g_dev.ready = true;
         noalloc_setup();
         setup_tile_halo();
+        setup_tile_faces();
 
         // set_solution (finite_volume.c:189-201): exact function per domain
         // cell on the host, then the update hook's work on the device

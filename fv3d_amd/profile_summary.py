@@ -23,7 +23,10 @@ import sys
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GIB = float(1 << 30)
-KERNELS = {"gradient": "fv_gradient_tile_kernel", "residual": "fv_residual_group_kernel<0, true, true, 4, double>"}
+# the stage's two kernels (the residual kernel: face-centred, or lane per slot)
+KERNELS = {"gradient": ["fv_gradient_tile_kernel"],
+           "residual": ["fv_residual_face_kernel<0, true>",
+                        "fv_residual_group_kernel<0, true, true, 4, double>"]}
 
 
 def _one(d, suffix):
@@ -52,10 +55,11 @@ def counters(d):
     return acc
 
 
-def _match(acc, key):
-    for name, v in acc.items():
-        if key in name:
-            return name, v
+def _match(acc, keys):
+    for key in ([keys] if isinstance(keys, str) else keys):
+        for name, v in acc.items():
+            if key in name:
+                return name, v
     return None, None
 
 
@@ -130,9 +134,9 @@ def main(argv=None):
                 print("- %s: %.6g (mean of %d dispatches)" % (c, mean(vals), len(vals)))
     if a.json:
         avg = {}
-        for tag, key in KERNELS.items():
+        for tag, keys in KERNELS.items():
             for r in st:
-                if key in r["name"]:
+                if any(k in r["name"] for k in keys) and tag not in avg:
                     avg[tag] = r["avg_us"]
         with open(a.json, "w") as f:
             json.dump({"n": a.n, "sources_sha256": source_hash(),
