@@ -1406,9 +1406,14 @@ void launch_gradient(hipStream_t s, const DevMesh &m, const PhysDev &P,
 #ifndef FVH_K2F_MINW
 #define FVH_K2F_MINW 3
 #endif
-// 1: the update's operands are loaded before the face work (20 more VGPRs)
+// 1: the update's operands are loaded before the face work (20 more VGPRs);
+// 2: only the RK register
 #ifndef FVH_K2F_PRE
-#define FVH_K2F_PRE 0
+#define FVH_K2F_PRE 2
+#endif
+// 1: each XCD walks its chunk of tiles from the end (see xcd_tile_rev)
+#ifndef FVH_K2F_REV
+#define FVH_K2F_REV 1
 #endif
 template <int FLUX, bool VISC>
 __global__ __launch_bounds__(kTile, FVH_K2F_MINW) void fv_residual_face_kernel(DevMesh m,
@@ -1419,7 +1424,7 @@ __global__ __launch_bounds__(kTile, FVH_K2F_MINW) void fv_residual_face_kernel(D
         return;
     extern __shared__ double ct[]; // [kTile * 4][5]
     const int N = m.N;
-    const int t = xcd_tile(gridDim.x);
+    const int t = FVH_K2F_REV ? xcd_tile_rev(gridDim.x) : xcd_tile(gridDim.x);
     const int base = t * kTile;
     const int ncell = min(kTile, N - base);
     const int e0 = m.tile_face_off[t], e1 = m.tile_face_off[t + 1];
@@ -1434,7 +1439,7 @@ __global__ __launch_bounds__(kTile, FVH_K2F_MINW) void fv_residual_face_kernel(D
         {
             const int p = (int)threadIdx.x + i * kTile;
             const size_t g = (size_t)base * 5 + (p < ncell * 5 ? p : 0);
-            u_in[i] = sp.do_update ? sp.U_in[g] : 0.0;
+            u_in[i] = (sp.do_update && FVH_K2F_PRE == 1) ? sp.U_in[g] : 0.0;
             k_in[i] = (sp.do_update && !sp.stage0) ? sp.k[g] : 0.0;
         }
     // the next face's record is loaded while this face is computed
@@ -1524,7 +1529,7 @@ __global__ __launch_bounds__(kTile, FVH_K2F_MINW) void fv_residual_face_kernel(D
             double kj = Rj;
             if (!sp.stage0)
                 kj = Rj + (FVH_K2F_PRE ? k_in[i] : sp.k[g]) * sp.rk_a;
-            const double un = (FVH_K2F_PRE ? u_in[i] : sp.U_in[g]) + kj * dt * sp.rk_g;
+            const double un = (FVH_K2F_PRE == 1 ? u_in[i] : sp.U_in[g]) + kj * dt * sp.rk_g;
             u_el[i] = un;
             sp.k[g] = kj;
             sp.U_out[g] = un;
