@@ -144,13 +144,14 @@ def _cus(L):
     return max(1, cus.value)
 
 
-def autotune(cases, tiles=(4, 8), blocks=(256, 512), grids=(2, 4, 8), subs=(1, 4, 16, 64),
+def autotune(cases, tiles=(4, 8), blocks=(256, 512), grids=(2, 4, 8, -1), subs=(1, 4, 16),
              steps=10, warmup=2, progress=None, **batch_kwargs):
     """Times one graph-replayed batch step for every cell-ordering tile of
     the case meshes (hexahedra per side of a locality tile), persistent
     workgroup size (256 or 512 threads: runs of 64 or 128 tetrahedra),
-    persistent grid (2, 4, 8 workgroups per CU) and runs per work-queue tile
-    (1, 4, 16, 64), and the per-group schedule; returns (best, table).
+    persistent grid (2, 4, 8 workgroups per CU, or -1: one workgroup per tile
+    without the work queue) and runs per tile (1, 4, 16), and the per-group
+    schedule; returns (best, table).
     `progress`, when given, is called with each table row as it is
     measured.  The explicit path has no cell colouring, so there is no
     colour-block size to tune."""
@@ -162,10 +163,11 @@ def autotune(cases, tiles=(4, 8), blocks=(256, 512), grids=(2, 4, 8), subs=(1, 4
         for block in blocks:
             for sub in subs:
                 for g in grids:
-                    b.set_schedule(True, block, g * cus, sub)
+                    grid = g * cus if g > 0 else -1
+                    b.set_schedule(True, block, grid, sub)
                     b.run(1, graph=True)
                     sec = b.run(steps, graph=True)
-                    table.append(dict(tile=tile, persistent=True, block=block, grid=g * cus,
+                    table.append(dict(tile=tile, persistent=True, block=block, grid=grid,
                                       sub_tiles=sub, ms_per_step=sec / steps * 1e3))
                     if progress:
                         progress(table[-1])

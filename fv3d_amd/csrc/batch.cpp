@@ -57,7 +57,7 @@ struct BatchState
     int steps = 0;
     // persistent schedule
     bool persistent = false; // the per-group schedule measured faster
-    int block = 256, grid = 0; // grid 0: min(tiles, 4 x CUs)
+    int block = 256, grid = 0; // grid 0: min(tiles, 4 x CUs); -1: one workgroup per tile
     int sub_tiles = 1;         // runs of block / lanes cells per tile
     // tiles by class (viscous * 2 + flux: AUSM and Rusanov inviscid, then
     // viscous), each case's tiles consecutive
@@ -223,13 +223,15 @@ dev::BatchTiles tiles_view(int stage, int cls, unsigned long long *stamps)
     bt.case_end = g_b.case_end;
     bt.case_mu = g_b.d_mu;
     bt.case_lam = g_b.d_lam;
-    bt.counter = g_b.counters + stage * kClasses + cls;
+    bt.counter = g_b.grid < 0 ? nullptr : g_b.counters + stage * kClasses + cls;
     bt.stamps = stamps;
     return bt;
 }
 
 int persistent_grid(int n_tiles)
 {
+    if (g_b.grid < 0)
+        return n_tiles; // one workgroup per tile
     int grid = g_b.grid;
     if (grid <= 0)
     {

@@ -163,7 +163,7 @@ struct BatchTiles
     const int *tile_begin; // [all tiles] first cell of the tile (tiles end with their case)
     const int *case_end;   // [K] one past the case's last cell
     const double *case_mu, *case_lam; // [K]
-    unsigned int *counter; // zero before the launch
+    unsigned int *counter; // zero before the launch; null: one workgroup per tile
     unsigned long long *stamps; // [all tiles][2] wall-clock start/end per tile (null: off)
 };
 
@@ -209,7 +209,8 @@ void launch_dt_partials(hipStream_t s, const DevMesh &m, const PhysDev &P,
 // cell_case; partials per tile when do_reduce) over the tiles of bt, all of
 // cases with flux `flux` and viscosity on or off (`viscous`): `block`
 // threads per workgroup (128, 256 or 512; a tile is bt.sub_tiles runs of
-// block / lanes cells), `grid` persistent workgroups.
+// block / lanes cells), `grid` persistent workgroups taking tiles from the
+// work queue bt.counter, or with bt.counter null one workgroup per tile.
 void launch_batch_stage(hipStream_t s, const DevMesh &m, const PhysDev &P,
                         const StageParams &sp, const BatchTiles &bt, int flux, bool viscous,
                         int block, int grid);

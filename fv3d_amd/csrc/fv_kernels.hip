@@ -916,6 +916,34 @@ __global__ __launch_bounds__(kBlock, (FS && VISC && std::is_same<Real, double>::
 // per SIMD of the single-case kernel of the same variant (the tile loop
 // would otherwise keep enough loop-invariant values live to lose one).
 // ---------------------------------------------------------------------------
+// Tile t of a batched launch: its runs of BS / LANES cells one after the
+// other, with its case's viscosity.
+template <int BS, int LANES, int FLUX, bool VISC>
+__device__ __forceinline__ void batch_tile_body(const DevMesh &m, const PhysDev &P,
+                                                const StageParams &sp, const BatchTiles &bt,
+                                                int t, double *sh)
+{
+    constexpr int GC = BS / LANES;
+    const int k = bt.tile_case[t];
+    const int begin = bt.tile_begin[t], end = bt.case_end[k];
+    PhysDev Pk = P;
+    Pk.mu = bt.case_mu[k];
+    Pk.lam = bt.case_lam[k];
+    const int s = threadIdx.x & (LANES - 1);
+    for (int u = 0; u < bt.sub_tiles; ++u)
+    {
+        StageParams q = sp;
+        q.c_begin = begin + u * GC;
+        if (q.c_begin >= end)
+            break;
+        q.c_end = min(q.c_begin + GC, end);
+        const int c = q.c_begin + (int)threadIdx.x / LANES;
+        double *row = sp.partials + ((size_t)t * bt.sub_tiles + u) * kPartialStride;
+        residual_group_body<FLUX, false, VISC, LANES, double>(m, Pk, q, c, s, c < q.c_end, sh,
+                                                             row);
+    }
+}
+
 template <int BS, int LANES, int FLUX, bool VISC>
 __global__ __launch_bounds__(BS, LANES == 4 ? (VISC ? 3 : 5) : (VISC ? 2 : 3)) void
 fv_batch_stage_kernel(DevMesh m, PhysDev P, StageParams sp,
@@ -923,7 +951,6 @@ fv_batch_stage_kernel(DevMesh m, PhysDev P, StageParams sp,
 {
     __shared__ double sh[8 * BS / 64];
     __shared__ int s_tile;
-    constexpr int GC = BS / LANES;
     // One lane-0 section per iteration (end stamp of the previous tile, the
     // next tile, its start stamp) and a wave-uniform tile index: the loop
     // exit is a scalar branch and every barrier sits in code that all lanes
@@ -953,24 +980,27 @@ fv_batch_stage_kernel(DevMesh m, PhysDev P, StageParams sp,
         t = __builtin_amdgcn_readfirstlane(s_tile);
         if (t < 0)
             return;
-        const int k = bt.tile_case[t];
-        const int begin = bt.tile_begin[t], end = bt.case_end[k];
-        PhysDev Pk = P;
-        Pk.mu = bt.case_mu[k];
-        Pk.lam = bt.case_lam[k];
-        const int s = threadIdx.x & (LANES - 1);
-        for (int u = 0; u < bt.sub_tiles; ++u)
-        {
-            StageParams q = sp;
-            q.c_begin = begin + u * GC;
-            if (q.c_begin >= end)
-                break;
-            q.c_end = min(q.c_begin + GC, end);
-            const int c = q.c_begin + (int)threadIdx.x / LANES;
-            double *row = sp.partials + ((size_t)t * bt.sub_tiles + u) * kPartialStride;
-            residual_group_body<FLUX, false, VISC, LANES, double>(m, Pk, q, c, s, c < q.c_end,
-                                                                 sh, row);
-        }
+        batch_tile_body<BS, LANES, FLUX, VISC>(m, P, sp, bt, t, sh);
+    }
+}
+
+// The same tiles, one workgroup each (no work queue: the hardware
+// dispatcher hands out the tiles), t = bt.tile_base + blockIdx.x.
+template <int BS, int LANES, int FLUX, bool VISC>
+__global__ __launch_bounds__(BS, LANES == 4 ? (VISC ? 3 : 5) : (VISC ? 2 : 3)) void
+fv_batch_tile_kernel(DevMesh m, PhysDev P, StageParams sp,
+                                                           BatchTiles bt)
+{
+    __shared__ double sh[8 * BS / 64];
+    const int t = bt.tile_base + (int)blockIdx.x;
+    if (bt.stamps && threadIdx.x == 0)
+        bt.stamps[(size_t)t * 2] = wall_clock64();
+    batch_tile_body<BS, LANES, FLUX, VISC>(m, P, sp, bt, t, sh);
+    if (bt.stamps)
+    {
+        __syncthreads();
+        if (threadIdx.x == 0)
+            bt.stamps[(size_t)t * 2 + 1] = wall_clock64();
     }
 }
 
@@ -978,6 +1008,22 @@ template <int LANES, int FLUX, bool VISC>
 static void launch_batch_lanes(hipStream_t s, const DevMesh &m, const PhysDev &P,
                                const StageParams &sp, const BatchTiles &bt, int block, int grid)
 {
+    if (!bt.counter)
+    {
+        // one workgroup per tile
+        if (block == 128)
+            hipLaunchKernelGGL((fv_batch_tile_kernel<128, LANES, FLUX, VISC>), dim3(bt.n_tiles),
+                               dim3(128), 0, s, m, P, sp, bt);
+        else if (block == 256)
+            hipLaunchKernelGGL((fv_batch_tile_kernel<256, LANES, FLUX, VISC>), dim3(bt.n_tiles),
+                               dim3(256), 0, s, m, P, sp, bt);
+        else if (block == 512)
+            hipLaunchKernelGGL((fv_batch_tile_kernel<512, LANES, FLUX, VISC>), dim3(bt.n_tiles),
+                               dim3(512), 0, s, m, P, sp, bt);
+        else
+            throw std::runtime_error("launch_batch_stage: block must be 128, 256 or 512");
+        return;
+    }
     if (block == 128)
         hipLaunchKernelGGL((fv_batch_stage_kernel<128, LANES, FLUX, VISC>), dim3(grid),
                            dim3(128), 0, s, m, P, sp, bt);

@@ -256,7 +256,8 @@ int fvh_batch_info(int *n_cases, int *offsets, int max_offsets);
  * (flux, viscous) class of cases (workgroups of `block` = 128, 256 or 512
  * threads take tiles of sub_tiles * block / lanes cells of one case from a
  * work queue, lanes = 4 on tetrahedra; `grid` workgroups, 0 = min(tiles,
- * 4 x CUs)) and reduces in its last stage.  Both run one gradient launch
+ * 4 x CUs), -1 = one workgroup per tile and no queue) and reduces in its
+ * last stage.  Both run one gradient launch
  * per run of viscous cases and give the same states and time steps. */
 int fvh_batch_set_schedule(int persistent, int block, int grid, int sub_tiles);
 /* Per-case step time inside the batched launches (persistent schedule):

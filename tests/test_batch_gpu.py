@@ -48,7 +48,9 @@ SCHEDULES = {"group": dict(persistent=False),
              "persistent-512": dict(persistent=True, block=512, grid=0),
              "persistent-256-sub4": dict(persistent=True, block=256, grid=0, sub_tiles=4),
              "persistent-512-grid5-sub16": dict(persistent=True, block=512, grid=5,
-                                                sub_tiles=16)}
+                                                sub_tiles=16),
+             "tiles-256": dict(persistent=True, block=256, grid=-1),
+             "tiles-128-sub4": dict(persistent=True, block=128, grid=-1, sub_tiles=4)}
 
 
 @pytest.mark.parametrize("graph", [False, True])
@@ -56,8 +58,8 @@ SCHEDULES = {"group": dict(persistent=False),
 def test_batch_matches_single_case_oracle(graph, schedule):
     """Every schedule (per-group launches; persistent launches per class
     with tiles of one, 4 or 16 runs of 32, 64 or 128 cells, some ending
-    inside a run, and with 3 or 5 workgroups looping over all tiles) gives
-    each case its own oracle run."""
+    inside a run, and with 3 or 5 workgroups looping over all tiles; one
+    workgroup per tile) gives each case its own oracle run."""
     ref = _oracle_runs()
     b = B.Batch(CASES, tile=TILE, mu_ns=MU, schedule=SCHEDULES[schedule])
     b.run(STEPS, graph=graph)
