@@ -96,7 +96,7 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=96, help="hexahedra per side")
-    ap.add_argument("--tile", type=int, default=8, help="hexahedra per side of a "
+    ap.add_argument("--tile", type=int, default=4, help="hexahedra per side of a "
                     "cell-ordering tile of the mesh")
     ap.add_argument("--cpu-sample-n", type=int, default=64)
     ap.add_argument("--cpu-sample-steps", type=int, default=4)

@@ -26,7 +26,7 @@ def density_wave_box(n=16, steps=100, flux="Rusanov", scheme="RK3-3"):
     )
 
 
-def taylor_green(n=96, steps=200, tile=8, mach=0.1, reynolds=1600.0):
+def taylor_green(n=96, steps=200, tile=4, mach=0.1, reynolds=1600.0):
     """Navier-Stokes Taylor-Green vortex on [0, 2 pi]^3 periodic, n^3
     hexahedra each Kuhn-split into 6 tetrahedra, rho0 = U0 = 1, R = 1,
     p0 = 1/(kappa Ma^2), mu = 1/Re; AUSM, linear reconstruction with the

@@ -88,8 +88,13 @@ struct DevMesh
     const int4 *__restrict__ tile_faces;
 };
 
-// cells per tile of the LDS-staged gradient kernel
-constexpr int kTile = 256;
+// cells per tile of the LDS-staged gradient kernel and the face-centred
+// residual kernel (a multiple of 64)
+#ifndef FVH_KTILE
+#define FVH_KTILE 128
+#endif
+constexpr int kTile = FVH_KTILE;
+static_assert(kTile % 64 == 0 && kTile <= 1024, "cell tile: whole waves, one workgroup");
 
 struct PhysDev
 {

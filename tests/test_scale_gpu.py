@@ -21,7 +21,7 @@ pytestmark = pytest.mark.gpu
 
 
 def test_taylor_green_96_conservation_and_determinism():
-    cfg = configs.taylor_green(n=96, steps=5, tile=8)
+    cfg = configs.taylor_green(n=96, steps=5, tile=4)
     case = S.Case(**cfg)
     V = S.mesh_arrays()["cell_volume"]
     U0 = case.state()
