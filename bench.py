@@ -18,9 +18,11 @@ Beside it, in the same JSON line:
   norm 1e-12; `survey_outlet_limiter` a bounded plain run of the channel
   with the zero-gradient outlet and the limiter;
 - `batched`: 64 periodic cases of log-uniform size (seeded), one device
-  state, one persistent launch per stage on the autotuned schedule,
-  graph-replayed steps: aggregate cell updates per second and the per-case
-  step time inside the batched launches, p50/p99;
+  state, graph-replayed steps on the autotuned schedule (per-group
+  launches, lane per face slot or face-centred, or persistent work-queue
+  launches per class), the other schedules beside it: aggregate cell
+  updates per second and the per-case step time inside the batched
+  persistent launches, p50/p99;
 - `roofline` of the explicit stage (the gradient and residual kernels that
   make one RK stage): 1,276 algorithmic bytes per cell-stage times the cells
   over the two kernels' HIP-event-timed average durations, with the measured
@@ -148,7 +150,7 @@ def main():
 
         k_grad_s, k_stage_s = k_us[0] * 1e-6, k_us[1] * 1e-6
         # the residual kernel on tetrahedra: face-centred unless switched off
-        k2 = ("fv_residual_face_kernel<AUSM,viscous>" if os.environ.get("FVH_K2_FACE", "1") != "0"
+        k2 = ("fv_residual_face_kernel<AUSM,face states,viscous>" if os.environ.get("FVH_K2_FACE", "1") != "0"
               else "fv_residual_group_kernel<AUSM,face states,viscous,4>")
         traffic = measured_traffic(args.n)
         # roofline: the whole stage (gradient + residual kernel launches,
@@ -364,7 +366,8 @@ def bench_batch(n_cases, steps, alone=False, retune=False):
         tuned_here = True
     best, table = tuned
     sched = dict(persistent=best["persistent"], block=best["block"] or 256,
-                 grid=best["grid"], sub_tiles=best.get("sub_tiles") or 1)
+                 grid=best["grid"], sub_tiles=best.get("sub_tiles") or 1,
+                 face=bool(best.get("face", False)))
     pers = [r for r in table if r["persistent"]]
     best_p = min(pers, key=lambda r: r["ms_per_step"]) if pers else None
     t0 = time.time()
@@ -377,6 +380,9 @@ def bench_batch(n_cases, steps, alone=False, retune=False):
     b.set_schedule(False)
     b.run(1, graph=True)
     sec_group = b.run(steps, graph=True)
+    b.set_schedule(False, face=True)
+    b.run(1, graph=True)
+    sec_face = b.run(steps, graph=True)
     # per-case step times inside the shared launches: timestamps of the
     # persistent schedule's tiles (its best autotuned configuration)
     per, sec_pers, p_sched = None, None, None
@@ -398,6 +404,7 @@ def bench_batch(n_cases, steps, alone=False, retune=False):
                             "autotuner cache fv3d_amd/data/batch_tuning.json"),
            "ms_per_step_graph": ms, "ms_per_step_eager": sec_eager / steps * 1e3,
            "ms_per_step_group_schedule_graph": sec_group / steps * 1e3,
+           "ms_per_step_face_schedule_graph": sec_face / steps * 1e3,
            "ms_per_step_persistent_schedule_graph": (sec_pers / steps * 1e3
                                                      if sec_pers else None),
            "persistent_schedule": p_sched,
@@ -411,9 +418,10 @@ def bench_batch(n_cases, steps, alone=False, retune=False):
                                             "max": float(per.max()),
                                             "schedule": "persistent, as persistent_schedule"}
     b.close()
-    print("batched: %.3f ms/step (%d cells), group schedule %.3f ms, persistent %s ms, "
-          "in-batch per-case p50 %s p99 %s ms"
-          % (ms, cells, sec_group / steps * 1e3, out["ms_per_step_persistent_schedule_graph"],
+    print("batched: %.3f ms/step (%d cells), group schedule %.3f ms, face schedule %.3f ms, "
+          "persistent %s ms, in-batch per-case p50 %s p99 %s ms"
+          % (ms, cells, sec_group / steps * 1e3, sec_face / steps * 1e3,
+             out["ms_per_step_persistent_schedule_graph"],
              out.get("per_case_step_ms_in_batch", {}).get("p50"),
              out.get("per_case_step_ms_in_batch", {}).get("p99")),
           file=sys.stderr, flush=True)

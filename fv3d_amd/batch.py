@@ -97,8 +97,12 @@ class Batch:
         if schedule is not None:
             self.set_schedule(**schedule)
 
-    def set_schedule(self, persistent=True, block=256, grid=0, sub_tiles=1):
-        S.check(self.L.fvh_batch_set_schedule(1 if persistent else 0, block, grid, sub_tiles))
+    def set_schedule(self, persistent=True, block=256, grid=0, sub_tiles=1, face=False):
+        """persistent: the work-queue launches per class; face: the
+        face-centred launches per group over case-aligned tiles; neither:
+        the lane-per-slot launches per group."""
+        mode = 2 if face else (1 if persistent else 0)
+        S.check(self.L.fvh_batch_set_schedule(mode, block, grid, sub_tiles))
 
     def case_step_times(self, steps=5):
         """Per-case step time (ms) inside the batched launches."""
@@ -150,8 +154,9 @@ def autotune(cases, tiles=(4, 8), blocks=(256, 512), grids=(2, 4, 8, -1), subs=(
     the case meshes (hexahedra per side of a locality tile), persistent
     workgroup size (256 or 512 threads: runs of 64 or 128 tetrahedra),
     persistent grid (2, 4, 8 workgroups per CU, or -1: one workgroup per tile
-    without the work queue) and runs per tile (1, 4, 16), and the per-group
-    schedule; returns (best, table).
+    without the work queue) and runs per tile (1, 4, 16), and the two
+    per-group schedules (lane per face slot; face-centred over case-aligned
+    tiles); returns (best, table).
     `progress`, when given, is called with each table row as it is
     measured.  The explicit path has no cell colouring, so there is no
     colour-block size to tune."""
@@ -171,13 +176,14 @@ def autotune(cases, tiles=(4, 8), blocks=(256, 512), grids=(2, 4, 8, -1), subs=(
                                       sub_tiles=sub, ms_per_step=sec / steps * 1e3))
                     if progress:
                         progress(table[-1])
-        b.set_schedule(False)
-        b.run(1, graph=True)
-        sec = b.run(steps, graph=True)
-        table.append(dict(tile=tile, persistent=False, block=0, grid=0, sub_tiles=0,
-                          ms_per_step=sec / steps * 1e3))
-        if progress:
-            progress(table[-1])
+        for face in (False, True):
+            b.set_schedule(False, face=face)
+            b.run(1, graph=True)
+            sec = b.run(steps, graph=True)
+            table.append(dict(tile=tile, persistent=False, face=face, block=0, grid=0,
+                              sub_tiles=0, ms_per_step=sec / steps * 1e3))
+            if progress:
+                progress(table[-1])
         b.close()
     best = min(table, key=lambda r: r["ms_per_step"])
     return best, table

@@ -32,6 +32,7 @@ struct Group
 {
     int flux, viscous, c_begin, c_end;
     double mu;
+    int t_begin = 0, t_end = 0; // the group's face-table tiles (face schedule)
 };
 
 constexpr int kMaxStages = 5, kClasses = 4, kCounters = kMaxStages * kClasses;
@@ -69,6 +70,16 @@ struct BatchState
     unsigned long long *stamps = nullptr; // [stages][n_tiles][2] when timing
     std::vector<std::pair<int, int>> visc_runs; // cell ranges of viscous cases
     std::vector<int> h_tile_lo, h_tile_hi; // a case's tiles
+    // face schedule (tetrahedral cases): per group one launch of the
+    // face-centred residual kernel over case-aligned tiles of at most
+    // dev::kTile cells; the last stage writes kTile / 64 partials rows per
+    // tile, and a case's rows are f_row_lo[k] .. f_row_hi[k] - 1
+    bool face = false;
+    int f_tiles = 0;
+    int *f_tile_c0 = nullptr, *f_face_off = nullptr, *f_row_lo = nullptr,
+        *f_row_hi = nullptr;
+    int4 *f_faces = nullptr;
+    double *f_partials = nullptr;
 };
 
 
@@ -301,8 +312,70 @@ void enqueue_step_persistent(const RuntimeView &v, const StageBuffers &sb,
 
 // Enqueues one RK step of every case (the stages of every group, then the
 // per-case residuals and next time steps).
+// One RK step of every case on the face schedule: per stage and group, the
+// viscous gradient subset (viscous groups) and the face-centred residual
+// kernel over the group's tiles; the last stage's partials per tile, folded
+// per case.
+void enqueue_step_face(const RuntimeView &v, const StageBuffers &sb)
+{
+    hipStream_t s = v.stream;
+    dev::DevMesh m = v.m;
+    m.tile_c0 = g_b.f_tile_c0;
+    m.tile_face_off = g_b.f_face_off;
+    m.tile_faces = g_b.f_faces;
+    for (int st = 0; st < sb.stages; ++st)
+    {
+        const bool last = st == sb.stages - 1;
+        double *Uin = sb.U[*sb.cur], *Uout = sb.U[1 - *sb.cur];
+        for (const Group &g : g_b.groups)
+        {
+            dev::PhysDev P = v.P;
+            P.mu = g.mu;
+            P.lam = g.mu * cp / Pr;
+            if (g.viscous)
+            {
+                dev::GradOut o{};
+                o.vg = g_b.vg;
+                o.c_begin = g.c_begin;
+                o.c_end = g.c_end;
+                o.prims = sb.prims[*sb.cur];
+                dev::launch_gradient(s, m, P, Uin, nullptr, o, 0, nullptr);
+            }
+            dev::StageParams sp{};
+            sp.U_in = Uin;
+            sp.U_out = Uout;
+            sp.k = sb.k;
+            sp.R_out = last ? sb.R : nullptr;
+            sp.vg = g_b.vg;
+            sp.rk_a = sb.rk_a[st];
+            sp.rk_g = sb.rk_g[st];
+            sp.stage0 = st == 0;
+            sp.do_update = 1;
+            sp.do_reduce = last;
+            sp.partials = g_b.f_partials;
+            sp.partials_cap = g_b.f_tiles * (dev::kTile / 64);
+            sp.c_begin = g.c_begin;
+            sp.c_end = g.c_end;
+            sp.case_dt = g_b.d_dt;
+            sp.cell_case = g_b.cell_case;
+            sp.prims_in = sb.prims[*sb.cur];
+            sp.prims_out = sb.prims[1 - *sb.cur];
+            dev::launch_residual_tiles(s, m, P, sp, g.flux, false, g.viscous != 0, g.t_begin,
+                                       g.t_end - g.t_begin);
+        }
+        *sb.cur = 1 - *sb.cur;
+    }
+    batch::case_fold(s, 1, v.P, g_b.K, g_b.f_row_lo, g_b.f_row_hi, g_b.f_partials, g_b.d_mu,
+                     g_b.d_vol, g_b.d_dt, g_b.d_t, g_b.d_res);
+}
+
 void enqueue_step(const RuntimeView &v, const StageBuffers &sb)
 {
+    if (g_b.face)
+    {
+        enqueue_step_face(v, sb);
+        return;
+    }
     if (g_b.persistent)
     {
         enqueue_step_persistent(v, sb, nullptr);
@@ -502,6 +575,53 @@ int fvh_batch_setup(int n_cases, const int *flux, const int *viscous, const doub
             batch::reset_counters(v.stream, g_b.counters, kCounters);
             build_tiles(v);
         }
+        // face schedule: case-aligned tiles and their face tables
+        if (v.m.maxf == 4)
+        {
+            const int rows = dev::kTile / 64;
+            std::vector<int> c0, case_t0(n_cases + 1), rlo(n_cases), rhi(n_cases);
+            for (int k = 0; k < n_cases; ++k)
+            {
+                case_t0[k] = (int)c0.size();
+                for (int c = g_b.offsets[k]; c < g_b.offsets[k + 1]; c += dev::kTile)
+                    c0.push_back(c);
+                rlo[k] = case_t0[k] * rows;
+                rhi[k] = (int)c0.size() * rows;
+            }
+            case_t0[n_cases] = (int)c0.size();
+            g_b.f_tiles = (int)c0.size();
+            c0.push_back(N);
+            int k = 0;
+            for (Group &g : g_b.groups)
+            {
+                while (g_b.offsets[k] != g.c_begin)
+                    ++k;
+                g.t_begin = case_t0[k];
+                int k1 = k;
+                while (g_b.offsets[k1] != g.c_end)
+                    ++k1;
+                g.t_end = case_t0[k1];
+            }
+            std::vector<int> off;
+            std::vector<int4> ent;
+            build_tile_faces(c0, off, ent);
+            g_b.f_tile_c0 = balloc<int>(c0.size());
+            g_b.f_face_off = balloc<int>(off.size());
+            g_b.f_faces = balloc<int4>(ent.size());
+            g_b.f_row_lo = balloc<int>(n_cases);
+            g_b.f_row_hi = balloc<int>(n_cases);
+            g_b.f_partials = balloc<double>((size_t)g_b.f_tiles * rows * dev::kPartialStride);
+            FVH_HIP_CHECK(hipMemcpy(g_b.f_tile_c0, c0.data(), c0.size() * sizeof(int),
+                                    hipMemcpyHostToDevice));
+            FVH_HIP_CHECK(hipMemcpy(g_b.f_face_off, off.data(), off.size() * sizeof(int),
+                                    hipMemcpyHostToDevice));
+            FVH_HIP_CHECK(hipMemcpy(g_b.f_faces, ent.data(), ent.size() * sizeof(int4),
+                                    hipMemcpyHostToDevice));
+            FVH_HIP_CHECK(hipMemcpy(g_b.f_row_lo, rlo.data(), n_cases * sizeof(int),
+                                    hipMemcpyHostToDevice));
+            FVH_HIP_CHECK(hipMemcpy(g_b.f_row_hi, rhi.data(), n_cases * sizeof(int),
+                                    hipMemcpyHostToDevice));
+        }
         std::vector<double> zeros((size_t)5 * n_cases, 0.0);
         FVH_HIP_CHECK(hipMemcpy(g_b.d_t, zeros.data(), n_cases * sizeof(double),
                                 hipMemcpyHostToDevice));
@@ -604,6 +724,19 @@ int fvh_batch_case_scalars(double *dt, double *t, double *res)
 int fvh_batch_set_schedule(int persistent, int block, int grid, int sub_tiles)
 {
     B_TRY({
+        if (persistent == 2)
+        {
+            if (!g_b.ready || !g_b.f_faces)
+                throw std::runtime_error("fvh_batch_set_schedule: the face schedule needs a "
+                                         "batch of tetrahedral cases");
+            g_b.face = true;
+            g_b.persistent = false;
+            release_graphs();
+            return FVH_OK;
+        }
+        if (persistent < 0 || persistent > 2)
+            throw std::runtime_error("fvh_batch_set_schedule: schedule must be 0, 1 or 2");
+        g_b.face = false;
         if (persistent && block != 128 && block != 256 && block != 512)
             throw std::runtime_error("fvh_batch_set_schedule: block must be 128, 256 or 512");
         if (persistent && (sub_tiles < 1 || sub_tiles > 1024))

@@ -86,6 +86,10 @@ struct DevMesh
     // 0xffff where that side's cell is outside the tile
     const int *__restrict__ tile_face_off;
     const int4 *__restrict__ tile_faces;
+    // first cell of each face-table tile, [tiles + 1] (null: tile t is cells
+    // [t*kTile, (t+1)*kTile)); set for tiles that follow other boundaries
+    // (batched cases: no tile spans two cases), each at most kTile cells
+    const int *__restrict__ tile_c0;
 };
 
 // cells per tile of the LDS-staged gradient kernel and the face-centred
@@ -151,6 +155,7 @@ struct StageParams
     const double *prims_in; // primitives [N][5] of U_in (null: computed)
     double *prims_out;      // primitives of U_out, written by the stage update
     const float *U32;       // the FP32 kernel's input state in FP32 (null: U_in)
+    int tile_begin;         // face-centred kernel: first tile of the launch
 };
 
 constexpr int kBlock = 256;
@@ -202,6 +207,15 @@ void launch_gradient(hipStream_t s, const DevMesh &m, const PhysDev &P,
 void launch_residual(hipStream_t s, const DevMesh &m, const PhysDev &P,
                      const StageParams &sp, int flux, bool face_states,
                      bool viscous, bool fp32 = false);
+// The face-centred residual kernel over face-table tiles tile_begin ..
+// tile_begin + n_tiles - 1 of m (tetrahedral meshes; m.tile_c0 gives the
+// tiles' cells).  Without face_states the face states are the cell values
+// (first order); sp.case_dt / sp.cell_case give per-case time steps; with
+// do_reduce, tile t writes partials rows t * (kTile / 64) .. + kTile / 64 - 1,
+// one per 64 of its cells (identity rows past its last cell).
+void launch_residual_tiles(hipStream_t s, const DevMesh &m, const PhysDev &P,
+                           const StageParams &sp, int flux, bool face_states, bool viscous,
+                           int tile_begin, int n_tiles);
 // The RK stage update of StageParams (U_out, k and, with do_reduce, the
 // partials of cell_blocks(N) workgroups) from the residual R [N][5].
 void launch_stage_update(hipStream_t s, const DevMesh &m, const PhysDev &P,

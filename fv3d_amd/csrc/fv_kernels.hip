@@ -1461,7 +1461,7 @@ void launch_gradient(hipStream_t s, const DevMesh &m, const PhysDev &P,
 #ifndef FVH_K2F_REV
 #define FVH_K2F_REV 1
 #endif
-template <int FLUX, bool VISC>
+template <int FLUX, bool FS, bool VISC>
 __global__ __launch_bounds__(kTile, FVH_K2F_MINW) void fv_residual_face_kernel(DevMesh m,
                                                                               PhysDev P,
                                                                               StageParams sp)
@@ -1470,9 +1470,10 @@ __global__ __launch_bounds__(kTile, FVH_K2F_MINW) void fv_residual_face_kernel(D
         return;
     extern __shared__ double ct[]; // [kTile * 4][5]
     const int N = m.N;
-    const int t = FVH_K2F_REV ? xcd_tile_rev(gridDim.x) : xcd_tile(gridDim.x);
-    const int base = t * kTile;
-    const int ncell = min(kTile, N - base);
+    const int t =
+        sp.tile_begin + (FVH_K2F_REV ? xcd_tile_rev(gridDim.x) : xcd_tile(gridDim.x));
+    const int base = m.tile_c0 ? m.tile_c0[t] : t * kTile;
+    const int ncell = m.tile_c0 ? m.tile_c0[t + 1] - base : min(kTile, N - base);
     const int e0 = m.tile_face_off[t], e1 = m.tile_face_off[t + 1];
     // element p = lc * 5 + j of the tile's rows; with FVH_K2F_PRE the
     // update's operands are loaded before the face work so that their
@@ -1501,6 +1502,7 @@ __global__ __launch_bounds__(kTile, FVH_K2F_MINW) void fv_residual_face_kernel(D
         Frame fr;
         load_frame(m, f, fr);
         double L[10], R[10];
+        if (FS)
         {
             const double2 *pl = reinterpret_cast<const double2 *>(sp.fs + (size_t)r.y * 8);
             const double2 *pr = reinterpret_cast<const double2 *>(sp.fs + (size_t)r.z * 8);
@@ -1514,6 +1516,21 @@ __global__ __launch_bounds__(kTile, FVH_K2F_MINW) void fv_residual_face_kernel(D
                 R[2 * q + 1] = b.y;
             }
             L[8] = L[9] = R[8] = R[9] = 0.0;
+        }
+        else
+        {
+            // first order: the two cells' values (a ghost row on a boundary
+            // face), as the lane-per-slot kernel takes them
+            load_phi(P, sp.U_in, sp.prims_in, r.y >> 2, L);
+            if (ghost)
+            {
+                const double *gp = sp.gphi + (size_t)(r.z - N * 4) * 10;
+#pragma unroll
+                for (int j = 0; j < 10; ++j)
+                    R[j] = gp[j];
+            }
+            else
+                load_phi(P, sp.U_in, sp.prims_in, r.z >> 2, R);
         }
         ViscGrad gL, gR;
         if (VISC)
@@ -1547,7 +1564,7 @@ __global__ __launch_bounds__(kTile, FVH_K2F_MINW) void fv_residual_face_kernel(D
     __syncthreads();
 
     // per (cell, variable)
-    const double dt = sp.do_update ? sp.sc->dt : 0.0;
+    const double dt_all = (sp.do_update && !sp.case_dt) ? sp.sc->dt : 0.0;
     double r_el[kE], u_el[kE];
 #pragma unroll
     for (int i = 0; i < kE; ++i)
@@ -1572,6 +1589,7 @@ __global__ __launch_bounds__(kTile, FVH_K2F_MINW) void fv_residual_face_kernel(D
             sp.R_out[g] = Rj;
         if (sp.do_update)
         {
+            const double dt = sp.case_dt ? sp.case_dt[sp.cell_case[c]] : dt_all;
             double kj = Rj;
             if (!sp.stage0)
                 kj = Rj + (FVH_K2F_PRE ? k_in[i] : sp.k[g]) * sp.rk_a;
@@ -1650,10 +1668,17 @@ __global__ __launch_bounds__(kTile, FVH_K2F_MINW) void fv_residual_face_kernel(D
     }
     if (!sp.do_reduce)
         return;
-    // one row per wave of 64 cells
-    const int wrow = (base + (lc & ~63)) / 64;
-    if (base + (lc & ~63) >= N)
-        return;
+    // one row per wave of 64 cells: by cell over the whole mesh, or by tile
+    // (every wave of the tile writes its row)
+    int wrow;
+    if (m.tile_c0)
+        wrow = t * (kTile / 64) + (lc >> 6);
+    else
+    {
+        wrow = (base + (lc & ~63)) / 64;
+        if (base + (lc & ~63) >= N)
+            return;
+    }
     double w[7];
 #pragma unroll
     for (int q = 0; q < 5; ++q)
@@ -1675,14 +1700,15 @@ static bool use_face_kernel(const DevMesh &m, const StageParams &sp, bool fs, bo
            !sp.case_dt && sp.fs && (!sp.do_update || sp.sc);
 }
 
-template <int FLUX, bool VISC>
+template <int FLUX, bool FS, bool VISC>
 static void launch_face_t(hipStream_t s, const DevMesh &m, const PhysDev &P,
-                          const StageParams &sp)
+                          const StageParams &sp_in, int tile_begin, int n_tiles)
 {
-    const int tiles = (m.N + kTile - 1) / kTile;
+    StageParams sp = sp_in;
+    sp.tile_begin = tile_begin;
     const size_t lds = (size_t)kTile * 4 * 5 * sizeof(double);
-    hipLaunchKernelGGL((fv_residual_face_kernel<FLUX, VISC>), dim3(tiles), dim3(kTile), lds, s,
-                       m, P, sp);
+    hipLaunchKernelGGL((fv_residual_face_kernel<FLUX, FS, VISC>), dim3(n_tiles), dim3(kTile),
+                       lds, s, m, P, sp);
 }
 
 template <int FLUX, bool FS, bool VISC>
@@ -1691,7 +1717,7 @@ static void launch_res_t(hipStream_t s, const DevMesh &m, const PhysDev &P,
 {
     if (FS && use_face_kernel(m, sp, FS, fp32))
     {
-        launch_face_t<FLUX, VISC>(s, m, P, sp);
+        launch_face_t<FLUX, true, VISC>(s, m, P, sp, 0, (m.N + kTile - 1) / kTile);
         return;
     }
     const int lanes = m.maxf <= 4 ? 4 : 8;
@@ -1751,6 +1777,46 @@ void launch_residual(hipStream_t s, const DevMesh &m, const PhysDev &P,
         else
             viscous ? launch_res_t<FluxAUSM, false, true>(s, m, P, sp, fp32)
                     : launch_res_t<FluxAUSM, false, false>(s, m, P, sp, fp32);
+    }
+}
+
+void launch_residual_tiles(hipStream_t s, const DevMesh &m, const PhysDev &P,
+                           const StageParams &sp, int flux, bool face_states, bool viscous,
+                           int tile_begin, int n_tiles)
+{
+    check_faces(m);
+    if (m.maxf != 4 || !m.tile_faces || !m.tile_face_off)
+        throw std::runtime_error("launch_residual_tiles: needs the face tables of a "
+                                 "tetrahedral mesh");
+    if (tile_begin < 0 || n_tiles < 1)
+        throw std::runtime_error("launch_residual_tiles: bad tile range");
+    if (face_states && !sp.fs)
+        throw std::runtime_error("launch_residual_tiles: no face states");
+    if (viscous && !sp.vg)
+        throw std::runtime_error("launch_residual_tiles: no viscous gradients");
+    if (sp.do_update && !sp.case_dt && !sp.sc)
+        throw std::runtime_error("launch_residual_tiles: stage update without a time step");
+    if (sp.do_reduce && (!m.tile_c0 || (size_t)(tile_begin + n_tiles) * (kTile / 64) >
+                                           (size_t)sp.partials_cap))
+        throw std::runtime_error("launch_residual_tiles: reduction needs tile rows "
+                                 "(tile_c0) and a partials buffer of kTile / 64 rows per tile");
+    if (flux == FluxRusanov)
+    {
+        if (face_states)
+            viscous ? launch_face_t<FluxRusanov, true, true>(s, m, P, sp, tile_begin, n_tiles)
+                    : launch_face_t<FluxRusanov, true, false>(s, m, P, sp, tile_begin, n_tiles);
+        else
+            viscous ? launch_face_t<FluxRusanov, false, true>(s, m, P, sp, tile_begin, n_tiles)
+                    : launch_face_t<FluxRusanov, false, false>(s, m, P, sp, tile_begin, n_tiles);
+    }
+    else
+    {
+        if (face_states)
+            viscous ? launch_face_t<FluxAUSM, true, true>(s, m, P, sp, tile_begin, n_tiles)
+                    : launch_face_t<FluxAUSM, true, false>(s, m, P, sp, tile_begin, n_tiles);
+        else
+            viscous ? launch_face_t<FluxAUSM, false, true>(s, m, P, sp, tile_begin, n_tiles)
+                    : launch_face_t<FluxAUSM, false, false>(s, m, P, sp, tile_begin, n_tiles);
     }
 }
 

@@ -7,6 +7,7 @@
 
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "fv_device.h"
 
@@ -59,6 +60,13 @@ struct StageBuffers
     const double *rk_a, *rk_g;
 };
 StageBuffers stage_buffers();
+
+// Face tables of the face-centred residual kernel for cell tiles
+// [c0[t], c0[t+1]) of at most dev::kTile cells (tetrahedral meshes):
+// per tile every face of its cells once (off[t] .. off[t+1] - 1 of ent), in
+// the DevMesh tile_faces layout.  Reads the host mesh.
+void build_tile_faces(const std::vector<int> &c0, std::vector<int> &off,
+                      std::vector<int4> &ent);
 
 // Enqueues R = dU/dt(U) (finite_volume_time_derivative semantics, the
 // reference's ghost rows recomputed from U) on the runtime stream.  fp32:

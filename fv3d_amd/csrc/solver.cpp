@@ -721,14 +721,12 @@ void setup_tile_halo()
 // tile-local slots that receive its contributions.  A cell that is its own
 // periodic neighbour keeps one entry per slot, as the lane-per-slot kernel
 // evaluates it: its own row left, the face's other slot right.
-void setup_tile_faces()
+void tile_faces_table(const std::vector<int> &c0, std::vector<int> &off,
+                      std::vector<int4> &ent)
 {
-    const char *e = std::getenv("FVH_K2_FACE");
-    if (e && std::atoi(e) == 0)
-        return;
-    const int N = g_dev.m.N, maxf = g_dev.m.maxf, T = dev::kTile;
+    const int N = g_dev.m.N, maxf = g_dev.m.maxf;
     if (maxf != 4)
-        return;
+        fatal("tile faces: tetrahedral meshes only");
     const Cells_t *cells = solver_mesh->cells;
     const Faces_t *faces = solver_mesh->faces;
     auto slot_of = [&](int x, int f, int not_slot) {
@@ -739,21 +737,23 @@ void setup_tile_faces()
               std::to_string(x));
         return -1;
     };
-    const int nt = (N + T - 1) / T;
-    std::vector<int> off(nt + 1, 0);
-    std::vector<int4> ent;
+    const int nt = (int)c0.size() - 1;
+    off.assign(nt + 1, 0);
+    ent.clear();
     ent.reserve((size_t)N * 5 / 2);
     for (int t = 0; t < nt; ++t)
     {
-        const int b0 = t * T, b1 = std::min(N, b0 + T);
+        const int b0 = c0[t], b1 = c0[t + 1];
+        if (b0 < 0 || b1 > N || b1 <= b0 || b1 - b0 > dev::kTile)
+            fatal("tile faces: a tile must hold 1 .. kTile cells of the mesh");
         auto in_tile = [&](int x) { return x >= b0 && x < b1; };
         for (int c = b0; c < b1; ++c)
             for (int s = 0; s < cells->n_faces[c]; ++s)
             {
                 const int f = cells->faces[(size_t)c * maxf + s];
-                const int c0 = faces->cells[2 * f], c1 = faces->cells[2 * f + 1];
+                const int fc0 = faces->cells[2 * f], fc1 = faces->cells[2 * f + 1];
                 int rowL, rowR, dA, dB = 0xffff;
-                if (c0 == c1)
+                if (fc0 == fc1)
                 {
                     rowL = c * 4 + s;
                     rowR = c * 4 + slot_of(c, f, s);
@@ -761,26 +761,45 @@ void setup_tile_faces()
                 }
                 else
                 {
-                    const bool interior = c1 < N;
-                    if (interior && c == c1 && in_tile(c0))
+                    const bool interior = fc1 < N;
+                    if (interior && c == fc1 && in_tile(fc0))
                         continue; // evaluated from its side-0 cell
-                    const int s0 = c == c0 ? s : slot_of(c0, f, -1);
-                    rowL = c0 * 4 + s0;
-                    dA = in_tile(c0) ? (c0 - b0) * 4 + s0 : 0xffff;
+                    const int s0 = c == fc0 ? s : slot_of(fc0, f, -1);
+                    rowL = fc0 * 4 + s0;
+                    dA = in_tile(fc0) ? (fc0 - b0) * 4 + s0 : 0xffff;
                     if (interior)
                     {
-                        const int s1 = c == c1 ? s : slot_of(c1, f, -1);
-                        rowR = c1 * 4 + s1;
-                        if (in_tile(c1))
-                            dB = (c1 - b0) * 4 + s1;
+                        const int s1 = c == fc1 ? s : slot_of(fc1, f, -1);
+                        rowR = fc1 * 4 + s1;
+                        if (in_tile(fc1))
+                            dB = (fc1 - b0) * 4 + s1;
                     }
                     else
-                        rowR = N * 4 + (c1 - N);
+                        rowR = N * 4 + (fc1 - N);
                 }
                 ent.push_back(make_int4(f, rowL, rowR, dA | (dB << 16)));
             }
         off[t + 1] = (int)ent.size();
     }
+}
+
+// The face tables over tiles of dev::kTile consecutive cells (the explicit
+// path's residual kernel); not built with FVH_K2_FACE=0.
+void setup_tile_faces()
+{
+    const char *e = std::getenv("FVH_K2_FACE");
+    if (e && std::atoi(e) == 0)
+        return;
+    const int N = g_dev.m.N, T = dev::kTile;
+    if (g_dev.m.maxf != 4)
+        return;
+    std::vector<int> c0;
+    for (int c = 0; c < N; c += T)
+        c0.push_back(c);
+    c0.push_back(N);
+    std::vector<int> off;
+    std::vector<int4> ent;
+    tile_faces_table(c0, off, ent);
     g_dev.m.tile_face_off = dupload(off.data(), off.size());
     g_dev.m.tile_faces = dupload(ent.data(), ent.size());
 }
@@ -917,6 +936,12 @@ void calc_mesh_metrics(Mesh_t *mesh)
 }
 
 } // namespace
+
+void build_tile_faces(const std::vector<int> &c0, std::vector<int> &off,
+                      std::vector<int4> &ent)
+{
+    tile_faces_table(c0, off, ent);
+}
 
 void set_error(const std::string &msg) { g_error = msg; }
 

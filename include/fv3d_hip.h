@@ -252,7 +252,10 @@ int fvh_batch_run(int steps, int use_graph, double *seconds);
 int fvh_batch_info(int *n_cases, int *offsets, int max_offsets);
 /* Step schedule: persistent = 0 (the default) launches the stage kernel per
  * (flux, viscous) group of cases and reduces the per-case norms in a pass
- * of its own; persistent != 0 runs each stage as one persistent launch per
+ * of its own; persistent = 2 (tetrahedral cases) launches the face-centred
+ * stage kernel per group over tiles of at most 128 cells of one case (each
+ * face's flux evaluated once per tile) and reduces in its last stage
+ * (block, grid and sub_tiles unused); persistent = 1 runs each stage as one persistent launch per
  * (flux, viscous) class of cases (workgroups of `block` = 128, 256 or 512
  * threads take tiles of sub_tiles * block / lanes cells of one case from a
  * work queue, lanes = 4 on tetrahedra; `grid` workgroups, 0 = min(tiles,

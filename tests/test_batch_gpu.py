@@ -43,6 +43,7 @@ def _oracle_runs():
 
 
 SCHEDULES = {"group": dict(persistent=False),
+             "face": dict(persistent=False, face=True),
              "persistent-128": dict(persistent=True, block=128, grid=0),
              "persistent-256-grid3": dict(persistent=True, block=256, grid=3),
              "persistent-512": dict(persistent=True, block=512, grid=0),
@@ -56,10 +57,12 @@ SCHEDULES = {"group": dict(persistent=False),
 @pytest.mark.parametrize("graph", [False, True])
 @pytest.mark.parametrize("schedule", list(SCHEDULES))
 def test_batch_matches_single_case_oracle(graph, schedule):
-    """Every schedule (per-group launches; persistent launches per class
-    with tiles of one, 4 or 16 runs of 32, 64 or 128 cells, some ending
-    inside a run, and with 3 or 5 workgroups looping over all tiles; one
-    workgroup per tile) gives each case its own oracle run."""
+    """Every schedule (per-group launches, lane per face slot or
+    face-centred over tiles of at most 128 cells of one case, the last tile
+    of every case partial; persistent launches per class with tiles of one,
+    4 or 16 runs of 32, 64 or 128 cells, some ending inside a run, and with
+    3 or 5 workgroups looping over all tiles; one workgroup per tile) gives
+    each case its own oracle run."""
     ref = _oracle_runs()
     b = B.Batch(CASES, tile=TILE, mu_ns=MU, schedule=SCHEDULES[schedule])
     b.run(STEPS, graph=graph)
