@@ -10,9 +10,12 @@
 //                       cdx[N][3], cx[N][3] (centroids), cf[N][maxf]
 //                       (face ids, ascending, -1
 //                       padded); vol[N], svol[N] = 1/vol, ncf[N]
-//   face records      : fs[F][16] (reconstructed state of variables 0..7
-//                       on side 0 = cells[0], then side 1 = cells[1] or the
-//                       ghost), fgA[F][10] (n, t1, t2, area: flux kernel),
+//   face-state rows   : fs[N*maxf + B][8] (row c*maxf + s: cell c's
+//                       reconstructed state of variables 0..7 on its face
+//                       slot s; row N*maxf + b: the ghost state of boundary
+//                       b); nsl[N*maxf] gives the slot of the same face in
+//                       the neighbour's list
+//   face records      : fgA[F][10] (n, t1, t2, area: flux kernel),
 //                       fgB[F][6] (n, area, lambda, pad: gradient
 //                       kernel), fsh[F] (index into shtab[][3],
 //                       the exact periodic shifts; 0 = none), fcell[F][2]
@@ -123,8 +126,8 @@ struct DevScalars
 struct GradOut
 {
     double *vg;   // [N*12] viscous gradient subset
-    double *fs;   // [F*16] face states
-    float *fs32;  // [F*16] face states in FP32 (the FP32 kernels write these)
+    double *fs;   // [N*maxf + B][8] face-state rows
+    float *fs32;  // the same rows in FP32 (the FP32 kernels write these)
     double *grad; // [N*30] all gradients (tests)
     double *mnmx; // [N*20] neighbourhood min/max (tests)
     int c_begin, c_end; // cells [c_begin, c_end); c_end = 0: all cells
@@ -139,8 +142,8 @@ struct StageParams
     double *k;
     double *R_out;         // may be null
     const double *vg;      // [N*12] when viscous
-    const double *fs;      // [F*16] face states (reconstructed path)
-    const float *fs32;     // [F*16] FP32 face states (the FP32 kernels read these)
+    const double *fs;      // [N*maxf + B][8] face-state rows (reconstructed path)
+    const float *fs32;     // the same rows in FP32 (the FP32 kernels read these)
     const double *gphi;    // [B*10] ghost rows (first-order path)
     double rk_a, rk_g;
     int stage0;            // first stage: k = R

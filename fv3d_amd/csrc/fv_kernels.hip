@@ -2,24 +2,33 @@
 //
 // Per RK stage (or residual evaluation):
 //   fv_ghost_kernel     ghost rows of the boundary faces (B threads)
-//   fv_gradient_kernel  one thread per cell: Green-Gauss gradients and
-//                       neighbourhood min/max, then the cell's limited state
-//                       on each of its faces, written by face id and side
-//   fv_residual_group_kernel
-//                       LANES lanes per cell, one face per lane: the face
-//                       flux from the two stored face states, summed in
-//                       ascending face order, then the fused RK stage and,
-//                       in the last stage, the residual-norm and time-step
-//                       partials
-//   fv_finalize_kernel  one block: the time loop's bookkeeping
+//   fv_gradient_tile_kernel (FP64, tables built) / fv_gradient_kernel
+//                       one thread per cell: Green-Gauss gradients and
+//                       neighbourhood min/max (the tile kernel from its
+//                       tile's states staged in LDS), then the cell's
+//                       limited state on each of its faces, written to the
+//                       cell's face-state rows (one per face slot)
+//   fv_residual_face_kernel (tetrahedra, FP64, tables built)
+//                       each face of a cell tile once: the face flux from
+//                       the two sides' states, contributions per cell slot
+//                       in LDS, summed in slot order, then the RK stage per
+//                       (cell, variable) and, in the last stage, the
+//                       residual-norm and time-step partials
+//   fv_residual_group_kernel (the other cases)
+//                       LANES lanes per cell, one face per lane, summed in
+//                       slot order through lane shuffles, then the same
+//                       stage and partials
+//   fv_fold_partials_kernel, fv_finalize_kernel
+//                       fixed-order folds and the time loop's bookkeeping
 //
-// Every cell gathers over its own faces in ascending face id (the order of
-// the reference's face-ordered scatter, src/Solver/src/finite_volume/
+// Every cell sums its faces in ascending face id (its slot order: the order
+// of the reference's face-ordered scatter, src/Solver/src/finite_volume/
 // finite_volume.c:100-112), so its sums equal the reference's exactly and
-// no atomics are used: results are bitwise reproducible run to run.  Each
-// face flux is evaluated by both of its cells (same inputs, same
-// operations, same bits), which keeps the flux[F][5] array of the reference
-// out of HBM.
+// no atomics are used: results are bitwise reproducible run to run.  The
+// flux[F][5] array of the reference stays out of HBM: the lane-per-slot
+// kernel evaluates a face flux in both of its cells (same inputs, same
+// operations, same bits), the face-centred kernel once per tile that holds
+// one of its cells.
 //
 // Arithmetic follows the reference operation by operation and is compiled
 // with -ffp-contract=off, so IEEE-rounded +,-,*,/ and sqrt give the same bits
@@ -106,8 +115,8 @@ __global__ __launch_bounds__(kBlock) void fv_ghost_kernel(
 // K1: Green-Gauss gradients of all 10 variables and neighbourhood min/max of
 // one cell (reconstruction.c:20-93 in gather form, limiter.c:57-82), then
 // the cell's reconstructed state on each of its faces
-// (reconstruction.c:168-255 / 142-163).  Writes the face states by face id
-// and side, and the gradient subset the viscous flux needs.
+// (reconstruction.c:168-255 / 142-163).  Writes the cell's face-state rows
+// and the gradient subset the viscous flux needs.
 // ---------------------------------------------------------------------------
 // ALL: also the gradient of p and the bounds of p and T, which only the
 // test outputs grad/mnmx read (fewer live registers without them).
