@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -273,7 +274,6 @@ void enqueue_step_persistent(const RuntimeView &v, const StageBuffers &sb,
             o.vg = g_b.vg;
             o.c_begin = r.first;
             o.c_end = r.second;
-            o.prims = sb.prims[*sb.cur];
             dev::launch_gradient(s, m, v.P, Uin, nullptr, o, 0, nullptr);
         }
         dev::StageParams sp{};
@@ -293,8 +293,6 @@ void enqueue_step_persistent(const RuntimeView &v, const StageBuffers &sb,
         sp.c_end = m.N;
         sp.case_dt = g_b.d_dt;
         sp.cell_case = g_b.cell_case;
-        sp.prims_in = sb.prims[*sb.cur];
-        sp.prims_out = sb.prims[1 - *sb.cur];
         unsigned long long *sst = stamps ? stamps + (size_t)st * 2 * g_b.n_tiles : nullptr;
         for (int cls = 0; cls < kClasses; ++cls)
             if (g_b.cls_tiles[cls] > 0)
@@ -338,7 +336,6 @@ void enqueue_step_face(const RuntimeView &v, const StageBuffers &sb)
                 o.vg = g_b.vg;
                 o.c_begin = g.c_begin;
                 o.c_end = g.c_end;
-                o.prims = sb.prims[*sb.cur];
                 dev::launch_gradient(s, m, P, Uin, nullptr, o, 0, nullptr);
             }
             dev::StageParams sp{};
@@ -358,8 +355,6 @@ void enqueue_step_face(const RuntimeView &v, const StageBuffers &sb)
             sp.c_end = g.c_end;
             sp.case_dt = g_b.d_dt;
             sp.cell_case = g_b.cell_case;
-            sp.prims_in = sb.prims[*sb.cur];
-            sp.prims_out = sb.prims[1 - *sb.cur];
             dev::launch_residual_tiles(s, m, P, sp, g.flux, false, g.viscous != 0, g.t_begin,
                                        g.t_end - g.t_begin);
         }
@@ -398,8 +393,7 @@ void enqueue_step(const RuntimeView &v, const StageBuffers &sb)
                 o.vg = g_b.vg;
                 o.c_begin = g.c_begin;
                 o.c_end = g.c_end;
-                o.prims = sb.prims[*sb.cur];
-                dev::launch_gradient(s, m, P, Uin, nullptr, o, 0, nullptr);
+                    dev::launch_gradient(s, m, P, Uin, nullptr, o, 0, nullptr);
             }
             dev::StageParams sp{};
             sp.U_in = Uin;
@@ -415,9 +409,7 @@ void enqueue_step(const RuntimeView &v, const StageBuffers &sb)
             sp.c_end = g.c_end;
             sp.case_dt = g_b.d_dt;
             sp.cell_case = g_b.cell_case;
-            sp.prims_in = sb.prims[*sb.cur];
-            sp.prims_out = sb.prims[1 - *sb.cur];
-            dev::launch_residual(s, m, P, sp, g.flux, false, g.viscous != 0);
+                    dev::launch_residual(s, m, P, sp, g.flux, false, g.viscous != 0);
         }
         *sb.cur = 1 - *sb.cur;
     }
@@ -647,8 +639,6 @@ int fvh_batch_run(int steps, int use_graph, double *seconds)
         if (sb.stages % 2 == 0 && use_graph)
             throw std::runtime_error("fvh_batch_run: graphs assume an odd stage count");
         hipStream_t s = v.stream;
-        // primitive rows of the current state (the steps keep them current)
-        dev::launch_prims(s, v.m, v.P, sb.U[*sb.cur], sb.prims[*sb.cur]);
         if (use_graph)
             for (int par = 0; par < 2; ++par)
             {
@@ -766,7 +756,6 @@ int fvh_batch_case_step_times(int steps, double *ms)
         RuntimeView v = runtime_view();
         StageBuffers sb = stage_buffers();
         hipStream_t s = v.stream;
-        dev::launch_prims(s, v.m, v.P, sb.U[*sb.cur], sb.prims[*sb.cur]);
         const size_t n = (size_t)sb.stages * 2 * g_b.n_tiles;
         if (!g_b.stamps)
             g_b.stamps = balloc<unsigned long long>((size_t)kMaxStages * 2 * g_b.n_tiles);

@@ -131,7 +131,6 @@ struct GradOut
     double *grad; // [N*30] all gradients (tests)
     double *mnmx; // [N*20] neighbourhood min/max (tests)
     int c_begin, c_end; // cells [c_begin, c_end); c_end = 0: all cells
-    const double *prims; // input: primitives [N][5] of U (null: computed)
     const float *U32;    // input of the FP32 kernel: the state in FP32 (null: U)
 };
 
@@ -155,8 +154,6 @@ struct StageParams
     int c_begin, c_end;    // cells [c_begin, c_end); c_end = 0: all cells
     const double *case_dt; // batched cases: dt of case cell_case[c] (sc unused)
     const int *cell_case;
-    const double *prims_in; // primitives [N][5] of U_in (null: computed)
-    double *prims_out;      // primitives of U_out, written by the stage update
     const float *U32;       // the FP32 kernel's input state in FP32 (null: U_in)
     int tile_begin;         // face-centred kernel: first tile of the launch
 };
@@ -236,9 +233,6 @@ void launch_dt_partials(hipStream_t s, const DevMesh &m, const PhysDev &P,
 void launch_batch_stage(hipStream_t s, const DevMesh &m, const PhysDev &P,
                         const StageParams &sp, const BatchTiles &bt, int flux, bool viscous,
                         int block, int grid);
-// Primitive rows [N][5] (u, v, w, p, T) of U (con_to_prim).
-void launch_prims(hipStream_t s, const DevMesh &m, const PhysDev &P, const double *U,
-                  double *prims);
 // Host-mirror rows: AoS phi_total[(N+B)*10] (conservative columns of U,
 // primitives by con_to_prim of U_prims and ghost rows by the boundary
 // conditions from U_prims; U_prims null: U) and phi_dt[N*5] (R may be null).

@@ -141,7 +141,7 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
     if (f32)
         load_phi(P, U32, nullptr, c, pc);
     else
-        load_phi(P, U, out.prims, c, pc);
+        load_phi(P, U, nullptr, c, pc);
 
     Real gx[10], gy[10], gz[10], mn[10], mx[10];
 #pragma unroll
@@ -191,13 +191,6 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
                 load_cons(U32, nb, pn[i]);
             else
                 load_cons(U, nb, pn[i]);
-            if (!f32 && out.prims)
-            {
-                const double *q = out.prims + (size_t)nb * 5;
-#pragma unroll
-                for (int j = 0; j < 5; ++j)
-                    pn[i][5 + j] = Real(q[j]);
-            }
             if (f32)
             {
                 const float4 *q = reinterpret_cast<const float4 *>(m.fgB32 + (size_t)fid[s] * 8);
@@ -228,7 +221,7 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
                 for (int j = 0; j < 10; ++j)
                     pn[i][j] = Real(gp[j]);
             }
-            else if (f32 || !out.prims)
+            else
                 con_to_prim(P, pn[i]);
             const bool own0 = own[s];
             const Real n0 = gn[i][0], n1 = gn[i][1], n2 = gn[i][2], area = gn[i][3],
@@ -698,9 +691,9 @@ __device__ __forceinline__ void residual_group_body(const DevMesh &m, const Phys
             constexpr bool f32 = std::is_same<Real, float>::value;
             Real pc[10], pn[10];
             if (f32 && sp.U32)
-                load_phi(P, sp.U32, sp.prims_in, c, pc);
+                load_phi(P, sp.U32, nullptr, c, pc);
             else
-                load_phi(P, sp.U_in, sp.prims_in, c, pc);
+                load_phi(P, sp.U_in, nullptr, c, pc);
             if (ghost)
             {
                 const double *gp = sp.gphi + (size_t)(nb - N) * 10;
@@ -710,11 +703,11 @@ __device__ __forceinline__ void residual_group_body(const DevMesh &m, const Phys
             }
             else if (f32 && sp.U32)
             {
-                load_phi(P, sp.U32, sp.prims_in, nb, pn);
+                load_phi(P, sp.U32, nullptr, nb, pn);
             }
             else
             {
-                load_phi(P, sp.U_in, sp.prims_in, nb, pn);
+                load_phi(P, sp.U_in, nullptr, nb, pn);
             }
 #pragma unroll
             for (int j = 0; j < 10; ++j)
@@ -835,27 +828,11 @@ __device__ __forceinline__ void residual_group_body(const DevMesh &m, const Phys
 #pragma unroll
             for (int j = 0; j < 5; ++j)
                 pc[j] = u5[j];
-            if (sp.prims_out)
-            {
-                double q[10];
-#pragma unroll
-                for (int j = 0; j < 5; ++j)
-                    q[j] = unew[j];
-                con_to_prim(P, q);
-#pragma unroll
-                for (int j = 0; j < 5; ++j)
-                    sp.prims_out[(size_t)c * 5 + j] = q[5 + j];
-            }
             if (sp.do_reduce)
             {
                 // time step with the new density and the primitives of this
                 // stage's input (navier_stokes.c:171-187)
-                if (sp.prims_in)
-#pragma unroll
-                    for (int j = 0; j < 5; ++j)
-                        pc[5 + j] = sp.prims_in[(size_t)c * 5 + j];
-                else
-                    con_to_prim(P, pc);
+                con_to_prim(P, pc);
 #pragma unroll
                 for (int j = 0; j < 5; ++j)
                     part[j] = rabs(R5[j]) * vol;
@@ -1422,7 +1399,7 @@ void launch_gradient(hipStream_t s, const DevMesh &m, const PhysDev &P,
                                sc);
         return;
     }
-    if (m.tile_halo_off && !out.prims && out.c_begin % kTile == 0 &&
+    if (m.tile_halo_off && out.c_begin % kTile == 0 &&
         (out.c_end % kTile == 0 || out.c_end == m.N))
     {
         const size_t lds = (size_t)max(10 * m.tile_rows, 16 * kTile) * sizeof(double);
@@ -1530,7 +1507,7 @@ __global__ __launch_bounds__(kTile, FVH_K2F_MINW) void fv_residual_face_kernel(D
         {
             // first order: the two cells' values (a ghost row on a boundary
             // face), as the lane-per-slot kernel takes them
-            load_phi(P, sp.U_in, sp.prims_in, r.y >> 2, L);
+            load_phi(P, sp.U_in, nullptr, r.y >> 2, L);
             if (ghost)
             {
                 const double *gp = sp.gphi + (size_t)(r.z - N * 4) * 10;
@@ -1539,7 +1516,7 @@ __global__ __launch_bounds__(kTile, FVH_K2F_MINW) void fv_residual_face_kernel(D
                     R[j] = gp[j];
             }
             else
-                load_phi(P, sp.U_in, sp.prims_in, r.z >> 2, R);
+                load_phi(P, sp.U_in, nullptr, r.z >> 2, R);
         }
         ViscGrad gL, gR;
         if (VISC)
@@ -1608,12 +1585,11 @@ __global__ __launch_bounds__(kTile, FVH_K2F_MINW) void fv_residual_face_kernel(D
             sp.U_out[g] = un;
         }
     }
-    if (!sp.do_reduce && !(sp.do_update && sp.prims_out))
+    if (!sp.do_reduce)
         return;
 
-    // per cell: the new state and residual rows through LDS (the
-    // contributions are no longer read), then the primitives of the new
-    // state and the partials of the last stage
+    // per cell, last stage: the new state and residual rows through LDS (the
+    // contributions are no longer read), then the partials
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < kE; ++i)
@@ -1639,44 +1615,23 @@ __global__ __launch_bounds__(kTile, FVH_K2F_MINW) void fv_residual_face_kernel(D
             R5[j] = ct[kTile * 5 + lc * 5 + j];
             pc[j] = sp.U_in[(size_t)c * 5 + j];
         }
-        if (sp.prims_out)
-        {
-            double q[10];
+        // time step with the new density and the primitives of this
+        // stage's input (navier_stokes.c:171-187)
+        con_to_prim(P, pc);
+        const double vol = m.vol[c];
 #pragma unroll
-            for (int j = 0; j < 5; ++j)
-                q[j] = unew[j];
-            con_to_prim(P, q);
-#pragma unroll
-            for (int j = 0; j < 5; ++j)
-                sp.prims_out[(size_t)c * 5 + j] = q[5 + j];
-        }
-        if (sp.do_reduce)
-        {
-            // time step with the new density and the primitives of this
-            // stage's input (navier_stokes.c:171-187)
-            if (sp.prims_in)
-#pragma unroll
-                for (int j = 0; j < 5; ++j)
-                    pc[5 + j] = sp.prims_in[(size_t)c * 5 + j];
-            else
-                con_to_prim(P, pc);
-            const double vol = m.vol[c];
-#pragma unroll
-            for (int j = 0; j < 5; ++j)
-                part[j] = rabs(R5[j]) * vol;
-            const double dx0 = m.cdx[(size_t)c * 3], dx1 = m.cdx[(size_t)c * 3 + 1],
-                         dx2 = m.cdx[(size_t)c * 3 + 2];
-            const double s_rho = 1.0 / unew[0];
-            const double cs = sqrt(P.kappa * pc[8] * s_rho);
-            const double ds1 = (rabs(pc[5]) + cs) * dx0 + (rabs(pc[6]) + cs) * dx1 +
-                               (rabs(pc[7]) + cs) * dx2;
-            part[5] = vol / ds1;
-            const double ds2 = dot3(dx0, dx1, dx2, dx0, dx1, dx2);
-            part[6] = s_rho * ds2 / vol;
-        }
+        for (int j = 0; j < 5; ++j)
+            part[j] = rabs(R5[j]) * vol;
+        const double dx0 = m.cdx[(size_t)c * 3], dx1 = m.cdx[(size_t)c * 3 + 1],
+                     dx2 = m.cdx[(size_t)c * 3 + 2];
+        const double s_rho = 1.0 / unew[0];
+        const double cs = sqrt(P.kappa * pc[8] * s_rho);
+        const double ds1 = (rabs(pc[5]) + cs) * dx0 + (rabs(pc[6]) + cs) * dx1 +
+                           (rabs(pc[7]) + cs) * dx2;
+        part[5] = vol / ds1;
+        const double ds2 = dot3(dx0, dx1, dx2, dx0, dx1, dx2);
+        part[6] = s_rho * ds2 / vol;
     }
-    if (!sp.do_reduce)
-        return;
     // one row per wave of 64 cells: by cell over the whole mesh, or by tile
     // (every wave of the tile writes its row)
     int wrow;
@@ -1839,27 +1794,6 @@ void launch_dt_partials(hipStream_t s, const DevMesh &m, const PhysDev &P,
                        dim3(kBlock), 0, s, m, P, U, partials, prims);
 }
 
-__global__ __launch_bounds__(kBlock) void fv_prims_kernel(int N, PhysDev P,
-                                                         const double *__restrict__ U,
-                                                         double *__restrict__ prims)
-{
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= N)
-        return;
-    double q[10];
-    load_cons(U, c, q);
-    con_to_prim(P, q);
-#pragma unroll
-    for (int j = 0; j < 5; ++j)
-        prims[(size_t)c * 5 + j] = q[5 + j];
-}
-
-void launch_prims(hipStream_t s, const DevMesh &m, const PhysDev &P, const double *U,
-                  double *prims)
-{
-    hipLaunchKernelGGL(fv_prims_kernel, dim3(cell_blocks(m.N)), dim3(kBlock), 0, s, m.N, P, U,
-                       prims);
-}
 
 void launch_mirror(hipStream_t s, const DevMesh &m, const PhysDev &P,
                    const double *U, const double *R, double *phi_total_aos,

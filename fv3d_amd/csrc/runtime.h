@@ -53,7 +53,6 @@ void noalloc_setup();
 struct StageBuffers
 {
     double *U[2];
-    double *prims[2]; // primitive rows of U[i] (kept by the stage updates)
     int *cur;
     double *k, *R;
     int stages;

@@ -150,7 +150,6 @@ struct Device
     std::vector<void *> allocs;
     dev::DevMesh m{};
     double *U[2] = {nullptr, nullptr};
-    double *prims[2] = {nullptr, nullptr}; // primitives of U[i] during stepping
     int cur = 0;
     // the buffer whose primitives the reference's phi_total holds: after an
     // RK step, the last stage's input (its update hook ran on that state);
@@ -423,15 +422,6 @@ struct KernelTimer
     }
 };
 
-void ensure_prims()
-{
-    if (!g_dev.prims[0])
-    {
-        g_dev.prims[0] = dmalloc<double>((size_t)5 * g_dev.m.N);
-        g_dev.prims[1] = dmalloc<double>((size_t)5 * g_dev.m.N);
-    }
-}
-
 // ---- host-mirror residual path ---------------------------------------------
 mirror::MirrorMesh mirror_mesh()
 {
@@ -564,7 +554,6 @@ void mirror_residual(double t, const double *U_in, double *R_out, bool host_all)
 
 void enqueue_residual(const double *U_in, double *U_out, double *R_out,
                       int stage, bool update, bool reduce, dev::DevScalars *sc,
-                      const double *prims_in = nullptr, double *prims_out = nullptr,
                       KernelTimer *tm = nullptr, bool fp32 = false,
                       const float *U32 = nullptr)
 {
@@ -608,7 +597,6 @@ void enqueue_residual(const double *U_in, double *U_out, double *R_out,
                       is_linear() && !fs32 ? g_dev.fs : nullptr, sc,
                       fs32 ? g_dev.fs32 : nullptr, U32);
     dev::GradOut go = grad_out();
-    go.prims = prims_in;
     go.U32 = U32;
     if (fs32)
     {
@@ -642,8 +630,6 @@ void enqueue_residual(const double *U_in, double *U_out, double *R_out,
     sp.sc = sc;
     sp.partials = g_dev.partials;
     sp.partials_cap = g_dev.partials_cap;
-    sp.prims_in = prims_in;
-    sp.prims_out = update ? prims_out : nullptr;
     if (tm)
         tm->start();
     dev::launch_residual(g_dev.stream, g_dev.m, P, sp, g_flux, is_linear(),
@@ -1139,7 +1125,7 @@ void enqueue_explicit_step(dev::DevScalars *sc, KernelTimer *tm = nullptr, doubl
         // costs more than the divides it saves (8.57 against 8.06 ms per
         // step on the 96^3 Taylor-Green mesh)
         enqueue_residual(Uin, Uout, last ? g_dev.R : nullptr, s, true, last,
-                         sc, nullptr, nullptr, tm);
+                         sc, tm);
         g_dev.cur = 1 - g_dev.cur;
     }
     if (tm)
@@ -1669,8 +1655,7 @@ RuntimeView runtime_view(bool allocate)
 
 void eval_residual(const double *U, double *R, bool fp32, const float *U32)
 {
-    enqueue_residual(U, nullptr, R, 0, false, false, nullptr, nullptr, nullptr, nullptr,
-                     fp32, U32);
+    enqueue_residual(U, nullptr, R, 0, false, false, nullptr, nullptr, fp32, U32);
 }
 
 void state_changed() { g_dev.r_valid = false; }
@@ -1681,11 +1666,8 @@ StageBuffers stage_buffers()
     if (g_rk_stages <= 0)
         throw std::runtime_error("explicit scheme not initialised");
     StageBuffers b{};
-    ensure_prims();
     b.U[0] = g_dev.U[0];
     b.U[1] = g_dev.U[1];
-    b.prims[0] = g_dev.prims[0];
-    b.prims[1] = g_dev.prims[1];
     b.cur = &g_dev.cur;
     b.k = g_dev.k;
     b.R = g_dev.R;
