@@ -81,6 +81,7 @@ struct BatchState
         *f_row_hi = nullptr;
     int4 *f_faces = nullptr;
     double *f_partials = nullptr;
+    std::vector<int> f_case_t0; // a case's tiles f_case_t0[k] .. f_case_t0[k+1] - 1
 };
 
 
@@ -314,7 +315,8 @@ void enqueue_step_persistent(const RuntimeView &v, const StageBuffers &sb,
 // viscous gradient subset (viscous groups) and the face-centred residual
 // kernel over the group's tiles; the last stage's partials per tile, folded
 // per case.
-void enqueue_step_face(const RuntimeView &v, const StageBuffers &sb)
+void enqueue_step_face(const RuntimeView &v, const StageBuffers &sb,
+                       unsigned long long *stamps = nullptr)
 {
     hipStream_t s = v.stream;
     dev::DevMesh m = v.m;
@@ -355,6 +357,12 @@ void enqueue_step_face(const RuntimeView &v, const StageBuffers &sb)
             sp.c_end = g.c_end;
             sp.case_dt = g_b.d_dt;
             sp.cell_case = g_b.cell_case;
+            sp.stamps = stamps ? stamps + (size_t)st * 2 * g_b.f_tiles : nullptr;
+            // consecutive tiles (a case's) spread over the XCDs: each case is
+            // done in a short window of the launch (in-batch p50 / p99 step
+            // time 0.11 / 0.23 ms against 0.41 / 1.45 ms with XCD ranges),
+            // and the step is no slower
+            sp.tiles_round_robin = 1;
             dev::launch_residual_tiles(s, m, P, sp, g.flux, false, g.viscous != 0, g.t_begin,
                                        g.t_end - g.t_begin);
         }
@@ -581,6 +589,7 @@ int fvh_batch_setup(int n_cases, const int *flux, const int *viscous, const doub
                 rhi[k] = (int)c0.size() * rows;
             }
             case_t0[n_cases] = (int)c0.size();
+            g_b.f_case_t0 = case_t0;
             g_b.f_tiles = (int)c0.size();
             c0.push_back(N);
             int k = 0;
@@ -749,16 +758,20 @@ int fvh_batch_set_schedule(int persistent, int block, int grid, int sub_tiles)
 int fvh_batch_case_step_times(int steps, double *ms)
 {
     B_TRY({
-        if (!g_b.ready || !g_b.persistent)
-            throw std::runtime_error("fvh_batch_case_step_times: needs the persistent schedule");
+        if (!g_b.ready || !(g_b.persistent || g_b.face))
+            throw std::runtime_error("fvh_batch_case_step_times: needs the persistent or the "
+                                     "face schedule");
         if (steps < 1 || !ms)
             throw std::runtime_error("fvh_batch_case_step_times: bad arguments");
         RuntimeView v = runtime_view();
         StageBuffers sb = stage_buffers();
         hipStream_t s = v.stream;
-        const size_t n = (size_t)sb.stages * 2 * g_b.n_tiles;
+        // the schedule's tiles and each case's range of them
+        const int tiles = g_b.face ? g_b.f_tiles : g_b.n_tiles;
+        const size_t n = (size_t)sb.stages * 2 * tiles;
         if (!g_b.stamps)
-            g_b.stamps = balloc<unsigned long long>((size_t)kMaxStages * 2 * g_b.n_tiles);
+            g_b.stamps = balloc<unsigned long long>(
+                (size_t)kMaxStages * 2 * std::max(g_b.n_tiles, g_b.f_tiles));
         std::vector<unsigned long long> h(n);
         std::vector<double> acc(g_b.K, 0.0);
         int dev = 0, khz = 100000;
@@ -766,7 +779,10 @@ int fvh_batch_case_step_times(int steps, double *ms)
         FVH_HIP_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
         for (int it = 0; it < steps; ++it)
         {
-            enqueue_step_persistent(v, sb, g_b.stamps);
+            if (g_b.face)
+                enqueue_step_face(v, sb, g_b.stamps);
+            else
+                enqueue_step_persistent(v, sb, g_b.stamps);
             FVH_HIP_CHECK(hipMemcpyAsync(h.data(), g_b.stamps, n * sizeof(unsigned long long),
                                          hipMemcpyDeviceToHost, s));
             FVH_HIP_CHECK(hipStreamSynchronize(s));
@@ -774,11 +790,13 @@ int fvh_batch_case_step_times(int steps, double *ms)
             // the end of its last, inside the shared launch
             for (int st = 0; st < sb.stages; ++st)
             {
-                const unsigned long long *p = h.data() + (size_t)st * 2 * g_b.n_tiles;
+                const unsigned long long *p = h.data() + (size_t)st * 2 * tiles;
                 for (int k = 0; k < g_b.K; ++k)
                 {
                     unsigned long long lo = ~0ull, hi = 0;
-                    for (int t = g_b.h_tile_lo[k]; t < g_b.h_tile_hi[k]; ++t)
+                    const int t0 = g_b.face ? g_b.f_case_t0[k] : g_b.h_tile_lo[k],
+                              t1 = g_b.face ? g_b.f_case_t0[k + 1] : g_b.h_tile_hi[k];
+                    for (int t = t0; t < t1; ++t)
                     {
                         lo = std::min(lo, p[(size_t)t * 2]);
                         hi = std::max(hi, p[(size_t)t * 2 + 1]);

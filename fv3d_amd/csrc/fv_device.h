@@ -156,6 +156,13 @@ struct StageParams
     const int *cell_case;
     const float *U32;       // the FP32 kernel's input state in FP32 (null: U_in)
     int tile_begin;         // face-centred kernel: first tile of the launch
+    // face-centred kernel over tile_c0 tiles: wall-clock start and end of
+    // tile t at stamps[2t], stamps[2t + 1] (null: not recorded)
+    unsigned long long *stamps;
+    // face-centred kernel: 1 = workgroup b takes tile tile_begin + b (the
+    // hardware spreads consecutive tiles over the XCDs); 0 = each XCD takes
+    // a contiguous range of tiles
+    int tiles_round_robin;
 };
 
 constexpr int kBlock = 256;

@@ -1456,10 +1456,15 @@ __global__ __launch_bounds__(kTile, FVH_K2F_MINW) void fv_residual_face_kernel(D
         return;
     extern __shared__ double ct[]; // [kTile * 4][5]
     const int N = m.N;
-    const int t =
-        sp.tile_begin + (FVH_K2F_REV ? xcd_tile_rev(gridDim.x) : xcd_tile(gridDim.x));
+    const int t = sp.tile_begin + (sp.tiles_round_robin ? (int)blockIdx.x
+                                   : FVH_K2F_REV ? xcd_tile_rev(gridDim.x)
+                                                 : xcd_tile(gridDim.x));
     const int base = m.tile_c0 ? m.tile_c0[t] : t * kTile;
     const int ncell = m.tile_c0 ? m.tile_c0[t + 1] - base : min(kTile, N - base);
+    // per-tile timing (tile_c0 tiles only: every thread reaches the end)
+    const bool stamp = sp.stamps && m.tile_c0;
+    if (stamp && threadIdx.x == 0)
+        sp.stamps[(size_t)t * 2] = wall_clock64();
     const int e0 = m.tile_face_off[t], e1 = m.tile_face_off[t + 1];
     // element p = lc * 5 + j of the tile's rows; with FVH_K2F_PRE the
     // update's operands are loaded before the face work so that their
@@ -1586,7 +1591,15 @@ __global__ __launch_bounds__(kTile, FVH_K2F_MINW) void fv_residual_face_kernel(D
         }
     }
     if (!sp.do_reduce)
+    {
+        if (stamp)
+        {
+            __syncthreads();
+            if (threadIdx.x == 0)
+                sp.stamps[(size_t)t * 2 + 1] = wall_clock64();
+        }
         return;
+    }
 
     // per cell, last stage: the new state and residual rows through LDS (the
     // contributions are no longer read), then the partials
@@ -1653,6 +1666,12 @@ __global__ __launch_bounds__(kTile, FVH_K2F_MINW) void fv_residual_face_kernel(D
 #pragma unroll
         for (int q = 0; q < 7; ++q)
             sp.partials[(size_t)wrow * kPartialStride + q] = w[q];
+    if (stamp)
+    {
+        __syncthreads();
+        if (threadIdx.x == 0)
+            sp.stamps[(size_t)t * 2 + 1] = wall_clock64();
+    }
 }
 
 // Whether the face-centred kernel runs a residual launch: FP64 face states
@@ -1760,6 +1779,8 @@ void launch_residual_tiles(hipStream_t s, const DevMesh &m, const PhysDev &P,
         throw std::runtime_error("launch_residual_tiles: no viscous gradients");
     if (sp.do_update && !sp.case_dt && !sp.sc)
         throw std::runtime_error("launch_residual_tiles: stage update without a time step");
+    if (sp.stamps && !m.tile_c0)
+        throw std::runtime_error("launch_residual_tiles: per-tile stamps need tile_c0 tiles");
     if (sp.do_reduce && (!m.tile_c0 || (size_t)(tile_begin + n_tiles) * (kTile / 64) >
                                            (size_t)sp.partials_cap))
         throw std::runtime_error("launch_residual_tiles: reduction needs tile rows "

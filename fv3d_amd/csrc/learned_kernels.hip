@@ -2,11 +2,17 @@
 // inference and training (forward + backward, FP32 accumulation), Adam,
 // and the block conversions.
 //
-// MFMA tiles: v_mfma_f32_16x16x16_bf16 on one 64-wide wavefront.  Lane l
-// holds A[l % 16][4 (l / 16) + q] and B[4 (l / 16) + q][l % 16] (q = 0..3)
-// of the operands and D[4 (l / 16) + q][l % 16] of the result.  Matrices
-// are staged row-major in LDS in FP32; fragments are rounded to BF16
-// (round to nearest even) when they are loaded.
+// MFMA tiles on one 64-wide wavefront, D[16][16] += A[16][K] B[K][16]:
+// - the dense layers (K a multiple of 32): gfx950's v_mfma_f32_16x16x32_bf16.
+//   Lane l holds A[l % 16][8 (l / 16) + q] and B[8 (l / 16) + q][l % 16]
+//   (q = 0..7); the activation fragments of a 16-row tile are loaded once
+//   and reused for every 16-column output tile;
+// - the weight-gradient products over the 16 rows of a tile (K = 16):
+//   v_mfma_f32_16x16x16_bf16, lane l holding A[l % 16][4 (l / 16) + q] and
+//   B[4 (l / 16) + q][l % 16] (q = 0..3).
+// Both give D[4 (l / 16) + q][l % 16] (q = 0..3) in lane l.  Matrices are
+// staged row-major in LDS in FP32; fragments are rounded to BF16 (round to
+// nearest even) when they are loaded.
 #include <cfloat>
 #include <cmath>
 
@@ -22,6 +28,7 @@ using dev::PhysDev;
 namespace {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kB = 256;
@@ -31,6 +38,11 @@ __device__ __forceinline__ short f2bf(float f)
     unsigned u = __float_as_uint(f);
     u += 0x7fffu + ((u >> 16) & 1u);
     return (short)(u >> 16);
+}
+
+__device__ __forceinline__ __bf16 f2bf8(float f)
+{
+    return __builtin_bit_cast(__bf16, f2bf(f));
 }
 
 __device__ __forceinline__ int lane() { return threadIdx.x & 63; }
@@ -68,21 +80,56 @@ __device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c)
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+// The K = 32 fragments: A[i][k] of the 16x32 tile (row0, k0) of M
+// (row-major, leading dim ld) and B[k][n] of the 32x16 tile (k0, col0) of M
+// (row-major [K][N]; TRANS: B[k][n] = M[n][k]).
+__device__ __forceinline__ bf16x8 frag_a8(const float *M, int ld, int row0, int k0)
+{
+    const int l = lane();
+    const float *p = M + (row0 + (l & 15)) * ld + k0 + 8 * (l >> 4);
+    bf16x8 r;
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+        r[q] = f2bf8(p[q]);
+    return r;
+}
+
+template <bool TRANS>
+__device__ __forceinline__ bf16x8 frag_b8(const float *M, int ld, int k0, int col0)
+{
+    const int l = lane();
+    const int n = col0 + (l & 15), kb = k0 + 8 * (l >> 4);
+    bf16x8 r;
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+        r[q] = f2bf8(TRANS ? M[n * ld + kb + q] : M[(kb + q) * ld + n]);
+    return r;
+}
+
+__device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
 // out[16][N] = act(A[16][K] W[K][N] + bias), A and W row-major in LDS.
 // ACT 0: identity, 1: tanh.
 template <int K, int N, int ACT>
 __device__ __forceinline__ void dense(const float *A, const float *W, const float *bias,
                                       float *out)
 {
+    static_assert(K % 32 == 0 && N % 16 == 0, "dense: K a multiple of 32");
     const int l = lane();
+    bf16x8 a[K / 32];
+#pragma unroll
+    for (int kt = 0; kt < K / 32; ++kt)
+        a[kt] = frag_a8(A, K, 0, 32 * kt);
 #pragma unroll
     for (int nt = 0; nt < N / 16; ++nt)
     {
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int kt = 0; kt < K / 16; ++kt)
-            acc = mfma(frag_a<false>(A, K, 0, 16 * kt), frag_b<false>(W, N, 16 * kt, 16 * nt),
-                       acc);
+        for (int kt = 0; kt < K / 32; ++kt)
+            acc = mfma32(a[kt], frag_b8<false>(W, N, 32 * kt, 16 * nt), acc);
         const int j = 16 * nt + (l & 15);
 #pragma unroll
         for (int q = 0; q < 4; ++q)
@@ -99,15 +146,19 @@ __device__ __forceinline__ void dense(const float *A, const float *W, const floa
 template <int K, int N>
 __device__ __forceinline__ void dense_bt(const float *A, const float *B, float *out)
 {
+    static_assert(K % 32 == 0 && N % 16 == 0, "dense_bt: K a multiple of 32");
     const int l = lane();
+    bf16x8 a[K / 32];
+#pragma unroll
+    for (int kt = 0; kt < K / 32; ++kt)
+        a[kt] = frag_a8(A, K, 0, 32 * kt);
 #pragma unroll
     for (int nt = 0; nt < N / 16; ++nt)
     {
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int kt = 0; kt < K / 16; ++kt)
-            acc = mfma(frag_a<false>(A, K, 0, 16 * kt), frag_b<true>(B, K, 16 * kt, 16 * nt),
-                       acc);
+        for (int kt = 0; kt < K / 32; ++kt)
+            acc = mfma32(a[kt], frag_b8<true>(B, K, 32 * kt, 16 * nt), acc);
 #pragma unroll
         for (int q = 0; q < 4; ++q)
             out[(4 * (l >> 4) + q) * N + 16 * nt + (l & 15)] = acc[q];

@@ -264,7 +264,8 @@ int fvh_batch_info(int *n_cases, int *offsets, int max_offsets);
  * last stage.  Both run one gradient launch
  * per run of viscous cases and give the same states and time steps. */
 int fvh_batch_set_schedule(int persistent, int block, int grid, int sub_tiles);
-/* Per-case step time inside the batched launches (persistent schedule):
+/* Per-case step time inside the batched launches (persistent or face
+ * schedule; the stage kernel's launches, not the gradient launches):
  * runs `steps` steps and returns in ms[K] each case's mean per step of the
  * time from the start of its first tile to the end of its last, summed over
  * the stages (device wall clock). */

@@ -79,13 +79,14 @@ def test_batch_matches_single_case_oracle(graph, schedule):
     b.close()
 
 
-def test_batch_case_step_times_inside_the_launch():
-    """Per-case step times measured inside the shared persistent launches:
-    one positive time per case, the larger cases taking longer; the steps
-    they time are ordinary steps (the states still match the oracle)."""
+@pytest.mark.parametrize("schedule", ["persistent-256-sub4", "face"])
+def test_batch_case_step_times_inside_the_launch(schedule):
+    """Per-case step times measured inside the shared launches (persistent
+    or face schedule): one positive time per case, the larger cases taking
+    longer; the steps they time are ordinary steps (the states still match
+    the oracle)."""
     ref = _oracle_runs()
-    b = B.Batch(CASES, tile=TILE, mu_ns=MU,
-                schedule=dict(persistent=True, block=256, grid=0, sub_tiles=4))
+    b = B.Batch(CASES, tile=TILE, mu_ns=MU, schedule=SCHEDULES[schedule])
     b.run(STEPS - 2, graph=True)
     ms = b.case_step_times(2)
     assert ms.shape == (len(CASES),) and np.all(ms > 0) and np.all(np.isfinite(ms))
