@@ -54,6 +54,39 @@ def test_config_from_reference_json(tmp_path):
     assert cfg["title"] == "drv" and restart is None
 
 
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def test_config_from_the_reference_parameter_file():
+    """The reference's own parameter file (misc/gmsh/naca0012/
+    naca0012_simulation.json, copied as a fixture by
+    tests/golden/make_reference_params.py): every key the solver reads maps
+    to the case configuration, with the values the file holds; the
+    isothermal wall carries only T, as boundary.c:99-105 reads it."""
+    with open(os.path.join(GOLDEN, "naca0012_simulation.json")) as f:
+        params = json.load(f)
+    cfg, restart = driver.config_from_json(params, "/case")
+    assert cfg["mesh"] == ("file", "/case/naca0012.mesh.h5")
+    assert cfg["title"] == "untitled" and restart is None
+    assert cfg["physics"] == dict(cfl_scale=1.0, dfl_scale=1.0, mu_mix=1.813e-5, R_mix=287.102,
+                                  Pr=0.718, kappa=1.4, flux="AUSM")
+    assert cfg["regions"] == {
+        "inflow": ("inflow", (100.0, 20.0, 0.0, 1e5, 300.0)),
+        "outflow": ("outflow", None),
+        "profile": ("wall-isothermal", (0.0, 0.0, 0.0, 0.0, 800.0)),
+        "flow": ("flow", (100.0, 20.0, 0.0, 1e5, 300.0))}
+    assert cfg["numerics"] == dict(
+        reconstruction="Linear", limiter="Barth-Jespersenn", i_output_data=10,
+        timestep="Explicit", max_iter=250000, transient=True, abort_residual=1e-10,
+        t_start=0.0, t_end=2.0, scheme="RK3-3")
+    # the same file's implicit settings
+    params["TimeDisc"]["timestep"]["value"] = "Implicit"
+    n = driver.config_from_json(params, "/case")[0]["numerics"]
+    assert (n["scheme"], n["solver"], n["max_iter_inner"]) == ("Euler", "BiCGStab", 100)
+    assert (n["tolerance_lsoe"], n["max_iter_lsoe"], n["max_krylov_dims"],
+            n["max_krylov_restarts"]) == (1e-3, 100, 10, 3)
+
+
 @pytest.mark.gpu
 def test_driver_run_equals_direct_case(tmp_path):
     L = lib()
