@@ -79,14 +79,28 @@ def test_taylor_green_gradients_and_residual():
     assert er <= TOL
 
 
-def test_taylor_green_steps():
+@pytest.mark.parametrize("flux,mu,limiter", [("AUSM", True, "Barth-Jespersenn"),
+                                              ("Rusanov", True, "Barth-Jespersenn"),
+                                              ("Rusanov", False, "Barth-Jespersenn"),
+                                              ("AUSM", False, "None")])
+def test_taylor_green_steps(flux, mu, limiter):
+    """Every instantiation of the face-centred residual kernel on face
+    states (AUSM / Rusanov, viscous / inviscid) and the unlimited linear
+    reconstruction, 10 RK3-3 steps on 6^3 x 6 tetrahedra (the last cell
+    tile partial)."""
     cfg = configs.taylor_green(n=6, steps=10, tile=2)
+    cfg["physics"]["flux"] = flux
+    if not mu:
+        cfg["physics"]["mu_mix"] = 0.0
+        cfg["physics"]["visc_dt_mode"] = 1
+    cfg["numerics"]["limiter"] = limiter
     case, o, _ = make_pair(cfg)
     rc, log_g = case.run()
     n, log_o = o.timedisc("RK3-3", 10, True, 1e-10, 0.0, 1e300)
     assert rc == 0 and n == 10
     e = rel_err(case.state(), o.state())
-    _print("tgv U(10)", rel=e, viscous_flags=log_g[:, 2].tolist())
+    _print("tgv U(10)", flux=flux, viscous=mu, limiter=limiter, rel=e,
+           viscous_flags=log_g[:, 2].tolist())
     assert e <= TOL
     assert np.array_equal(log_g[:, 2], log_o[:, 2])
 
