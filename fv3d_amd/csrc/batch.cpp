@@ -14,7 +14,6 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <cstdlib>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -309,12 +308,10 @@ void enqueue_step_persistent(const RuntimeView &v, const StageBuffers &sb,
     batch::reset_counters(s, g_b.counters, kCounters);
 }
 
-// Enqueues one RK step of every case (the stages of every group, then the
-// per-case residuals and next time steps).
 // One RK step of every case on the face schedule: per stage and group, the
-// viscous gradient subset (viscous groups) and the face-centred residual
-// kernel over the group's tiles; the last stage's partials per tile, folded
-// per case.
+// viscous gradients (viscous groups) and the face-centred residual kernel
+// over the group's tiles; the last stage's partials per tile, folded per
+// case.
 void enqueue_step_face(const RuntimeView &v, const StageBuffers &sb,
                        unsigned long long *stamps = nullptr)
 {
@@ -372,6 +369,8 @@ void enqueue_step_face(const RuntimeView &v, const StageBuffers &sb,
                      g_b.d_vol, g_b.d_dt, g_b.d_t, g_b.d_res);
 }
 
+// Enqueues one RK step of every case on the selected schedule (the stages of
+// every group, then the per-case residuals and next time steps).
 void enqueue_step(const RuntimeView &v, const StageBuffers &sb)
 {
     if (g_b.face)
@@ -401,7 +400,7 @@ void enqueue_step(const RuntimeView &v, const StageBuffers &sb)
                 o.vg = g_b.vg;
                 o.c_begin = g.c_begin;
                 o.c_end = g.c_end;
-                    dev::launch_gradient(s, m, P, Uin, nullptr, o, 0, nullptr);
+                dev::launch_gradient(s, m, P, Uin, nullptr, o, 0, nullptr);
             }
             dev::StageParams sp{};
             sp.U_in = Uin;
@@ -417,7 +416,7 @@ void enqueue_step(const RuntimeView &v, const StageBuffers &sb)
             sp.c_end = g.c_end;
             sp.case_dt = g_b.d_dt;
             sp.cell_case = g_b.cell_case;
-                    dev::launch_residual(s, m, P, sp, g.flux, false, g.viscous != 0);
+            dev::launch_residual(s, m, P, sp, g.flux, false, g.viscous != 0);
         }
         *sb.cur = 1 - *sb.cur;
     }

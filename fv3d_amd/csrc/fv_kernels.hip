@@ -1423,17 +1423,22 @@ void launch_gradient(hipStream_t s, const DevMesh &m, const PhysDev &P,
 }
 
 // ---------------------------------------------------------------------------
-// K2 on face states, face-centred (tetrahedral meshes): the workgroup of tile
-// t evaluates each face of its kTile cells once (a face between two of the
-// tile's cells gives both their contributions; a face to a cell outside the
-// tile is also evaluated by that cell's tile), with the same states,
-// gradients and flux as the lane-per-slot kernel, and keeps the
-// contributions +-flux*area per cell slot in LDS.  Then one thread per
-// (cell, variable) sums the cell's slots in slot order (the lane-per-slot
-// kernel's order) and applies the 2N-storage RK stage, so that the stores of
-// a wave are contiguous; in the last stage one thread per cell forms the
-// residual-norm and time-step partials, one row per 64 cells (the rows of
-// the lane-per-slot kernel's grid).  LDS: kTile * 4 * 5 doubles.
+// K2 face-centred (tetrahedral meshes): the workgroup of tile t evaluates
+// each face of its cells once (a face between two of the tile's cells gives
+// both their contributions; a face to a cell outside the tile is also
+// evaluated by that cell's tile), with the same states, gradients and flux
+// as the lane-per-slot kernel, and keeps the contributions +-flux*area per
+// cell slot in LDS.  Then one thread per (cell, variable) sums the cell's
+// slots in slot order (the lane-per-slot kernel's order) and applies the
+// 2N-storage RK stage, so that the stores of a wave are contiguous; in the
+// last stage one thread per cell forms the residual-norm and time-step
+// partials, one row per 64 cells (over the whole mesh: the rows of the
+// lane-per-slot kernel's grid; with tile_c0 tiles: kTile / 64 rows per
+// tile).  FS: the face states written by the gradient kernel; otherwise
+// (first order) the two cells' values.  Tiles are kTile consecutive cells,
+// or DevMesh::tile_c0's (batched cases: no tile spans two cases); a launch
+// covers tiles sp.tile_begin .. + gridDim.x - 1.  LDS: kTile * 4 * 5
+// doubles.
 // ---------------------------------------------------------------------------
 #ifndef FVH_K2F_MINW
 #define FVH_K2F_MINW 3
