@@ -355,11 +355,6 @@ void enqueue_step_face(const RuntimeView &v, const StageBuffers &sb,
             sp.case_dt = g_b.d_dt;
             sp.cell_case = g_b.cell_case;
             sp.stamps = stamps ? stamps + (size_t)st * 2 * g_b.f_tiles : nullptr;
-            // consecutive tiles (a case's) spread over the XCDs: each case is
-            // done in a short window of the launch (in-batch p50 / p99 step
-            // time 0.11 / 0.23 ms against 0.41 / 1.45 ms with XCD ranges),
-            // and the step is no slower
-            sp.tiles_round_robin = 1;
             dev::launch_residual_tiles(s, m, P, sp, g.flux, false, g.viscous != 0, g.t_begin,
                                        g.t_end - g.t_begin);
         }

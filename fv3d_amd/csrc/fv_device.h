@@ -159,10 +159,6 @@ struct StageParams
     // face-centred kernel over tile_c0 tiles: wall-clock start and end of
     // tile t at stamps[2t], stamps[2t + 1] (null: not recorded)
     unsigned long long *stamps;
-    // face-centred kernel: 1 = workgroup b takes tile tile_begin + b (the
-    // hardware spreads consecutive tiles over the XCDs); 0 = each XCD takes
-    // a contiguous range of tiles
-    int tiles_round_robin;
 };
 
 constexpr int kBlock = 256;

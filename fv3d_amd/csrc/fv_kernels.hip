@@ -392,7 +392,8 @@ __global__ __launch_bounds__(kTile, FVH_K1T_MINW) void fv_gradient_tile_kernel(
     extern __shared__ double tl[];
     const int N = m.N;
     const int rows = m.tile_rows;
-    const int t = out.c_begin / kTile + xcd_tile(gridDim.x);
+    // tiles in launch order, as in the face-centred residual kernel
+    const int t = out.c_begin / kTile + (int)blockIdx.x;
     const int base = t * kTile;
     const int h0 = m.tile_halo_off[t];
     const int nrow = kTile + (m.tile_halo_off[t + 1] - h0);
@@ -1448,10 +1449,6 @@ void launch_gradient(hipStream_t s, const DevMesh &m, const PhysDev &P,
 #ifndef FVH_K2F_PRE
 #define FVH_K2F_PRE 2
 #endif
-// 1: each XCD walks its chunk of tiles from the end (see xcd_tile_rev)
-#ifndef FVH_K2F_REV
-#define FVH_K2F_REV 1
-#endif
 template <int FLUX, bool FS, bool VISC>
 __global__ __launch_bounds__(kTile, FVH_K2F_MINW) void fv_residual_face_kernel(DevMesh m,
                                                                               PhysDev P,
@@ -1461,9 +1458,11 @@ __global__ __launch_bounds__(kTile, FVH_K2F_MINW) void fv_residual_face_kernel(D
         return;
     extern __shared__ double ct[]; // [kTile * 4][5]
     const int N = m.N;
-    const int t = sp.tile_begin + (sp.tiles_round_robin ? (int)blockIdx.x
-                                   : FVH_K2F_REV ? xcd_tile_rev(gridDim.x)
-                                                 : xcd_tile(gridDim.x));
+    // tiles in launch order: the hardware hands consecutive workgroups to
+    // the 8 XCDs in turn, so all XCDs work on one compact region of the mesh
+    // at a time (5% faster than XCD-contiguous tile ranges on the 96^3 x 6
+    // Taylor-Green mesh; batched cases: each case done in a short window)
+    const int t = sp.tile_begin + (int)blockIdx.x;
     const int base = m.tile_c0 ? m.tile_c0[t] : t * kTile;
     const int ncell = m.tile_c0 ? m.tile_c0[t + 1] - base : min(kTile, N - base);
     // per-tile timing (tile_c0 tiles only: every thread reaches the end)

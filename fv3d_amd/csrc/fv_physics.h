@@ -247,18 +247,6 @@ __device__ __forceinline__ int xcd_tile(int nblk)
     return x * q + (x < r ? x : r) + i;
 }
 
-// The same XCD chunks, each walked from its end: a kernel that reads what
-// the previous launch (mapped by xcd_tile) wrote starts with the tiles that
-// launch wrote last.
-__device__ __forceinline__ int xcd_tile_rev(int nblk)
-{
-    const int b = blockIdx.x;
-    const int x = b & 7, i = b >> 3;
-    const int q = nblk >> 3, r = nblk & 7;
-    const int len = q + (x < r ? 1 : 0);
-    return x * q + (x < r ? x : r) + (len - 1 - i);
-}
-
 // Viscous gradient subset: momentum (vars 1,2,3) and T (9) per direction,
 // v[d*4 + q] = d/dx_d of var {1,2,3,9}[q].
 template <class Real> struct ViscGradT

@@ -16,7 +16,6 @@
 // of the face (face_shift), so that lambda and dist_cell_2 are measured
 // through the periodic image.
 #include <algorithm>
-#include <cstdlib>
 #include <array>
 #include <cmath>
 #include <cstdint>
@@ -367,15 +366,7 @@ int classify_box(const int (*lat)[3], int nfv, const void *vctx)
     throw std::runtime_error("mesh: unclassified boundary face");
 }
 
-// Hexahedra in lexicographic (tile = 0) or blocked order: tile^3 blocks,
-// the blocks in super-blocks of kSuper^3 blocks, each level lexicographic.
-// The residual kernels work on tiles of consecutive cells and each XCD on a
-// contiguous range of tiles; with two levels the tiles one XCD has in
-// flight at a time lie in a compact region, so more of the face-neighbour
-// rows their halos read are still in that XCD's L2 (96^3 hexahedra x 6
-// tetrahedra, tile 4: 3^3 super-blocks 2.5% faster than one level, a
-// Z-order curve over the blocks 1.7%).
-constexpr int kSuper = 3;
+// hexahedra in lexicographic (tile = 0) or tile^3-blocked order
 std::vector<std::array<int, 3>> hex_order(int nx, int ny, int nz, int tile)
 {
     std::vector<std::array<int, 3>> o;
@@ -388,23 +379,13 @@ std::vector<std::array<int, 3>> hex_order(int nx, int ny, int nz, int tile)
                     o.push_back({i, j, k});
         return o;
     }
-    // blocks grouped in kSuper^3 super-blocks (FVH_MESH_SUPER overrides;
-    // 1: one level)
-    int sup = kSuper;
-    if (const char *e = std::getenv("FVH_MESH_SUPER"))
-        sup = std::atoi(e);
-    sup = std::max(1, sup);
-    const int st = tile * sup;
-    for (int ks = 0; ks < nz; ks += st)
-        for (int js = 0; js < ny; js += st)
-            for (int is = 0; is < nx; is += st)
-                for (int kb = ks; kb < std::min(nz, ks + st); kb += tile)
-                    for (int jb = js; jb < std::min(ny, js + st); jb += tile)
-                        for (int ib = is; ib < std::min(nx, is + st); ib += tile)
-                            for (int k = kb; k < std::min(nz, kb + tile); ++k)
-                                for (int j = jb; j < std::min(ny, jb + tile); ++j)
-                                    for (int i = ib; i < std::min(nx, ib + tile); ++i)
-                                        o.push_back({i, j, k});
+    for (int kb = 0; kb < nz; kb += tile)
+        for (int jb = 0; jb < ny; jb += tile)
+            for (int ib = 0; ib < nx; ib += tile)
+                for (int k = kb; k < std::min(nz, kb + tile); ++k)
+                    for (int j = jb; j < std::min(ny, jb + tile); ++j)
+                        for (int i = ib; i < std::min(nx, ib + tile); ++i)
+                            o.push_back({i, j, k});
     return o;
 }
 

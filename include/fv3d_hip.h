@@ -32,9 +32,8 @@ const char *fvh_last_error(void);
  * optionally Kuhn-split into 6 tetrahedra each (tets != 0).  periodic is a
  * bit mask (1 = x, 2 = y, 4 = z); non-periodic sides become boundary
  * regions "xmin","xmax","ymin","ymax","zmin","zmax" (in that order, after
- * the flow region "flow").  tile > 0 orders the hexahedra in tile^3 blocks,
- * the blocks in super-blocks of 3^3 blocks, for cache locality
- * (0 = lexicographic x-fastest).  Builds solver_mesh
+ * the flow region "flow").  tile > 0 orders the hexahedra in tile^3 blocks
+ * for cache locality (0 = lexicographic x-fastest).  Builds solver_mesh
  * exactly as init_mesh does (reader layout + calc_mesh_metrics). */
 int fvh_mesh_box(int nx, int ny, int nz, double x0, double y0, double z0,
                  double lx, double ly, double lz, int tets, int periodic,
