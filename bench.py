@@ -160,7 +160,7 @@ def main():
         ach = B_CELL_STAGE * N / stage_s / 1e9
         roofline = {
             "bound": "hbm",
-            "kernel": "explicit stage = fv_gradient_tile_kernel<4> + " + k2,
+            "kernel": "explicit stage = fv_gradient_tile_kernel<4, double> + " + k2,
             "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": round(ach / PEAK_HBM_GBS, 4),
             "traffic": (traffic["gradient"] + traffic["residual"]) if traffic else None,
@@ -178,7 +178,7 @@ def main():
                 "traffic": traffic["residual"] if traffic else None,
                 "avg_us": round(k_us[1], 2)},
             "gradient_kernel": {
-                "kernel": "fv_gradient_tile_kernel<4>",
+                "kernel": "fv_gradient_tile_kernel<4, double>",
                 "budget_bytes_per_cell": B_GRAD,
                 "achieved": round(ach_grad, 1), "unit": "GB/s",
                 "frac": round(ach_grad / PEAK_HBM_GBS, 4),

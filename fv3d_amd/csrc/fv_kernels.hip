@@ -2,7 +2,7 @@
 //
 // Per RK stage (or residual evaluation):
 //   fv_ghost_kernel     ghost rows of the boundary faces (B threads)
-//   fv_gradient_tile_kernel (FP64, tables built) / fv_gradient_kernel
+//   fv_gradient_tile_kernel (FP64 or FP32, tables built) / fv_gradient_kernel
 //                       one thread per cell: Green-Gauss gradients and
 //                       neighbourhood min/max (the tile kernel from its
 //                       tile's states staged in LDS), then the cell's
@@ -381,7 +381,7 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
 #ifndef FVH_K1T_PREREC
 #define FVH_K1T_PREREC 0
 #endif
-template <int MAXF>
+template <int MAXF, class Real>
 __global__ __launch_bounds__(kTile, FVH_K1T_MINW) void fv_gradient_tile_kernel(
     DevMesh m, PhysDev P, const double *__restrict__ U,
     const double *__restrict__ gphi, GradOut out, int linear,
@@ -389,7 +389,13 @@ __global__ __launch_bounds__(kTile, FVH_K1T_MINW) void fv_gradient_tile_kernel(
 {
     if (sc && sc->done)
         return;
-    extern __shared__ double tl[];
+    // FP32 (the Jacobian-vector products of the Newton-Krylov solve): the
+    // FP32 copy of the state and the FP32 face records and distance
+    // vectors, FP32 face-state rows out (the viscous gradients stay FP64
+    // storage), as fv_gradient_kernel<MAXF, float> takes and gives them
+    constexpr bool f32 = std::is_same<Real, float>::value;
+    extern __shared__ double smem[];
+    Real *tl = reinterpret_cast<Real *>(smem);
     const int N = m.N;
     const int rows = m.tile_rows;
     // tiles in launch order, as in the face-centred residual kernel
@@ -399,29 +405,24 @@ __global__ __launch_bounds__(kTile, FVH_K1T_MINW) void fv_gradient_tile_kernel(
     const int nrow = kTile + (m.tile_halo_off[t + 1] - h0);
     for (int i = threadIdx.x; i < nrow; i += kTile)
     {
-        double q[10];
-        if (i < kTile)
+        Real q[10];
+        const int g = i < kTile ? base + i : m.tile_halo[h0 + i - kTile];
+        if (i < kTile && g >= N)
+            continue;
+        if (g < N)
         {
-            if (base + i >= N)
-                continue;
-            load_cons(U, base + i, q);
+            if constexpr (f32)
+                load_cons(out.U32, g, q);
+            else
+                load_cons(U, g, q);
             con_to_prim(P, q);
         }
         else
         {
-            const int g = m.tile_halo[h0 + i - kTile];
-            if (g < N)
-            {
-                load_cons(U, g, q);
-                con_to_prim(P, q);
-            }
-            else
-            {
-                const double *gp = gphi + (size_t)(g - N) * 10;
+            const double *gp = gphi + (size_t)(g - N) * 10;
 #pragma unroll
-                for (int j = 0; j < 10; ++j)
-                    q[j] = gp[j];
-            }
+            for (int j = 0; j < 10; ++j)
+                q[j] = Real(gp[j]);
         }
 #pragma unroll
         for (int j = 0; j < 10; ++j)
@@ -434,17 +435,17 @@ __global__ __launch_bounds__(kTile, FVH_K1T_MINW) void fv_gradient_tile_kernel(
     const bool valid = c < out.c_end;
     const int cc = valid ? c : base;
 
-    double pc[10];
+    Real pc[10];
 #pragma unroll
     for (int j = 0; j < 10; ++j)
         pc[j] = tl[j * rows + threadIdx.x];
-    double gx[10], gy[10], gz[10], mn[10], mx[10];
+    Real gx[10], gy[10], gz[10], mn[10], mx[10];
 #pragma unroll
     for (int j = 0; j < 10; ++j)
     {
-        gx[j] = 0.0;
-        gy[j] = 0.0;
-        gz[j] = 0.0;
+        gx[j] = Real(0.0);
+        gy[j] = Real(0.0);
+        gz[j] = Real(0.0);
         mn[j] = pc[j];
         mx[j] = pc[j];
     }
@@ -467,12 +468,22 @@ __global__ __launch_bounds__(kTile, FVH_K1T_MINW) void fv_gradient_tile_kernel(
     // each face's record in its own iteration (FVH_K1T_PREREC 0: 158 VGPRs,
     // 3 waves per SIMD); the records of all faces first (one round of
     // gathers) need 178, 2 waves, and ran 17% slower on tetrahedra
-    double gn[MAXF][5];
+    Real gn[MAXF][5];
     auto load_rec = [&](int s) {
-        const double2 *g = reinterpret_cast<const double2 *>(m.fgB + (size_t)fid[s] * kFgB);
-        const double2 g0 = g[0], g1 = g[1], g2 = g[2];
-        gn[s][0] = g0.x, gn[s][1] = g0.y, gn[s][2] = g1.x;
-        gn[s][3] = g1.y, gn[s][4] = g2.x;
+        if constexpr (f32)
+        {
+            const float4 *q = reinterpret_cast<const float4 *>(m.fgB32 + (size_t)fid[s] * 8);
+            const float4 q0 = q[0], q1 = q[1];
+            gn[s][0] = q0.x, gn[s][1] = q0.y, gn[s][2] = q0.z;
+            gn[s][3] = q0.w, gn[s][4] = q1.x;
+        }
+        else
+        {
+            const double2 *g = reinterpret_cast<const double2 *>(m.fgB + (size_t)fid[s] * kFgB);
+            const double2 g0 = g[0], g1 = g[1], g2 = g[2];
+            gn[s][0] = g0.x, gn[s][1] = g0.y, gn[s][2] = g1.x;
+            gn[s][3] = g1.y, gn[s][4] = g2.x;
+        }
     };
     if (FVH_K1T_PREREC)
 #pragma unroll
@@ -485,22 +496,22 @@ __global__ __launch_bounds__(kTile, FVH_K1T_MINW) void fv_gradient_tile_kernel(
             break;
         if (!FVH_K1T_PREREC)
             load_rec(s);
-        double pn[10];
+        Real pn[10];
 #pragma unroll
         for (int j = 0; j < 10; ++j)
             pn[j] = tl[j * rows + loc[s]];
         const bool own0 = own[s];
-        const double n0 = gn[s][0], n1 = gn[s][1], n2 = gn[s][2], area = gn[s][3],
-                     lam = gn[s][4];
+        const Real n0 = gn[s][0], n1 = gn[s][1], n2 = gn[s][2], area = gn[s][3],
+                   lam = gn[s][4];
 #pragma unroll
         for (int j = 0; j < 10; ++j)
         {
-            const double p0 = own0 ? pc[j] : pn[j];
-            const double p1 = own0 ? pn[j] : pc[j];
-            const double pm = p0 + lam * (p1 - p0);
-            const double ax = pm * n0 * area;
-            const double ay = pm * n1 * area;
-            const double az = pm * n2 * area;
+            const Real p0 = own0 ? pc[j] : pn[j];
+            const Real p1 = own0 ? pn[j] : pc[j];
+            const Real pm = p0 + lam * (p1 - p0);
+            const Real ax = pm * n0 * area;
+            const Real ay = pm * n1 * area;
+            const Real az = pm * n2 * area;
             if (j != 8)
             {
                 if (own0)
@@ -523,7 +534,7 @@ __global__ __launch_bounds__(kTile, FVH_K1T_MINW) void fv_gradient_tile_kernel(
             }
         }
     }
-    const double sv = m.svol[cc];
+    const Real sv = Real(m.svol[cc]);
 #pragma unroll
     for (int j = 0; j < 10; ++j)
     {
@@ -531,16 +542,28 @@ __global__ __launch_bounds__(kTile, FVH_K1T_MINW) void fv_gradient_tile_kernel(
         gy[j] *= sv;
         gz[j] *= sv;
     }
-    double rv[MAXF][3];
-    if (linear && out.fs)
+    Real rv[MAXF][3];
+    const bool fs_out = f32 ? out.fs32 != nullptr : out.fs != nullptr;
+    if (linear && fs_out)
     {
 #pragma unroll
         for (int s = 0; s < MAXF; ++s)
         {
-            const double *r = m.crv + ((size_t)cc * m.maxf + (s < m.maxf ? s : 0)) * 3;
-            rv[s][0] = r[0];
-            rv[s][1] = r[1];
-            rv[s][2] = r[2];
+            const size_t sl = (size_t)cc * m.maxf + (s < m.maxf ? s : 0);
+            if constexpr (f32)
+            {
+                const float4 r = reinterpret_cast<const float4 *>(m.crv32)[sl];
+                rv[s][0] = r.x;
+                rv[s][1] = r.y;
+                rv[s][2] = r.z;
+            }
+            else
+            {
+                const double *r = m.crv + sl * 3;
+                rv[s][0] = r[0];
+                rv[s][1] = r[1];
+                rv[s][2] = r[2];
+            }
         }
     }
 
@@ -553,18 +576,18 @@ __global__ __launch_bounds__(kTile, FVH_K1T_MINW) void fv_gradient_tile_kernel(
     __syncthreads();
     if (out.vg)
     {
-        double *o = tl + threadIdx.x * 12;
+        double *o = smem + threadIdx.x * 12;
         o[0] = gx[1], o[1] = gx[2], o[2] = gx[3], o[3] = gx[9];
         o[4] = gy[1], o[5] = gy[2], o[6] = gy[3], o[7] = gy[9];
         o[8] = gz[1], o[9] = gz[2], o[10] = gz[3], o[11] = gz[9];
         __syncthreads();
-        const double2 *src = reinterpret_cast<const double2 *>(tl);
+        const double2 *src = reinterpret_cast<const double2 *>(smem);
         double2 *dst = reinterpret_cast<double2 *>(out.vg + (size_t)base * 12);
         for (int i = threadIdx.x; i < ncell * 6; i += kTile)
             dst[i] = src[i];
         __syncthreads();
     }
-    if (!out.fs)
+    if (!fs_out)
         return;
 #pragma unroll
     for (int s0 = 0; s0 < MAXF; s0 += 2)
@@ -578,22 +601,38 @@ __global__ __launch_bounds__(kTile, FVH_K1T_MINW) void fv_gradient_tile_kernel(
             const int s = s0 + h;
             if (h >= ns)
                 break;
-            double *o = tl + threadIdx.x * 16 + h * 8;
+            Real *o = tl + threadIdx.x * 16 + h * 8;
 #pragma unroll
             for (int j = 0; j < 8; ++j)
                 o[j] = linear ? limited(pc[j], gx[j], gy[j], gz[j], rv[s], mn[j], mx[j], P.bj)
                               : pc[j];
         }
         __syncthreads();
-        // 4 double2 per row, ns rows per cell
-        const double2 *src = reinterpret_cast<const double2 *>(tl);
-        const int per = 4 * ns;
-        for (int i = threadIdx.x; i < ncell * per; i += kTile)
+        if constexpr (f32)
         {
-            const int cl = i / per, q = i - cl * per;
-            double2 *dst = reinterpret_cast<double2 *>(
-                out.fs + ((size_t)(base + cl) * m.maxf + s0) * 8);
-            dst[q] = src[cl * 8 + q];
+            // 2 float4 per row, ns rows per cell
+            const float4 *src = reinterpret_cast<const float4 *>(tl);
+            const int per = 2 * ns;
+            for (int i = threadIdx.x; i < ncell * per; i += kTile)
+            {
+                const int cl = i / per, q = i - cl * per;
+                float4 *dst = reinterpret_cast<float4 *>(
+                    out.fs32 + ((size_t)(base + cl) * m.maxf + s0) * 8);
+                dst[q] = src[cl * 4 + q];
+            }
+        }
+        else
+        {
+            // 4 double2 per row, ns rows per cell
+            const double2 *src = reinterpret_cast<const double2 *>(tl);
+            const int per = 4 * ns;
+            for (int i = threadIdx.x; i < ncell * per; i += kTile)
+            {
+                const int cl = i / per, q = i - cl * per;
+                double2 *dst = reinterpret_cast<double2 *>(
+                    out.fs + ((size_t)(base + cl) * m.maxf + s0) * 8);
+                dst[q] = src[cl * 8 + q];
+            }
         }
         __syncthreads();
     }
@@ -1380,6 +1419,20 @@ void launch_gradient(hipStream_t s, const DevMesh &m, const PhysDev &P,
             throw std::runtime_error("launch_gradient: the test outputs are FP64 only");
         if (!out.U32 || !m.fgB32 || !m.crv32)
             throw std::runtime_error("launch_gradient: FP32 needs the FP32 state and face records");
+        if (m.tile_halo_off && out.c_begin % kTile == 0 &&
+            (out.c_end % kTile == 0 || out.c_end == m.N))
+        {
+            const size_t lds = max((size_t)10 * m.tile_rows * sizeof(float),
+                                   (size_t)12 * kTile * sizeof(double));
+            const int tiles = (out.c_end - out.c_begin + kTile - 1) / kTile;
+            if (m.maxf <= 4)
+                hipLaunchKernelGGL((fv_gradient_tile_kernel<4, float>), dim3(tiles), dim3(kTile),
+                                   lds, s, m, P, U, gphi, out, linear, sc);
+            else
+                hipLaunchKernelGGL((fv_gradient_tile_kernel<6, float>), dim3(tiles), dim3(kTile),
+                                   lds, s, m, P, U, gphi, out, linear, sc);
+            return;
+        }
         if (m.maxf <= 4)
             hipLaunchKernelGGL((fv_gradient_kernel<4, float, false>), dim3(grid), dim3(kBlock),
                                FVH_K1_LDS, s, m, P, U, gphi, out, linear, sc);
@@ -1406,11 +1459,11 @@ void launch_gradient(hipStream_t s, const DevMesh &m, const PhysDev &P,
         const size_t lds = (size_t)max(10 * m.tile_rows, 16 * kTile) * sizeof(double);
         const int tiles = (out.c_end - out.c_begin + kTile - 1) / kTile;
         if (m.maxf <= 4)
-            hipLaunchKernelGGL((fv_gradient_tile_kernel<4>), dim3(tiles), dim3(kTile), lds, s,
-                               m, P, U, gphi, out, linear, sc);
+            hipLaunchKernelGGL((fv_gradient_tile_kernel<4, double>), dim3(tiles), dim3(kTile),
+                               lds, s, m, P, U, gphi, out, linear, sc);
         else
-            hipLaunchKernelGGL((fv_gradient_tile_kernel<6>), dim3(tiles), dim3(kTile), lds, s,
-                               m, P, U, gphi, out, linear, sc);
+            hipLaunchKernelGGL((fv_gradient_tile_kernel<6, double>), dim3(tiles), dim3(kTile),
+                               lds, s, m, P, U, gphi, out, linear, sc);
         return;
     }
     if (m.maxf <= 4)

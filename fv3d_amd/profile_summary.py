@@ -24,7 +24,7 @@ import sys
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GIB = float(1 << 30)
 # the stage's two kernels (the residual kernel: face-centred, or lane per slot)
-KERNELS = {"gradient": ["fv_gradient_tile_kernel"],
+KERNELS = {"gradient": ["fv_gradient_tile_kernel<4, double>"],
            "residual": ["fv_residual_face_kernel<0, true, true>",
                         "fv_residual_group_kernel<0, true, true, 4, double>"]}
 
