@@ -282,12 +282,14 @@ __device__ __forceinline__ Real limited(Real phi, Real gx, Real gy, Real gz, con
     {
         // y >= 1 exactly when the numerator reaches the slope (division is
         // monotonic), and then min(1, y) = 1: the divide is only needed
-        // below that
+        // below that.  The numerator is chosen by the slope's sign first,
+        // so that a wave whose lanes differ in sign runs one divide, not two
+        const bool pos = slope > Real(0);
+        const Real num = pos ? mx - phi : mn - phi;
+        const bool below = pos ? num < slope : (slope < Real(0) && num > slope);
         Real y = Real(1.0);
-        if (slope > Real(0) && mx - phi < slope)
-            y = (mx - phi) / slope;
-        else if (slope < Real(0) && mn - phi > slope)
-            y = (mn - phi) / slope;
+        if (below)
+            y = num / slope;
         lim = rmin(Real(1.0), y);
     }
     return phi + lim * slope;
