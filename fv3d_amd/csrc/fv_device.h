@@ -15,9 +15,13 @@
 //                       slot s; row N*maxf + b: the ghost state of boundary
 //                       b); nsl[N*maxf] gives the slot of the same face in
 //                       the neighbour's list
-//   face records      : fgA[F][10] (n, t1, t2, area: flux kernel),
+//   face records      : fgA[F][10] (n, t1, t2, area: flux kernels),
 //                       fgB[F][6] (n, area, lambda, pad: gradient
-//                       kernel), fsh[F] (index into shtab[][3],
+//                       kernels), fgN[F][4] (n, area) with flam[F]
+//                       (lambda) for the tile gradient kernel, and fgN
+//                       alone for the face-centred residual kernel when
+//                       every face's stored t1 and t2 follow from n by the
+//                       builders' formula; fsh[F] (index into shtab[][3],
 //                       the exact periodic shifts; 0 = none), fcell[F][2]
 //                       (cells[1] >= N is the ghost row N + boundary)
 //   boundary records  : gphi[B][10] (ghost rows), bnd_region[B],
@@ -27,11 +31,37 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+
 namespace fvh {
 namespace dev {
 
 constexpr int kFgA = 10; // doubles per flux-kernel face record
 constexpr int kFgB = 6;  // doubles per gradient-kernel face record
+
+// A face's tangents from its unit normal n, as the mesh builders form them:
+// t1 = a / |a| with a = (-n_z, 0, n_x) when |n_x| > |n_y|, else
+// (0, -n_z, n_y); t2 = b / |b| with b = t1 x n; |a| = sqrt((a_x a_x +
+// a_y a_y) + a_z a_z), each component divided by it.  One definition for
+// the host, which checks a mesh's stored tangents against it, and for the
+// kernels that form the tangents instead of loading them.
+__host__ __device__ inline void tangents_from_normal(const double *n, double *t1, double *t2)
+{
+    const double ax = n[0] < 0 ? -n[0] : n[0], ay = n[1] < 0 ? -n[1] : n[1];
+    const bool xs = ax > ay;
+    const double x = xs ? -n[2] : 0.0, y = xs ? 0.0 : -n[2], z = xs ? n[0] : n[1];
+    const double l1 = sqrt(x * x + y * y + z * z);
+    t1[0] = x / l1;
+    t1[1] = y / l1;
+    t1[2] = z / l1;
+    const double bx = t1[1] * n[2] - t1[2] * n[1];
+    const double by = t1[2] * n[0] - t1[0] * n[2];
+    const double bz = t1[0] * n[1] - t1[1] * n[0];
+    const double l2 = sqrt(bx * bx + by * by + bz * bz);
+    t2[0] = bx / l2;
+    t2[1] = by / l2;
+    t2[2] = bz / l2;
+}
 
 struct DevMesh
 {
@@ -52,6 +82,13 @@ struct DevMesh
     const double *__restrict__ crv;
     const double *__restrict__ fgA; // [F*kFgA]
     const double *__restrict__ fgB; // [F*kFgB]
+    // [F][4] n, area and [F] lambda (null: fgA / fgB only)
+    const double *__restrict__ fgN;
+    const double *__restrict__ flam;
+    // 1 when t1 and t2 of every face are what tangents_from_normal forms
+    // from n (checked bit for bit at set-up): the face-centred residual
+    // kernel then reads fgN instead of fgA
+    int tangents_derived;
     const int *__restrict__ fsh;    // [F]
     const double *__restrict__ shtab; // [nsh*3]
     const double *__restrict__ cx;  // [N*3]

@@ -477,6 +477,14 @@ __global__ __launch_bounds__(kTile, FVH_K1T_MINW) void fv_gradient_tile_kernel(
             gn[s][0] = q0.x, gn[s][1] = q0.y, gn[s][2] = q0.z;
             gn[s][3] = q0.w, gn[s][4] = q1.x;
         }
+        else if (m.fgN)
+        {
+            // n and area (32 B) and lambda (8 B) instead of the 48-B record
+            const double2 *g = reinterpret_cast<const double2 *>(m.fgN + (size_t)fid[s] * 4);
+            const double2 g0 = g[0], g1 = g[1];
+            gn[s][0] = g0.x, gn[s][1] = g0.y, gn[s][2] = g1.x;
+            gn[s][3] = g1.y, gn[s][4] = m.flam[fid[s]];
+        }
         else
         {
             const double2 *g = reinterpret_cast<const double2 *>(m.fgB + (size_t)fid[s] * kFgB);
@@ -1548,7 +1556,17 @@ __global__ __launch_bounds__(kTile, FVH_K2F_MINW) void fv_residual_face_kernel(D
         const int f = r.x;
         const bool ghost = r.z >= N * 4;
         Frame fr;
-        load_frame(m, f, fr);
+        if (m.tangents_derived)
+        {
+            // 32 B per face instead of 80: the tangents follow from n
+            const double2 *g = reinterpret_cast<const double2 *>(m.fgN + (size_t)f * 4);
+            const double2 a = g[0], b = g[1];
+            fr.n[0] = a.x, fr.n[1] = a.y, fr.n[2] = b.x;
+            fr.area = b.y;
+            tangents_from_normal(fr.n, fr.t1, fr.t2);
+        }
+        else
+            load_frame(m, f, fr);
         double L[10], R[10];
         if (FS)
         {

@@ -1936,6 +1936,40 @@ void init_finite_volume(void)
             }
             m.fgA = dupload(a.data(), a.size());
             m.fgB = dupload(b.data(), b.size());
+            // 32-B records (n, area) and the interpolation weights apart:
+            // the tile gradient kernel reads 40 B per face slot instead of
+            // 48, and the face-centred kernel 32 B per face instead of 80
+            // when every stored t1 and t2 is bit for bit what it forms from
+            // n (the generated and Gmsh meshes; a file may hold other
+            // tangents).  FVH_FULL_FRAMES=1 keeps the face-centred kernel
+            // on fgA, 2 both kernels on fgA / fgB.
+            const char *full_env = std::getenv("FVH_FULL_FRAMES");
+            const int full = full_env ? std::atoi(full_env) : 0;
+            bool derivable = full == 0;
+            for (int f = 0; f < F && derivable; ++f)
+            {
+                double t1[3], t2[3];
+                dev::tangents_from_normal(&faces->n[3 * f], t1, t2);
+                derivable = std::memcmp(t1, &faces->t1[3 * f], sizeof t1) == 0 &&
+                            std::memcmp(t2, &faces->t2[3 * f], sizeof t2) == 0;
+            }
+            m.tangents_derived = derivable ? 1 : 0;
+            m.fgN = nullptr;
+            m.flam = nullptr;
+            if (full < 2)
+            {
+                std::vector<double> nrec((size_t)F * 4);
+                for (int f = 0; f < F; ++f)
+                {
+                    for (int d = 0; d < 3; ++d)
+                        nrec[(size_t)f * 4 + d] = faces->n[3 * f + d];
+                    nrec[(size_t)f * 4 + 3] = faces->area[f];
+                }
+                m.fgN = dupload(nrec.data(), nrec.size());
+                m.flam = dupload(faces->lambda, (size_t)F);
+            }
+            else
+                m.tangents_derived = 0;
             m.fsh = dupload(fsh.data(), fsh.size());
             m.shtab = dupload(shtab.data(), shtab.size());
         }

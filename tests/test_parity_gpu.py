@@ -105,6 +105,28 @@ def test_taylor_green_steps(flux, mu, limiter):
     assert np.array_equal(log_g[:, 2], log_o[:, 2])
 
 
+def test_face_kernel_full_and_compact_frame_records(monkeypatch):
+    """The face-centred kernel forms t1 and t2 from n when every face's
+    stored tangents match that formula (the generated meshes), and loads
+    them otherwise; FVH_FULL_FRAMES=1 forces the loads.  Both give the
+    oracle's states after 5 RK3-3 steps, and the same bits."""
+    states = {}
+    for full in ("0", "1"):
+        monkeypatch.setenv("FVH_FULL_FRAMES", full)
+        cfg = configs.taylor_green(n=6, steps=5, tile=2)
+        case, o, _ = make_pair(cfg)
+        rc, _ = case.run()
+        n, _ = o.timedisc("RK3-3", 5, True, 1e-10, 0.0, 1e300)
+        assert rc == 0 and n == 5
+        states[full] = case.state()
+        e = rel_err(states[full], o.state())
+        _print("tgv U(5)", full_frames=full, rel=e,
+               bitdiff=n_bit_diff(states[full], o.state()))
+        assert e <= TOL
+        case.close()
+    assert n_bit_diff(states["0"], states["1"]) == 0
+
+
 @pytest.mark.parametrize("tets", [True, False])
 def test_all_boundary_types(tets):
     cfg = configs.all_boundaries_box(tets=tets, steps=10)
