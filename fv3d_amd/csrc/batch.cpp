@@ -285,6 +285,7 @@ void enqueue_step_persistent(const RuntimeView &v, const StageBuffers &sb,
         sp.rk_a = sb.rk_a[st];
         sp.rk_g = sb.rk_g[st];
         sp.stage0 = st == 0;
+        sp.k_dead = last; // the next step starts k from R
         sp.do_update = 1;
         sp.do_reduce = last;
         sp.partials = g_b.tile_partials;
@@ -346,6 +347,7 @@ void enqueue_step_face(const RuntimeView &v, const StageBuffers &sb,
             sp.rk_a = sb.rk_a[st];
             sp.rk_g = sb.rk_g[st];
             sp.stage0 = st == 0;
+            sp.k_dead = last; // the next step starts k from R
             sp.do_update = 1;
             sp.do_reduce = last;
             sp.partials = g_b.f_partials;
@@ -406,6 +408,7 @@ void enqueue_step(const RuntimeView &v, const StageBuffers &sb)
             sp.rk_a = sb.rk_a[st];
             sp.rk_g = sb.rk_g[st];
             sp.stage0 = st == 0;
+            sp.k_dead = last; // the next step starts k from R
             sp.do_update = 1;
             sp.c_begin = g.c_begin;
             sp.c_end = g.c_end;

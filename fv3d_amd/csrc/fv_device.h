@@ -185,6 +185,7 @@ struct StageParams
     int stage0;            // first stage: k = R
     int do_update;         // RK stage update into U_out / k
     int do_reduce;         // write residual and dt partials (last stage)
+    int k_dead;            // k is not read after this stage: not stored
     DevScalars *sc;        // dt, done flag
     double *partials;      // [partials_cap * 8]
     int partials_cap;      // rows available in partials

@@ -865,7 +865,8 @@ __device__ __forceinline__ void residual_group_body(const DevMesh &m, const Phys
                     sp.R_out[(size_t)c * 5 + j] = pick(R5, j);
                 if (sp.do_update)
                 {
-                    sp.k[(size_t)c * 5 + j] = pick(kk, j);
+                    if (!sp.k_dead)
+                        sp.k[(size_t)c * 5 + j] = pick(kk, j);
                     sp.U_out[(size_t)c * 5 + j] = pick(unew, j);
                 }
             }
@@ -1661,7 +1662,8 @@ __global__ __launch_bounds__(kTile, FVH_K2F_MINW) void fv_residual_face_kernel(D
                 kj = Rj + (FVH_K2F_PRE ? k_in[i] : sp.k[g]) * sp.rk_a;
             const double un = (FVH_K2F_PRE == 1 ? u_in[i] : sp.U_in[g]) + kj * dt * sp.rk_g;
             u_el[i] = un;
-            sp.k[g] = kj;
+            if (!sp.k_dead)
+                sp.k[g] = kj;
             sp.U_out[g] = un;
         }
     }

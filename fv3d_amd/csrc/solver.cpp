@@ -627,6 +627,9 @@ void enqueue_residual(const double *U_in, double *U_out, double *R_out,
     sp.stage0 = stage == 0;
     sp.do_update = update;
     sp.do_reduce = reduce;
+    // the last stage of an explicit step: the next step's first stage
+    // starts k from R, so this stage's k is never read
+    sp.k_dead = update && reduce && stage == g_rk_stages - 1;
     sp.sc = sc;
     sp.partials = g_dev.partials;
     sp.partials_cap = g_dev.partials_cap;
@@ -1120,10 +1123,6 @@ void enqueue_explicit_step(dev::DevScalars *sc, KernelTimer *tm = nullptr, doubl
         g_hook_time = t + g_rk_b[s] * dt;
         double *Uin = g_dev.U[g_dev.cur];
         double *Uout = g_dev.U[1 - g_dev.cur];
-        // the primitive rows are not passed here: the gradient kernel is
-        // bound by its gathered bytes, and reading 40 more per neighbour
-        // costs more than the divides it saves (8.57 against 8.06 ms per
-        // step on the 96^3 Taylor-Green mesh)
         enqueue_residual(Uin, Uout, last ? g_dev.R : nullptr, s, true, last,
                          sc, tm);
         g_dev.cur = 1 - g_dev.cur;
