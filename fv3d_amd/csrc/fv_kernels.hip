@@ -449,7 +449,8 @@ __global__ __launch_bounds__(kTile, FVH_K1T_MINW) void fv_gradient_tile_kernel(
         mn[j] = pc[j];
         mx[j] = pc[j];
     }
-    const int nf = valid ? m.ncf[c] : 0;
+    // every cell of a mesh whose cells have at most 4 faces is a tetrahedron
+    const int nf = valid ? (MAXF == 4 ? 4 : m.ncf[c]) : 0;
     const int2 *cfn = m.cfn + (size_t)cc * m.maxf;
     const unsigned short *nbl = m.nbl + (size_t)cc * m.maxf;
     int fid[MAXF], loc[MAXF];
@@ -1643,12 +1644,12 @@ __global__ __launch_bounds__(kTile, FVH_K2F_MINW) void fv_residual_face_kernel(D
         if (lc >= ncell)
             continue;
         const int c = base + lc;
-        const int nf = m.ncf[c];
+        // tetrahedra (the face tables exist for tetrahedral meshes only):
+        // four slots per cell
         double a = 0.0;
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-            if (q < nf)
-                a = a + ct[(lc * 4 + q) * 5 + j];
+            a = a + ct[(lc * 4 + q) * 5 + j];
         const double Rj = -a / m.vol[c];
         const size_t g = (size_t)base * 5 + p;
         r_el[i] = Rj;
