@@ -1512,6 +1512,9 @@ void launch_gradient(hipStream_t s, const DevMesh &m, const PhysDev &P,
 #ifndef FVH_K2F_PRE
 #define FVH_K2F_PRE 2
 #endif
+#ifndef FVH_K2F_LATE
+#define FVH_K2F_LATE 1
+#endif
 template <int FLUX, bool FS, bool VISC>
 __global__ __launch_bounds__(kTile, FVH_K2F_MINW) void fv_residual_face_kernel(DevMesh m,
                                                                               PhysDev P,
@@ -1629,10 +1632,40 @@ __global__ __launch_bounds__(kTile, FVH_K2F_MINW) void fv_residual_face_kernel(D
                 ct[dB * 5 + j] = -v;
         }
     }
+    // the update's remaining operands (and the last stage's per-cell ones)
+    // are loaded here, after the face work, so that their latency overlaps
+    // the barrier's wait (FVH_K2F_LATE 0: loaded where they are used, each
+    // wait after the barrier)
+    double vol_el[kE], uin_el[kE];
+    double lastU[5], lastV = 1.0, lastD[3] = {0.0, 0.0, 0.0};
+    const double dt_all = (sp.do_update && !sp.case_dt) ? sp.sc->dt : 0.0;
+    if (FVH_K2F_LATE)
+    {
+#pragma unroll
+        for (int i = 0; i < kE; ++i)
+        {
+            const int p = (int)threadIdx.x + i * kTile;
+            const int lc = p < ncell * 5 ? p / 5 : 0;
+            vol_el[i] = m.vol[base + lc];
+            uin_el[i] = (sp.do_update && FVH_K2F_PRE != 1)
+                            ? sp.U_in[(size_t)base * 5 + (p < ncell * 5 ? p : 0)]
+                            : 0.0;
+        }
+        if (sp.do_reduce && sp.do_update)
+        {
+            const int c = base + ((int)threadIdx.x < ncell ? (int)threadIdx.x : 0);
+#pragma unroll
+            for (int j = 0; j < 5; ++j)
+                lastU[j] = sp.U_in[(size_t)c * 5 + j];
+            lastV = m.vol[c];
+            lastD[0] = m.cdx[(size_t)c * 3];
+            lastD[1] = m.cdx[(size_t)c * 3 + 1];
+            lastD[2] = m.cdx[(size_t)c * 3 + 2];
+        }
+    }
     __syncthreads();
 
     // per (cell, variable)
-    const double dt_all = (sp.do_update && !sp.case_dt) ? sp.sc->dt : 0.0;
     double r_el[kE], u_el[kE];
 #pragma unroll
     for (int i = 0; i < kE; ++i)
@@ -1650,7 +1683,7 @@ __global__ __launch_bounds__(kTile, FVH_K2F_MINW) void fv_residual_face_kernel(D
 #pragma unroll
         for (int q = 0; q < 4; ++q)
             a = a + ct[(lc * 4 + q) * 5 + j];
-        const double Rj = -a / m.vol[c];
+        const double Rj = -a / (FVH_K2F_LATE ? vol_el[i] : m.vol[c]);
         const size_t g = (size_t)base * 5 + p;
         r_el[i] = Rj;
         if (sp.R_out)
@@ -1661,7 +1694,8 @@ __global__ __launch_bounds__(kTile, FVH_K2F_MINW) void fv_residual_face_kernel(D
             double kj = Rj;
             if (!sp.stage0)
                 kj = Rj + (FVH_K2F_PRE ? k_in[i] : sp.k[g]) * sp.rk_a;
-            const double un = (FVH_K2F_PRE == 1 ? u_in[i] : sp.U_in[g]) + kj * dt * sp.rk_g;
+            const double uin = FVH_K2F_PRE == 1 ? u_in[i] : FVH_K2F_LATE ? uin_el[i] : sp.U_in[g];
+            const double un = uin + kj * dt * sp.rk_g;
             u_el[i] = un;
             if (!sp.k_dead)
                 sp.k[g] = kj;
@@ -1704,17 +1738,18 @@ __global__ __launch_bounds__(kTile, FVH_K2F_MINW) void fv_residual_face_kernel(D
         {
             unew[j] = ct[lc * 5 + j];
             R5[j] = ct[kTile * 5 + lc * 5 + j];
-            pc[j] = sp.U_in[(size_t)c * 5 + j];
+            pc[j] = FVH_K2F_LATE ? lastU[j] : sp.U_in[(size_t)c * 5 + j];
         }
         // time step with the new density and the primitives of this
         // stage's input (navier_stokes.c:171-187)
         con_to_prim(P, pc);
-        const double vol = m.vol[c];
+        const double vol = FVH_K2F_LATE ? lastV : m.vol[c];
 #pragma unroll
         for (int j = 0; j < 5; ++j)
             part[j] = rabs(R5[j]) * vol;
-        const double dx0 = m.cdx[(size_t)c * 3], dx1 = m.cdx[(size_t)c * 3 + 1],
-                     dx2 = m.cdx[(size_t)c * 3 + 2];
+        const double dx0 = FVH_K2F_LATE ? lastD[0] : m.cdx[(size_t)c * 3],
+                     dx1 = FVH_K2F_LATE ? lastD[1] : m.cdx[(size_t)c * 3 + 1],
+                     dx2 = FVH_K2F_LATE ? lastD[2] : m.cdx[(size_t)c * 3 + 2];
         const double s_rho = 1.0 / unew[0];
         const double cs = sqrt(P.kappa * pc[8] * s_rho);
         const double ds1 = (rabs(pc[5]) + cs) * dx0 + (rabs(pc[6]) + cs) * dx1 +
