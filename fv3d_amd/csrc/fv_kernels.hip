@@ -1508,7 +1508,9 @@ void launch_gradient(hipStream_t s, const DevMesh &m, const PhysDev &P,
 #define FVH_K2F_MINW 3
 #endif
 // 1: the update's operands are loaded before the face work (20 more VGPRs);
-// 2: only the RK register
+// 2: only the RK register; 0: none (with FVH_K2F_LATE the RK register is
+// then loaded with the others after the face work: 12 VGPRs fewer, the same
+// time on the 96^3 x 6 Taylor-Green mesh)
 #ifndef FVH_K2F_PRE
 #define FVH_K2F_PRE 2
 #endif
@@ -1636,7 +1638,7 @@ __global__ __launch_bounds__(kTile, FVH_K2F_MINW) void fv_residual_face_kernel(D
     // are loaded here, after the face work, so that their latency overlaps
     // the barrier's wait (FVH_K2F_LATE 0: loaded where they are used, each
     // wait after the barrier)
-    double vol_el[kE], uin_el[kE];
+    double vol_el[kE], uin_el[kE], k_late[kE];
     double lastU[5], lastV = 1.0, lastD[3] = {0.0, 0.0, 0.0};
     const double dt_all = (sp.do_update && !sp.case_dt) ? sp.sc->dt : 0.0;
     if (FVH_K2F_LATE)
@@ -1646,10 +1648,10 @@ __global__ __launch_bounds__(kTile, FVH_K2F_MINW) void fv_residual_face_kernel(D
         {
             const int p = (int)threadIdx.x + i * kTile;
             const int lc = p < ncell * 5 ? p / 5 : 0;
+            const size_t g = (size_t)base * 5 + (p < ncell * 5 ? p : 0);
             vol_el[i] = m.vol[base + lc];
-            uin_el[i] = (sp.do_update && FVH_K2F_PRE != 1)
-                            ? sp.U_in[(size_t)base * 5 + (p < ncell * 5 ? p : 0)]
-                            : 0.0;
+            uin_el[i] = (sp.do_update && FVH_K2F_PRE != 1) ? sp.U_in[g] : 0.0;
+            k_late[i] = (sp.do_update && !sp.stage0 && FVH_K2F_PRE == 0) ? sp.k[g] : 0.0;
         }
         if (sp.do_reduce && sp.do_update)
         {
@@ -1693,7 +1695,7 @@ __global__ __launch_bounds__(kTile, FVH_K2F_MINW) void fv_residual_face_kernel(D
             const double dt = sp.case_dt ? sp.case_dt[sp.cell_case[c]] : dt_all;
             double kj = Rj;
             if (!sp.stage0)
-                kj = Rj + (FVH_K2F_PRE ? k_in[i] : sp.k[g]) * sp.rk_a;
+                kj = Rj + (FVH_K2F_PRE ? k_in[i] : FVH_K2F_LATE ? k_late[i] : sp.k[g]) * sp.rk_a;
             const double uin = FVH_K2F_PRE == 1 ? u_in[i] : FVH_K2F_LATE ? uin_el[i] : sp.U_in[g];
             const double un = uin + kj * dt * sp.rk_g;
             u_el[i] = un;
