@@ -135,6 +135,49 @@ def test_channel_mesh():
     assert not np.array_equal(S.mesh_arrays()["cell_volume"], v1)
 
 
+def _tangents_from_normal(n):
+    """t1, t2 from unit normals n [F,3]: t1 = a/|a| with a = (-n_z, 0, n_x)
+    where |n_x| > |n_y|, else (0, -n_z, n_y); t2 = b/|b| with b = t1 x n;
+    |a| = sqrt((a_x a_x + a_y a_y) + a_z a_z) -- the order of operations the
+    face-centred kernel uses to form them from the 32-B face records."""
+    xs = np.abs(n[:, 0]) > np.abs(n[:, 1])
+    a = np.stack([np.where(xs, -n[:, 2], 0.0), np.where(xs, 0.0, -n[:, 2]),
+                  np.where(xs, n[:, 0], n[:, 1])], axis=1)
+    l1 = np.sqrt(a[:, 0] * a[:, 0] + a[:, 1] * a[:, 1] + a[:, 2] * a[:, 2])
+    t1 = a / l1[:, None]
+    b = np.stack([t1[:, 1] * n[:, 2] - t1[:, 2] * n[:, 1],
+                  t1[:, 2] * n[:, 0] - t1[:, 0] * n[:, 2],
+                  t1[:, 0] * n[:, 1] - t1[:, 1] * n[:, 0]], axis=1)
+    l2 = np.sqrt(b[:, 0] * b[:, 0] + b[:, 1] * b[:, 1] + b[:, 2] * b[:, 2])
+    return t1, b / l2[:, None]
+
+
+@pytest.mark.parametrize("kind", ["box-tets", "box-hexes", "channel"])
+def test_stored_tangents_follow_the_normal(kind):
+    """The generated meshes store t1 and t2 bit for bit as formed from the
+    normal, so the face-centred kernel reads 32-B records (n, area) and
+    forms the tangents instead of loading 80-B frames.  On the jittered
+    channel a changed summation order in |b| changes t2 on 461 of 3,780
+    faces, and a multiply by the reciprocal changes most of t1 and t2; the
+    boxes' regular normals exercise both branches of the formula."""
+    if kind == "channel":
+        L = lib()
+        L.init_solver(b"mesh-test")
+        S.check(L.fvh_mesh_channel(16, 6, 3, 0.05, 7))
+        m = S.mesh_arrays()
+    else:
+        m = _box((5, 4, 3), kind == "box-tets", 7, length=(1.0, 0.8, 0.6), tile=2)
+    F = m["n_faces"]
+    n = m["face_n"].reshape(F, 3)
+    t1, t2 = _tangents_from_normal(n)
+    for got, want in ((t1, m["face_t1"]), (t2, m["face_t2"])):
+        want = np.ascontiguousarray(want.reshape(F, 3))
+        assert np.array_equal(np.ascontiguousarray(got).view(np.int64), want.view(np.int64))
+    # both branches of the formula occur
+    xs = np.abs(n[:, 0]) > np.abs(n[:, 1])
+    assert xs.any() and (~xs).any()
+
+
 def test_mesh_file_round_trip(tmp_path):
     L = lib()
     L.init_solver(b"mesh-test")
