@@ -455,17 +455,30 @@ __global__ __launch_bounds__(kTile, FVH_K1T_MINW) void fv_gradient_tile_kernel(
     const unsigned short *nbl = m.nbl + (size_t)cc * m.maxf;
     int fid[MAXF], loc[MAXF];
     bool own[MAXF];
-#pragma unroll
-    for (int s = 0; s < MAXF; ++s)
+    if constexpr (MAXF == 4)
     {
-        // slots past the mesh's face count per cell (MAXF = 6 with 5-face
-        // cells) take slot 0, like the slots past a cell's own count
-        const int sl = s < m.maxf ? s : 0;
-        const int2 e = cfn[sl];
-        fid[s] = e.x;
-        own[s] = e.y >= 0;
-        loc[s] = nbl[sl];
+        // tetrahedra: the cell's four slots in two 16-B and one 8-B load
+        // (32 B of cfn and 8 B of nbl per cell, aligned)
+        const int4 *c4 = reinterpret_cast<const int4 *>(cfn);
+        const int4 a = c4[0], b = c4[1];
+        const ushort4 l = *reinterpret_cast<const ushort4 *>(nbl);
+        fid[0] = a.x, own[0] = a.y >= 0, fid[1] = a.z, own[1] = a.w >= 0;
+        fid[2] = b.x, own[2] = b.y >= 0, fid[3] = b.z, own[3] = b.w >= 0;
+        loc[0] = l.x, loc[1] = l.y, loc[2] = l.z, loc[3] = l.w;
     }
+    else
+#pragma unroll
+        for (int s = 0; s < MAXF; ++s)
+        {
+            // slots past the mesh's face count per cell (MAXF = 6 with
+            // 5-face cells) take slot 0, like the slots past a cell's own
+            // count
+            const int sl = s < m.maxf ? s : 0;
+            const int2 e = cfn[sl];
+            fid[s] = e.x;
+            own[s] = e.y >= 0;
+            loc[s] = nbl[sl];
+        }
     // each face's record in its own iteration (FVH_K1T_PREREC 0: 158 VGPRs,
     // 3 waves per SIMD); the records of all faces first (one round of
     // gathers) need 178, 2 waves, and ran 17% slower on tetrahedra
@@ -553,7 +566,17 @@ __global__ __launch_bounds__(kTile, FVH_K1T_MINW) void fv_gradient_tile_kernel(
     }
     Real rv[MAXF][3];
     const bool fs_out = f32 ? out.fs32 != nullptr : out.fs != nullptr;
-    if (linear && fs_out)
+    if (linear && fs_out && MAXF == 4 && !f32)
+    {
+        // tetrahedra: the cell's 96 B of distance vectors in six 16-B loads
+        const double2 *r2 = reinterpret_cast<const double2 *>(m.crv + (size_t)cc * 12);
+        const double2 q0 = r2[0], q1 = r2[1], q2 = r2[2], q3 = r2[3], q4 = r2[4], q5 = r2[5];
+        rv[0][0] = q0.x, rv[0][1] = q0.y, rv[0][2] = q1.x;
+        rv[1][0] = q1.y, rv[1][1] = q2.x, rv[1][2] = q2.y;
+        rv[2][0] = q3.x, rv[2][1] = q3.y, rv[2][2] = q4.x;
+        rv[MAXF - 1][0] = q4.y, rv[MAXF - 1][1] = q5.x, rv[MAXF - 1][2] = q5.y;
+    }
+    else if (linear && fs_out)
     {
 #pragma unroll
         for (int s = 0; s < MAXF; ++s)
