@@ -78,7 +78,7 @@ struct BatchState
     int f_tiles = 0;
     int *f_tile_c0 = nullptr, *f_face_off = nullptr, *f_row_lo = nullptr,
         *f_row_hi = nullptr;
-    int4 *f_faces = nullptr;
+    unsigned long long *f_faces = nullptr; // null: too large for 8-byte entries
     double *f_partials = nullptr;
     std::vector<int> f_case_t0; // a case's tiles f_case_t0[k] .. f_case_t0[k+1] - 1
 };
@@ -605,7 +605,6 @@ int fvh_batch_setup(int n_cases, const int *flux, const int *viscous, const doub
             build_tile_faces(c0, off, ent);
             g_b.f_tile_c0 = balloc<int>(c0.size());
             g_b.f_face_off = balloc<int>(off.size());
-            g_b.f_faces = balloc<int4>(ent.size());
             g_b.f_row_lo = balloc<int>(n_cases);
             g_b.f_row_hi = balloc<int>(n_cases);
             g_b.f_partials = balloc<double>((size_t)g_b.f_tiles * rows * dev::kPartialStride);
@@ -613,8 +612,14 @@ int fvh_batch_setup(int n_cases, const int *flux, const int *viscous, const doub
                                     hipMemcpyHostToDevice));
             FVH_HIP_CHECK(hipMemcpy(g_b.f_face_off, off.data(), off.size() * sizeof(int),
                                     hipMemcpyHostToDevice));
-            FVH_HIP_CHECK(hipMemcpy(g_b.f_faces, ent.data(), ent.size() * sizeof(int4),
-                                    hipMemcpyHostToDevice));
+            std::vector<unsigned long long> packed;
+            if (pack_tile_faces_table(c0, off, ent, packed))
+            {
+                g_b.f_faces = balloc<unsigned long long>(packed.size());
+                FVH_HIP_CHECK(hipMemcpy(g_b.f_faces, packed.data(),
+                                        packed.size() * sizeof(unsigned long long),
+                                        hipMemcpyHostToDevice));
+            }
             FVH_HIP_CHECK(hipMemcpy(g_b.f_row_lo, rlo.data(), n_cases * sizeof(int),
                                     hipMemcpyHostToDevice));
             FVH_HIP_CHECK(hipMemcpy(g_b.f_row_hi, rhi.data(), n_cases * sizeof(int),

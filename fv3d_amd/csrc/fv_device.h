@@ -117,15 +117,17 @@ struct DevMesh
     const unsigned short *__restrict__ nbl;
     int tile_rows;
     // faces of each cell tile for the face-centred residual kernel (null: not
-    // built; tetrahedral meshes): tile t's faces are tile_faces[
-    // tile_face_off[t] .. tile_face_off[t+1]), each face once, {face, row of
-    // the left state, row of the right state (N*4 + b: ghost b), slots}; the
-    // rows index the face-state table ([N*4 + B] rows of 8), the slots are
-    // the tile-local slot (cell - t*kTile)*4 + s that receives +flux*area
-    // (low 16 bits) and the one that receives -flux*area (high 16 bits),
-    // 0xffff where that side's cell is outside the tile
+    // built; tetrahedral meshes of fewer than 2^26 faces and 2^27 face-state
+    // rows): tile t's faces are tile_faces[tile_face_off[t] ..
+    // tile_face_off[t+1]), each face once, in 8 bytes: bits 0-25 the face,
+    // 26-34 a tile-local slot a = (cell - tile's first cell) * 4 + s; bit 63
+    // clear: both cells in the tile, bits 35-43 the right cell's slot b, the
+    // left state's row tile base * 4 + a and the right one's base * 4 + b;
+    // bit 63 set: one side in the tile (bit 35: 0 left, 1 right), its row
+    // base * 4 + a, the other side's row in bits 36-62 (N*4 + b: ghost b).
+    // The left slot receives +flux*area, the right one -flux*area.
     const int *__restrict__ tile_face_off;
-    const int4 *__restrict__ tile_faces;
+    const unsigned long long *__restrict__ tile_faces;
     // first cell of each face-table tile, [tiles + 1] (null: tile t is cells
     // [t*kTile, (t+1)*kTile)); set for tiles that follow other boundaries
     // (batched cases: no tile spans two cases), each at most kTile cells

@@ -1530,6 +1530,24 @@ void launch_gradient(hipStream_t s, const DevMesh &m, const PhysDev &P,
 #ifndef FVH_K2F_MINW
 #define FVH_K2F_MINW 3
 #endif
+// One entry of DevMesh::tile_faces as {face, left row, right row, left slot |
+// right slot << 16} (0xffff for a side outside the tile); base: the tile's
+// first cell.
+__device__ __forceinline__ int4 unpack_tile_face(unsigned long long q, int base)
+{
+    const int f = (int)(q & 0x3ffffffULL);
+    const int a = (int)((q >> 26) & 0x1ffULL);
+    const int b4 = base * 4;
+    if (q >> 63)
+    {
+        const bool right = (q >> 35) & 1ULL;
+        const int other = (int)((q >> 36) & 0x7ffffffULL);
+        return right ? make_int4(f, other, b4 + a, 0xffff | (a << 16))
+                     : make_int4(f, b4 + a, other, a | (0xffff << 16));
+    }
+    const int b = (int)((q >> 35) & 0x1ffULL);
+    return make_int4(f, b4 + a, b4 + b, a | (b << 16));
+}
 // 1: the update's operands are loaded before the face work (20 more VGPRs);
 // 2: only the RK register; 0: none (with FVH_K2F_LATE the RK register is
 // then loaded with the others after the face work: 12 VGPRs fewer, the same
@@ -1577,12 +1595,12 @@ __global__ __launch_bounds__(kTile, FVH_K2F_MINW) void fv_residual_face_kernel(D
         }
     // the next face's record is loaded while this face is computed
     int e = e0 + (int)threadIdx.x;
-    int4 r_next = m.tile_faces[e < e1 ? e : e0];
+    unsigned long long q_next = m.tile_faces[e < e1 ? e : e0];
     for (; e < e1; e += kTile)
     {
-        const int4 r = r_next;
+        const int4 r = unpack_tile_face(q_next, base);
         if (e + kTile < e1)
-            r_next = m.tile_faces[e + kTile];
+            q_next = m.tile_faces[e + kTile];
         const int f = r.x;
         const bool ghost = r.z >= N * 4;
         Frame fr;

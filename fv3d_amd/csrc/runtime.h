@@ -62,10 +62,15 @@ StageBuffers stage_buffers();
 
 // Face tables of the face-centred residual kernel for cell tiles
 // [c0[t], c0[t+1]) of at most dev::kTile cells (tetrahedral meshes):
-// per tile every face of its cells once (off[t] .. off[t+1] - 1 of ent), in
-// the DevMesh tile_faces layout.  Reads the host mesh.
+// per tile every face of its cells once (off[t] .. off[t+1] - 1 of ent) as
+// {face, left row, right row, left slot | right slot << 16}, 0xffff for a
+// side outside the tile.  Reads the host mesh.
 void build_tile_faces(const std::vector<int> &c0, std::vector<int> &off,
                       std::vector<int4> &ent);
+// The same table in the 8-byte DevMesh tile_faces layout, each entry checked
+// against its tile's slots; false when the mesh is too large for it.
+bool pack_tile_faces_table(const std::vector<int> &c0, const std::vector<int> &off,
+                           const std::vector<int4> &ent, std::vector<unsigned long long> &out);
 
 // Enqueues R = dU/dt(U) (finite_volume_time_derivative semantics, the
 // reference's ghost rows recomputed from U) on the runtime stream.  fp32:

@@ -774,6 +774,47 @@ void tile_faces_table(const std::vector<int> &c0, std::vector<int> &off,
 
 // The face tables over tiles of dev::kTile consecutive cells (the explicit
 // path's residual kernel); not built with FVH_K2_FACE=0.
+// The 8-byte entries of a face table (dev::DevMesh::tile_faces) from the
+// {face, left row, right row, slots} ones tile_faces_table builds, each
+// checked against its tile's slots; false when the mesh is too large for
+// their fields.
+bool pack_tile_faces(const std::vector<int> &c0, const std::vector<int> &off,
+                     const std::vector<int4> &ent, std::vector<unsigned long long> &out)
+{
+    const long long rows = (long long)g_dev.m.N * 4 + g_dev.m.B;
+    if (dev::kTile * 4 > 512 || g_dev.m.F >= (1 << 26) || rows >= (1LL << 27))
+        return false;
+    out.resize(ent.size());
+    for (size_t t = 0; t + 1 < c0.size(); ++t)
+    {
+        const long long b4 = (long long)c0[t] * 4;
+        for (int e = off[t]; e < off[t + 1]; ++e)
+        {
+            const int4 r = ent[e];
+            const int dA = r.w & 0xffff, dB = (r.w >> 16) & 0xffff;
+            unsigned long long q = (unsigned long long)r.x;
+            if (dA != 0xffff && dB != 0xffff)
+            {
+                if (r.y != b4 + dA || r.z != b4 + dB)
+                    fatal("tile faces: an entry's rows are not its tile slots");
+                q |= (unsigned long long)dA << 26 | (unsigned long long)dB << 35;
+            }
+            else
+            {
+                const bool right = dA == 0xffff;
+                const int a = right ? dB : dA;
+                const int in_row = right ? r.z : r.y, out_row = right ? r.y : r.z;
+                if (in_row != b4 + a || out_row < 0)
+                    fatal("tile faces: an entry's row is not its tile slot");
+                q |= (unsigned long long)a << 26 | (unsigned long long)(right ? 1 : 0) << 35 |
+                     (unsigned long long)out_row << 36 | 1ULL << 63;
+            }
+            out[e] = q;
+        }
+    }
+    return true;
+}
+
 void setup_tile_faces()
 {
     const char *e = std::getenv("FVH_K2_FACE");
@@ -789,8 +830,11 @@ void setup_tile_faces()
     std::vector<int> off;
     std::vector<int4> ent;
     tile_faces_table(c0, off, ent);
+    std::vector<unsigned long long> packed;
+    if (!pack_tile_faces(c0, off, ent, packed))
+        return; // too large for the 8-byte entries: the lane-per-slot kernel runs
     g_dev.m.tile_face_off = dupload(off.data(), off.size());
-    g_dev.m.tile_faces = dupload(ent.data(), ent.size());
+    g_dev.m.tile_faces = dupload(packed.data(), packed.size());
 }
 
 void sync_mirror(bool with_dt)
@@ -930,6 +974,12 @@ void build_tile_faces(const std::vector<int> &c0, std::vector<int> &off,
                       std::vector<int4> &ent)
 {
     tile_faces_table(c0, off, ent);
+}
+
+bool pack_tile_faces_table(const std::vector<int> &c0, const std::vector<int> &off,
+                           const std::vector<int4> &ent, std::vector<unsigned long long> &out)
+{
+    return pack_tile_faces(c0, off, ent, out);
 }
 
 void set_error(const std::string &msg) { g_error = msg; }
