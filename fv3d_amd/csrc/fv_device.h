@@ -73,6 +73,10 @@ struct DevMesh
     // face's first cell, else -(neighbour + 1); slots past the cell's face
     // count repeat slot 0 [N*maxf]
     const int2 *__restrict__ cfn;
+    // tetrahedral meshes: per slot the face id, with the sign bit set when
+    // the cell is the face's second cell (cfn's side), [N*4] (null: other
+    // meshes)
+    const int *__restrict__ cfo;
     // per cell face slot: the slot that the same face has in the neighbour's
     // list (0 on boundary faces and past the cell's face count) [N*maxf]
     const unsigned char *__restrict__ nsl;

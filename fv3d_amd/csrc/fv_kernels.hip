@@ -457,13 +457,24 @@ __global__ __launch_bounds__(kTile, FVH_K1T_MINW) void fv_gradient_tile_kernel(
     bool own[MAXF];
     if constexpr (MAXF == 4)
     {
-        // tetrahedra: the cell's four slots in two 16-B and one 8-B load
-        // (32 B of cfn and 8 B of nbl per cell, aligned)
-        const int4 *c4 = reinterpret_cast<const int4 *>(cfn);
-        const int4 a = c4[0], b = c4[1];
+        // tetrahedra: the cell's four slots in one 16-B and one 8-B load
+        // (the face ids with their side in the sign bit, and nbl; aligned)
         const ushort4 l = *reinterpret_cast<const ushort4 *>(nbl);
-        fid[0] = a.x, own[0] = a.y >= 0, fid[1] = a.z, own[1] = a.w >= 0;
-        fid[2] = b.x, own[2] = b.y >= 0, fid[3] = b.z, own[3] = b.w >= 0;
+        if (m.cfo)
+        {
+            const int4 a = *reinterpret_cast<const int4 *>(m.cfo + (size_t)cc * 4);
+            fid[0] = a.x & 0x7fffffff, own[0] = a.x >= 0;
+            fid[1] = a.y & 0x7fffffff, own[1] = a.y >= 0;
+            fid[2] = a.z & 0x7fffffff, own[2] = a.z >= 0;
+            fid[3] = a.w & 0x7fffffff, own[3] = a.w >= 0;
+        }
+        else
+        {
+            const int4 *c4 = reinterpret_cast<const int4 *>(cfn);
+            const int4 a = c4[0], b = c4[1];
+            fid[0] = a.x, own[0] = a.y >= 0, fid[1] = a.z, own[1] = a.w >= 0;
+            fid[2] = b.x, own[2] = b.y >= 0, fid[3] = b.z, own[3] = b.w >= 0;
+        }
         loc[0] = l.x, loc[1] = l.y, loc[2] = l.z, loc[3] = l.w;
     }
     else

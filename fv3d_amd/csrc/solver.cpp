@@ -1925,6 +1925,16 @@ void init_finite_volume(void)
                 }
             }
             m.cfn = dupload(cfn.data(), cfn.size());
+            // tetrahedra: the face ids with the side in the sign bit (the
+            // tile gradient kernel reads 16 B per cell instead of cfn's 32)
+            m.cfo = nullptr;
+            if (maxf == 4)
+            {
+                std::vector<int> cfo(cfn.size());
+                for (size_t i = 0; i < cfn.size(); ++i)
+                    cfo[i] = cfn[i].y >= 0 ? cfn[i].x : (int)((unsigned)cfn[i].x | 0x80000000u);
+                m.cfo = dupload(cfo.data(), cfo.size());
+            }
             // the neighbour's slot of each slot's face (a cell that is its
             // own periodic neighbour takes the face's other slot)
             std::vector<unsigned char> nsl((size_t)N * maxf, 0);
