@@ -160,7 +160,8 @@ __global__ __launch_bounds__(kBlock, FVH_K1_MINW) void fv_gradient_kernel(
     // three per face.  Slots s >= nf repeat slot 0 and are skipped in the
     // sums.
     constexpr int FB = MAXF % FVH_K1_FB == 0 ? FVH_K1_FB : (MAXF % 3 == 0 ? 3 : MAXF);
-    const int nf = m.ncf[c];
+    // MAXF = 4 runs on meshes of tetrahedra only
+    const int nf = MAXF == 4 ? 4 : m.ncf[c];
     const int2 *cfn = m.cfn + (size_t)c * m.maxf;
     int fid[MAXF];
     bool own[MAXF];
@@ -703,7 +704,9 @@ __device__ __forceinline__ void residual_group_body(const DevMesh &m, const Phys
                                                     double *partial_row)
 {
     const int N = m.N;
-    const int nf = cell_ok ? m.ncf[c] : 0;
+    // four lanes per cell only on meshes of tetrahedra: four faces, no load
+    // for the count before the face work
+    const int nf = cell_ok ? (LANES == 4 ? 4 : m.ncf[c]) : 0;
     // the epilogue's operands (the cell's state and RK register, element j in
     // lane j % LANES of the group, and its volume), loaded before the face
     // work so that their latency overlaps it
