@@ -418,7 +418,8 @@ __global__ __launch_bounds__(kTile, FVH_K1T_MINW) void fv_gradient_tile_kernel(
     Real *tl = reinterpret_cast<Real *>(smem);
     const int N = m.N;
     const int rows = m.tile_rows;
-    // tiles in launch order, as in the face-centred residual kernel
+    // tiles in runs of 8 * FVH_TILE_GROUP, as in the face-centred residual
+    // kernel
     const int t = out.c_begin / kTile + tile_of_block((int)blockIdx.x, (int)gridDim.x);
     const int base = t * kTile;
     const int h0 = m.tile_halo_off[t];
@@ -1600,10 +1601,12 @@ __global__ __launch_bounds__(kTile, FVH_K2F_MINW) void fv_residual_face_kernel(D
         return;
     extern __shared__ double ct[]; // [kTile * 4][5]
     const int N = m.N;
-    // tiles in launch order: the hardware hands consecutive workgroups to
-    // the 8 XCDs in turn, so all XCDs work on one compact region of the mesh
-    // at a time (5% faster than XCD-contiguous tile ranges on the 96^3 x 6
-    // Taylor-Green mesh; batched cases: each case done in a short window)
+    // tiles in runs: the hardware hands consecutive workgroups to the 8 XCDs
+    // in turn, so all XCDs work on one compact region of the mesh at a time
+    // (5% faster than XCD-contiguous tile ranges on the 96^3 x 6 Taylor-Green
+    // mesh), each XCD on FVH_TILE_GROUP adjacent tiles of a run
+    // (tile_of_block); batched cases' tiles in launch order, so that each
+    // case is done in a short window
     const int t = sp.tile_begin + (m.tile_c0 ? (int)blockIdx.x
                                              : tile_of_block((int)blockIdx.x, (int)gridDim.x));
     const int base = m.tile_c0 ? m.tile_c0[t] : t * kTile;
