@@ -1604,11 +1604,9 @@ __global__ __launch_bounds__(kTile, FVH_K2F_MINW) void fv_residual_face_kernel(D
     // tiles in runs: the hardware hands consecutive workgroups to the 8 XCDs
     // in turn, so all XCDs work on one compact region of the mesh at a time
     // (5% faster than XCD-contiguous tile ranges on the 96^3 x 6 Taylor-Green
-    // mesh), each XCD on FVH_TILE_GROUP adjacent tiles of a run
-    // (tile_of_block); batched cases' tiles in launch order, so that each
-    // case is done in a short window
-    const int t = sp.tile_begin + (m.tile_c0 ? (int)blockIdx.x
-                                             : tile_of_block((int)blockIdx.x, (int)gridDim.x));
+    // mesh; batched cases: each case done in a short window), each XCD on
+    // FVH_TILE_GROUP adjacent tiles of a run (tile_of_block)
+    const int t = sp.tile_begin + tile_of_block((int)blockIdx.x, (int)gridDim.x);
     const int base = m.tile_c0 ? m.tile_c0[t] : t * kTile;
     const int ncell = m.tile_c0 ? m.tile_c0[t + 1] - base : min(kTile, N - base);
     // per-tile timing (tile_c0 tiles only: every thread reaches the end)
