@@ -20,7 +20,10 @@ CASES = [dict(n_hex=4, mach=0.3, flux="Rusanov", viscous=False),
          dict(n_hex=3, mach=1.7, flux="AUSM", viscous=True),
          dict(n_hex=5, mach=0.8, flux="AUSM", viscous=False),
          dict(n_hex=4, mach=1.2, flux="Rusanov", viscous=True),
-         dict(n_hex=3, mach=0.5, flux="Rusanov", viscous=False)]
+         dict(n_hex=3, mach=0.5, flux="Rusanov", viscous=False),
+         # 81 tiles: the face schedule's workgroups take one full run of 64
+         # tiles (8 adjacent tiles per XCD) and a partial one
+         dict(n_hex=12, mach=0.6, flux="AUSM", viscous=False)]
 TILE, MU, STEPS = 2, 2e-3, 6
 
 

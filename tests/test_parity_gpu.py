@@ -105,6 +105,22 @@ def test_taylor_green_steps(flux, mu, limiter):
     assert np.array_equal(log_g[:, 2], log_o[:, 2])
 
 
+def test_taylor_green_full_runs_of_tiles():
+    """The tile kernels hand each XCD 8 adjacent tiles of every run of 64
+    workgroups (the last, partial run in launch order): 12^3 x 6 = 10,368
+    cells in 4^3-hexahedron blocks are 81 tiles, one full run and a partial
+    one.  5 RK3-3 steps (AUSM, viscous, Barth-Jespersen) against the oracle."""
+    cfg = configs.taylor_green(n=12, steps=5, tile=4)
+    case, o, _ = make_pair(cfg)
+    rc, _ = case.run()
+    n, _ = o.timedisc("RK3-3", 5, True, 1e-10, 0.0, 1e300)
+    assert rc == 0 and n == 5
+    e = rel_err(case.state(), o.state())
+    _print("tgv 12^3 U(5)", rel=e, bitdiff=n_bit_diff(case.state(), o.state()))
+    assert e <= TOL
+    case.close()
+
+
 def test_face_kernel_full_and_compact_frame_records(monkeypatch):
     """The face-centred kernel forms t1 and t2 from n when every face's
     stored tangents match that formula (the generated meshes), and loads
