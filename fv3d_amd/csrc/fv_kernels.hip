@@ -450,10 +450,12 @@ __global__ __launch_bounds__(kTile, FVH_K1T_MINW) void fv_gradient_tile_kernel(
             tl[j * rows + i] = q[j];
     }
     __syncthreads();
-    // cells past the launch range (the last tile) take part in the barriers
-    // of the output passes without computing
+    // cells outside the launch range (in the first and last tile of a range
+    // that does not start or end on a tile boundary) take part in the
+    // barriers of the output passes without computing; the output passes
+    // store the rows of cells lo .. ncell - 1 of the tile only
     const int c = base + threadIdx.x;
-    const bool valid = c < out.c_end;
+    const bool valid = c >= out.c_begin && c < out.c_end;
     const int cc = valid ? c : base;
 
     Real pc[10];
@@ -637,6 +639,7 @@ __global__ __launch_bounds__(kTile, FVH_K1T_MINW) void fv_gradient_tile_kernel(
     // states go out one pair of slots at a time (two 64-byte rows of a cell,
     // one 128-byte line).
     const int ncell = min(kTile, out.c_end - base);
+    const int lo = max(out.c_begin - base, 0);
     __syncthreads();
     if (out.vg)
     {
@@ -647,7 +650,7 @@ __global__ __launch_bounds__(kTile, FVH_K1T_MINW) void fv_gradient_tile_kernel(
         __syncthreads();
         const double2 *src = reinterpret_cast<const double2 *>(smem);
         double2 *dst = reinterpret_cast<double2 *>(out.vg + (size_t)base * 12);
-        for (int i = threadIdx.x; i < ncell * 6; i += kTile)
+        for (int i = lo * 6 + threadIdx.x; i < ncell * 6; i += kTile)
             dst[i] = src[i];
         __syncthreads();
     }
@@ -677,7 +680,7 @@ __global__ __launch_bounds__(kTile, FVH_K1T_MINW) void fv_gradient_tile_kernel(
             // 2 float4 per row, ns rows per cell
             const float4 *src = reinterpret_cast<const float4 *>(tl);
             const int per = 2 * ns;
-            for (int i = threadIdx.x; i < ncell * per; i += kTile)
+            for (int i = lo * per + threadIdx.x; i < ncell * per; i += kTile)
             {
                 const int cl = i / per, q = i - cl * per;
                 float4 *dst = reinterpret_cast<float4 *>(
@@ -690,7 +693,7 @@ __global__ __launch_bounds__(kTile, FVH_K1T_MINW) void fv_gradient_tile_kernel(
             // 4 double2 per row, ns rows per cell
             const double2 *src = reinterpret_cast<const double2 *>(tl);
             const int per = 4 * ns;
-            for (int i = threadIdx.x; i < ncell * per; i += kTile)
+            for (int i = lo * per + threadIdx.x; i < ncell * per; i += kTile)
             {
                 const int cl = i / per, q = i - cl * per;
                 double2 *dst = reinterpret_cast<double2 *>(
@@ -1466,6 +1469,13 @@ void launch_ghost(hipStream_t s, const DevMesh &m, const PhysDev &P,
                        s, m, P, U, gphi, fs, sc, fs32, U32);
 }
 
+// The tiles of the tile gradient kernel that cover the cells [c_begin,
+// c_end): from the tile of c_begin to the tile of c_end - 1.
+static int range_tiles(const GradOut &out)
+{
+    return (out.c_end + kTile - 1) / kTile - out.c_begin / kTile;
+}
+
 void launch_gradient(hipStream_t s, const DevMesh &m, const PhysDev &P,
                      const double *U, const double *gphi, const GradOut &out_in,
                      int linear, const DevScalars *sc, bool fp32)
@@ -1486,12 +1496,11 @@ void launch_gradient(hipStream_t s, const DevMesh &m, const PhysDev &P,
             throw std::runtime_error("launch_gradient: the test outputs are FP64 only");
         if (!out.U32 || !m.fgB32 || !m.crv32)
             throw std::runtime_error("launch_gradient: FP32 needs the FP32 state and face records");
-        if (m.tile_halo_off && out.c_begin % kTile == 0 &&
-            (out.c_end % kTile == 0 || out.c_end == m.N))
+        if (m.tile_halo_off)
         {
             const size_t lds = max((size_t)10 * m.tile_rows * sizeof(float),
                                    (size_t)12 * kTile * sizeof(double));
-            const int tiles = (out.c_end - out.c_begin + kTile - 1) / kTile;
+            const int tiles = range_tiles(out);
             if (m.maxf <= 4)
                 hipLaunchKernelGGL((fv_gradient_tile_kernel<4, float>), dim3(tiles), dim3(kTile),
                                    lds, s, m, P, U, gphi, out, linear, sc);
@@ -1520,11 +1529,10 @@ void launch_gradient(hipStream_t s, const DevMesh &m, const PhysDev &P,
                                sc);
         return;
     }
-    if (m.tile_halo_off && out.c_begin % kTile == 0 &&
-        (out.c_end % kTile == 0 || out.c_end == m.N))
+    if (m.tile_halo_off)
     {
         const size_t lds = (size_t)max(10 * m.tile_rows, 16 * kTile) * sizeof(double);
-        const int tiles = (out.c_end - out.c_begin + kTile - 1) / kTile;
+        const int tiles = range_tiles(out);
         if (m.maxf <= 4)
             hipLaunchKernelGGL((fv_gradient_tile_kernel<4, double>), dim3(tiles), dim3(kTile),
                                lds, s, m, P, U, gphi, out, linear, sc);
