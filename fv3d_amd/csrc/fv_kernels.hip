@@ -453,7 +453,9 @@ __global__ __launch_bounds__(kTile, FVH_K1T_MINW) void fv_gradient_tile_kernel(
     // cells outside the launch range (in the first and last tile of a range
     // that does not start or end on a tile boundary) take part in the
     // barriers of the output passes without computing; the output passes
-    // store the rows of cells lo .. ncell - 1 of the tile only
+    // store the rows of cells lo .. ncell - 1 of the tile only.  Their
+    // states are staged like the others (on a connected mesh a cell of the
+    // range may have them as neighbours)
     const int c = base + threadIdx.x;
     const bool valid = c >= out.c_begin && c < out.c_end;
     const int cc = valid ? c : base;
