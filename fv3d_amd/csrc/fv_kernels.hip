@@ -652,8 +652,8 @@ __global__ __launch_bounds__(kTile, FVH_K1T_MINW) void fv_gradient_tile_kernel(
         __syncthreads();
         const double2 *src = reinterpret_cast<const double2 *>(smem);
         double2 *dst = reinterpret_cast<double2 *>(out.vg + (size_t)base * 12);
-        for (int i = lo * 6 + threadIdx.x; i < ncell * 6; i += kTile)
-            dst[i] = src[i];
+        for (int i = threadIdx.x; i < (ncell - lo) * 6; i += kTile)
+            dst[lo * 6 + i] = src[lo * 6 + i];
         __syncthreads();
     }
     if (!fs_out)
@@ -682,9 +682,9 @@ __global__ __launch_bounds__(kTile, FVH_K1T_MINW) void fv_gradient_tile_kernel(
             // 2 float4 per row, ns rows per cell
             const float4 *src = reinterpret_cast<const float4 *>(tl);
             const int per = 2 * ns;
-            for (int i = lo * per + threadIdx.x; i < ncell * per; i += kTile)
+            for (int i = threadIdx.x; i < (ncell - lo) * per; i += kTile)
             {
-                const int cl = i / per, q = i - cl * per;
+                const int cl = lo + i / per, q = i % per;
                 float4 *dst = reinterpret_cast<float4 *>(
                     out.fs32 + ((size_t)(base + cl) * m.maxf + s0) * 8);
                 dst[q] = src[cl * 4 + q];
@@ -695,9 +695,9 @@ __global__ __launch_bounds__(kTile, FVH_K1T_MINW) void fv_gradient_tile_kernel(
             // 4 double2 per row, ns rows per cell
             const double2 *src = reinterpret_cast<const double2 *>(tl);
             const int per = 4 * ns;
-            for (int i = lo * per + threadIdx.x; i < ncell * per; i += kTile)
+            for (int i = threadIdx.x; i < (ncell - lo) * per; i += kTile)
             {
-                const int cl = i / per, q = i - cl * per;
+                const int cl = lo + i / per, q = i % per;
                 double2 *dst = reinterpret_cast<double2 *>(
                     out.fs + ((size_t)(base + cl) * m.maxf + s0) * 8);
                 dst[q] = src[cl * 8 + q];
